@@ -1,0 +1,212 @@
+"""Host-side mirror of the reference's step interface over the C ABI.
+
+The reference drives its step from ``program main`` (t3dsdsc.f:136-202):
+``setdt`` -> dt-collapse check -> ``stepon`` -> ``nstep += 1; time += dt``.
+``MHDStep`` exposes the same operations (``compute_dt``, ``step``,
+``init_residual``, ``current``, ``diagnostics``) on the HIP implementation
+in ``libmhd_gpu.so`` through ``include/mhd_gpu.h``.  There is no CPU path:
+constructing an ``MHDStep`` without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmhd_gpu.so")
+
+MHD_OK = 0
+ERRORS = {-1: "bad argument", -2: "HIP error", -3: "out of device memory",
+          -4: "dt collapse (dt < 0.1*dt0)", -5: "current blow-up (|Jy| >= 1e7)"}
+
+# every symbol declared in include/mhd_gpu.h
+EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state",
+           "mhd_init_residual", "mhd_compute_dt", "mhd_step", "mhd_advance", "mhd_get_current",
+           "mhd_diagnostics", "mhd_get_divb", "mhd_rhs", "mhd_get_xdif", "mhd_get_residual",
+           "mhd_set_residual", "mhd_set_profiling", "mhd_kernel_times", "mhd_last_error",
+           "mhd_destroy")
+
+
+class MHDConfig(C.Structure):
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("periodic_y", C.c_int),
+                ("hall", C.c_int), ("whistler_dt", C.c_int), ("di", C.c_double),
+                ("eta0", C.c_double), ("fmu0", C.c_double), ("frho0", C.c_double),
+                ("fpr0", C.c_double), ("gamma", C.c_double), ("cfl", C.c_double)]
+
+
+class MHDDiag(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("cxmax", "cxmin", "cymax", "cymin", "czmax", "czmin",
+                                           "e_kin", "e_mag", "e_th", "e_mag_pert", "divb_max",
+                                           "divb_rms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class MHDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIBPATH):
+    """Load libmhd_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library {path} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(path)
+    P, D, I = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    L.mhd_create.argtypes = [C.POINTER(MHDConfig), D, D, D, D, D, D, C.POINTER(P)]
+    L.mhd_set_equilibrium.argtypes = [P, D, D, D, D, D]
+    for n in ("mhd_set_state", "mhd_get_state", "mhd_get_current", "mhd_get_divb",
+              "mhd_get_xdif", "mhd_get_residual", "mhd_set_residual"):
+        getattr(L, n).argtypes = [P, D]
+    L.mhd_init_residual.argtypes = [P]
+    L.mhd_compute_dt.argtypes = [P, D]
+    L.mhd_step.argtypes = [P, C.c_double, C.c_double]
+    L.mhd_advance.argtypes = [P, I, C.c_double, D, D, D]
+    L.mhd_diagnostics.argtypes = [P, C.POINTER(MHDDiag)]
+    L.mhd_rhs.argtypes = [P, C.c_double]
+    L.mhd_set_profiling.argtypes = [P, I]
+    L.mhd_kernel_times.argtypes = [P, I, C.POINTER(C.c_char_p), D, C.POINTER(I)]
+    L.mhd_last_error.argtypes = [P]
+    L.mhd_last_error.restype = C.c_char_p
+    L.mhd_destroy.argtypes = [P]
+    _lib = L
+    return L
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class MHDStep:
+    """One configured domain on the GPU: stepon/setdt of t3dsdsc.f."""
+
+    def __init__(self, case):
+        self.L = load_library()
+        self.case = case
+        cfg = MHDConfig(mx=case.mx, my=case.my, mz=case.mz, periodic_y=int(case.periodic_y),
+                        hall=int(case.hall), whistler_dt=int(case.whistler_dt), di=case.di,
+                        eta0=case.eta0, fmu0=case.fmu0, frho0=case.frho0, fpr0=case.fpr0,
+                        gamma=case.gamma, cfl=case.cfl)
+        geo = [np.ascontiguousarray(a, dtype=np.float64)
+               for a in (case.xx, case.yy, case.zz, case.dx, case.dy, case.dz)]
+        h = C.c_void_p()
+        rc = self.L.mhd_create(C.byref(cfg), *[_p(a) for a in geo], C.byref(h))
+        self.h = h
+        self._check(rc)
+        self.shape3 = (case.mz + 4, case.my, case.mx + 4)
+        eq = [np.ascontiguousarray(a, dtype=np.float64)
+              for a in (case.rhoi, case.bxi, case.byi, case.bzi, case.pri)]
+        self._check(self.L.mhd_set_equilibrium(self.h, *[_p(a) for a in eq]))
+        self.set_state(case.x0)
+
+    def _check(self, rc):
+        if rc != MHD_OK:
+            msg = self.L.mhd_last_error(self.h).decode() if self.h else ""
+            raise MHDError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _get(self, fn, nf):
+        out = np.empty((nf,) + self.shape3)
+        self._check(getattr(self.L, fn)(self.h, _p(out)))
+        return out
+
+    def set_state(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        assert a.shape == (8,) + self.shape3
+        self._check(self.L.mhd_set_state(self.h, _p(a)))
+
+    def get_state(self):
+        return self._get("mhd_get_state", 8)
+
+    def get_current(self):
+        return self._get("mhd_get_current", 3)
+
+    def get_divb(self):
+        return self._get("mhd_get_divb", 1)[0]
+
+    def get_xdif(self):
+        return self._get("mhd_get_xdif", 8)
+
+    def get_residual(self):
+        return self._get("mhd_get_residual", 3)
+
+    def set_residual(self, r):
+        a = np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self.L.mhd_set_residual(self.h, _p(a)))
+
+    def init_residual(self):
+        self._check(self.L.mhd_init_residual(self.h))
+
+    def compute_dt(self) -> float:
+        v = C.c_double()
+        self._check(self.L.mhd_compute_dt(self.h, C.byref(v)))
+        return v.value
+
+    def step(self, time: float, dt: float):
+        self._check(self.L.mhd_step(self.h, time, dt))
+
+    def rhs(self, t: float):
+        self._check(self.L.mhd_rhs(self.h, t))
+
+    def advance(self, nsteps: int, time: float = 0.0, dt0: float = 0.0):
+        """nsteps of setdt+stepon with dt on the device; returns
+        (time, last dt, elapsed ms on the handle's stream)."""
+        t = C.c_double(time)
+        dl = C.c_double()
+        ms = C.c_double()
+        self._check(self.L.mhd_advance(self.h, int(nsteps), dt0, C.byref(t), C.byref(dl),
+                                       C.byref(ms)))
+        return t.value, dl.value, ms.value
+
+    def diagnostics(self) -> dict:
+        d = MHDDiag()
+        rc = self.L.mhd_diagnostics(self.h, C.byref(d))
+        if rc not in (MHD_OK, -5):
+            self._check(rc)
+        out = d.as_dict()
+        out["status"] = rc
+        return out
+
+    def set_profiling(self, on: bool):
+        self._check(self.L.mhd_set_profiling(self.h, int(on)))
+
+    def kernel_times(self) -> dict:
+        n = 32
+        names = (C.c_char_p * n)()
+        ms = (C.c_double * n)()
+        calls = (C.c_int * n)()
+        k = self.L.mhd_kernel_times(self.h, n, names, ms, calls)
+        return {names[i].decode(): (ms[i], calls[i]) for i in range(k)}
+
+    def run(self, steps: int, *, time: float = 0.0, init: bool = True):
+        """host loop mirroring t3dsdsc.f:136-176 (setdt, stepon, time += dt)"""
+        if init:
+            self.init_residual()
+        dts = []
+        for _ in range(steps):
+            dt = self.compute_dt()
+            self.step(time, dt)
+            time += dt
+            dts.append(dt)
+        return time, dts

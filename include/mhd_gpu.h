@@ -1,0 +1,120 @@
+/*
+ * mhd_gpu.h -- C ABI of the MI355X (gfx950) RK4 Hall-MHD step.
+ *
+ * Replaces the argument-less Fortran subroutines of t3dsdsc.f that talk
+ * through COMMON blocks:
+ *   setdt   (t3dsdsc.f:4694-4810)  -> mhd_compute_dt
+ *   stepon  (t3dsdsc.f:1063-1185)  -> mhd_step
+ *   initia's xresd capture (t3dsdsc.f:816-834) -> mhd_init_residual
+ *   current (t3dsdsc.f:2452-2500)  -> mhd_get_current
+ *   diagn   (t3dsdsc.f:1189-1235), diverB (2504-2535) -> mhd_diagnostics
+ * and the data the reference keeps in COMMON:
+ *   /var21/ x (mhd02.for:4)                    -> mhd_set_state / mhd_get_state
+ *   /var11/ rhoi,bxi,bzi,pri, /wrk/ byi       -> mhd_set_equilibrium
+ *   /var15/ xx,yy,zz,dx,dy,dz (mhd01.for:70)   -> mhd_create arguments
+ *   /cst/ eta0,gamma,cfl (mhd01.for:55-56), /logic/ hall,periody -> mhd_config
+ *
+ * Layout of every host array: the reference's column-major padded layout,
+ * a(mx+4, my, mz+4[, m]) with x fastest (C order a[m][k][j][i]).  Physical
+ * points are Fortran indices 3..mx+2 in x and z; the two outer columns per
+ * side are never read.  y has no ghosts when non-periodic; when periodic the
+ * caller supplies my = ny + 4 rows (two ghost rows per side).
+ *
+ * Ownership: host pointers are borrowed for the call only.  The handle owns
+ * all device memory and one HIP stream; calls return after the work they
+ * started has finished, except mhd_step/mhd_advance which return once
+ * queued work is complete as well (synchronous API).
+ * Errors: int status, 0 = OK, < 0 = error; text via mhd_last_error.
+ * Threading: one host thread per handle.
+ */
+#ifndef MHD_GPU_H
+#define MHD_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhd_config {
+    int mx, my, mz;        /* mhd01.for:3-4 (my = full y extent)            */
+    int periodic_y;        /* mhd03.for:15 periody                          */
+    int hall;              /* mhd03.for:21 hall                             */
+    int whistler_dt;       /* add pi*cht*|B|/(rho*dx) to the CFL speeds     */
+    double di;             /* Hall coefficient cht (uniform)                */
+    double eta0;           /* mhd03.for:61 (foreta, t3dsdsc.f:6507)          */
+    double fmu0, frho0, fpr0; /* mhd03.for:62-64 (formu, t3dsdsc.f:6524-6528) */
+    double gamma;          /* mhd03.for:60                                  */
+    double cfl;            /* mhd03.for:71                                  */
+} mhd_config;
+
+typedef struct mhd_diag {
+    /* diagn (t3dsdsc.f:1189-1235): current extrema on the plane jy=my/2+1 */
+    double cxmax, cxmin, cymax, cymin, czmax, czmin;
+    /* energy integrals over unique physical points (trapezoid weights) */
+    double e_kin, e_mag, e_th, e_mag_pert;
+    /* diverB + bndrydiv (t3dsdsc.f:2504-2535, 3031-3712) */
+    double divb_max, divb_rms;
+} mhd_diag;
+
+typedef struct mhd_handle mhd_handle;
+
+enum {
+    MHD_OK = 0,
+    MHD_ERR_ARG = -1,
+    MHD_ERR_HIP = -2,
+    MHD_ERR_NOMEM = -3,
+    MHD_ERR_DT_COLLAPSE = -4,   /* t3dsdsc.f:142-154 */
+    MHD_ERR_BLOWUP = -5         /* t3dsdsc.f:1232    */
+};
+
+/* geometry arrays: xx, dx of length mx+4; yy, dy of length my; zz, dz of
+ * length mz+4 (input: gridpntnew2 + message1, t3dsdsc.f:248-313) */
+int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const double *zz,
+               const double *dx, const double *dy, const double *dz, mhd_handle **out);
+/* rhoi, bxi, byi, bzi, pri: each (mx+4, my, mz+4) (initia, t3dsdsc.f:806-814) */
+int mhd_set_equilibrium(mhd_handle *h, const double *rhoi, const double *bxi,
+                        const double *byi, const double *bzi, const double *pri);
+/* x(mx+4, my, mz+4, 8) (mhd02.for:1, /var21/) */
+int mhd_set_state(mhd_handle *h, const double *x);
+int mhd_get_state(mhd_handle *h, double *x);
+/* xresd = RHS momentum at t=0 with eta=mu=0 (t3dsdsc.f:816-834) */
+int mhd_init_residual(mhd_handle *h);
+/* setdt (t3dsdsc.f:4694-4810) */
+int mhd_compute_dt(mhd_handle *h, double *dt);
+/* stepon (t3dsdsc.f:1063-1165): one RK4 step of size dt from time */
+int mhd_step(mhd_handle *h, double time, double dt);
+/* the time loop body of program main (t3dsdsc.f:139-176) without I/O:
+ * nsteps x (setdt; stepon; time += dt) with dt kept on the device.
+ * dt0 > 0 arms the collapse guard dt < 0.1*dt0 (t3dsdsc.f:142).
+ * On return *time is advanced, *dt_last holds the last dt and
+ * *elapsed_ms the HIP-event time of the loop on the handle's stream. */
+int mhd_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_last,
+                double *elapsed_ms);
+/* current (t3dsdsc.f:2452-2500) of the current state, cur(mx+4,my,mz+4,3) */
+int mhd_get_current(mhd_handle *h, double *cur);
+/* diagn + energy + div B of the current state (returns MHD_ERR_BLOWUP
+ * when |max Jy| >= 1e7, t3dsdsc.f:1232, after filling *d) */
+int mhd_diagnostics(mhd_handle *h, mhd_diag *d);
+/* diverB + bndrydiv field of the current state, (mx+4, my, mz+4) */
+int mhd_get_divb(mhd_handle *h, double *divb);
+
+/* stage-level access for parity checks of the RHS (right, t3dsdsc.f:1297-1673):
+ * mhd_rhs runs one right(t) on the current state (t == 0 skips the final
+ * smoothing as t3dsdsc.f:1669 does); the getters read the arrays right()
+ * leaves behind: xdif (8 fields), efld and xdh (3 fields, smoothed). */
+int mhd_rhs(mhd_handle *h, double t);
+int mhd_get_xdif(mhd_handle *h, double *xdif);
+int mhd_get_residual(mhd_handle *h, double *xresd);
+int mhd_set_residual(mhd_handle *h, const double *xresd);
+
+/* profiling: per-kernel-family HIP-event times of the last mhd_advance
+ * when enabled (names are stable strings; returns the count written) */
+int mhd_set_profiling(mhd_handle *h, int on);
+int mhd_kernel_times(mhd_handle *h, int max, const char **names, double *ms, int *calls);
+
+const char *mhd_last_error(const mhd_handle *h);
+void mhd_destroy(mhd_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

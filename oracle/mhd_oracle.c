@@ -1,0 +1,962 @@
+/*
+ * mhd_oracle.c -- CPU ORACLE for the RK4 Hall-MHD step of t3dsdsc.f.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP step in
+ * dasha_amd/ is compared against.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product never links it.
+ *
+ * It restates, loop by loop, the reference's single-rank data flow in the
+ * reference's own padded layout: x(mxpr,my,mzpr,8) column-major with
+ * mxpr = mx+4, mzpr = mz+4 (two never-read ghost columns per side), 1-based
+ * indices, xfirst=3, xlast=mx+2, x_first=5, x_last=mx (t3dsdsc.f:61-79 with
+ * nprx=nprz=1), same for z; y has no ghosts (1..my), interior 3..my-2.
+ * Every routine cites the reference lines it follows.  Arithmetic order
+ * follows the Fortran expressions (left-to-right evaluation of + - * /) so
+ * that a build with -ffp-contract=off reproduces the reference's rounding
+ * wherever the libm functions (tanh, sqrt) agree.
+ *
+ * Harris-sheet configuration choices (the reference's own default case is a
+ * magnetosphere that needs input files it does not ship):
+ *   - no inner-sphere mask (rr1 > rcr everywhere, t3dsdsc.f:1696 never true);
+ *   - x-left boundary of dU/dt uses the reference's x-right rule mirrored
+ *     (f(3)=0, f(4)=f(5)/2) instead of the solar-wind inflow t3dsdsc.f:1711-1772;
+ *   - uniform Hall coefficient cht = d_i instead of the radial profile
+ *     t3dsdsc.f:1398-1399;
+ *   - periodic y uses two ghost rows on each side (f(1)=f(my-3),
+ *     f(2)=f(my-2), f(my-1)=f(3), f(my)=f(4)) over the whole extended x/z
+ *     range, because the reference's one-row copy (t3dsdsc.f:1813-1818) leaves
+ *     rows 2 and my-1 without an update while the 5-point stencils read them;
+ *     non-periodic y keeps the reference's rules and ranges exactly;
+ *   - optional whistler speed pi*cht*|B|/(rho*|dx|) added to each directional
+ *     signal speed in setdt (t3dsdsc.f:4719-4721 has none).
+ *
+ * Parity pins: see tests/test_oracle_*.py (closed forms, polynomial
+ * exactness, invariants, textbook RK4 composition, brute force).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    int mx, my, mz;          /* physical points in x and z; full y extent  */
+    int periodic_y;          /* 1: two-row periodic ghosts; 0: f2=0 rules   */
+    int hall;                /* Hall term on (t3dsdsc.f:1391)              */
+    int whistler_dt;         /* add whistler speed to the CFL signal speed */
+    int filters;             /* 1 normal; 0 disables every smoothing pass (operator tests) */
+    int bcs;                 /* 1 normal; 0 disables every boundary routine (operator tests) */
+    double di, eta0, fmu0, frho0, fpr0, gamma, cfl;
+} orc_config;
+
+typedef struct {
+    orc_config c;
+    int nx, ny, nz;          /* padded extents: mx+4, my, mz+4 */
+    size_t n3;
+    int xf, xl, x_f, x_l;    /* xfirst, xlast, x_first, x_last */
+    int zf, zl, z_f, z_l;
+    double *xx, *yy, *zz, *dx, *dy, *dz;
+    /* 1-D FD coefficients, t3dsdsc.f:336-606 (mhd01.for:34-49) */
+    double *ax1, *bx1, *cx1, *dx1, *ax2, *bx2, *cx2, *dx2;
+    double *ay1, *by1, *cy1, *dy1, *ay2, *by2, *cy2, *dy2;
+    double *az1, *bz1, *cz1, *dz1, *az2, *bz2, *cz2, *dz2;
+    double *axp, *bxp, *cxp, *axm, *bxm, *cxm;
+    double *ayp, *byp, *cyp, *aym, *bym, *cym;
+    double *azp, *bzp, *czp, *azm, *bzm, *czm;
+    /* state and work arrays (mhd02.for, mhd01.for:15-27) */
+    double *x, *xm, *xfold, *xdif;           /* 8 fields each */
+    double *rhoi, *bxi, *byi, *bzi, *pri;
+    double *xresd, *cur, *efld, *xdh;        /* 3 fields each */
+    double *fs, *gs, *hs, *w, *u, *v;
+    double etan, fmu, frho, fpr;             /* foreta/formu give constants */
+    double time;
+} orc;
+
+#define I3(jx, jy, jz) ((size_t)((jz) - 1) * (size_t)o->ny * (size_t)o->nx + \
+                        (size_t)((jy) - 1) * (size_t)o->nx + (size_t)((jx) - 1))
+#define F(a, jx, jy, jz) (a)[I3(jx, jy, jz)]
+#define FM(a, jx, jy, jz, m) (a)[(size_t)((m) - 1) * o->n3 + I3(jx, jy, jz)]
+#define V1(a, j) (a)[(j) - 1]
+
+/* t3dsdsc.f:1308-1313 statement functions d1fc / d2fc (same expression) */
+static inline double d1fc(double fm2, double fm, double f0, double fp, double fp2,
+                          double a, double b, double c, double d)
+{
+    return a * (fp - f0) + b * (f0 - fm) + c * (fp2 - f0) + d * (f0 - fm2);
+}
+
+/* t3dsdsc.f:5618-5619 statement function difc */
+static inline double difc(double fm, double f0, double fp, double xm1, double x0, double xp1)
+{
+    return 2. * ((fp - f0) * (xp1 - x0) - (f0 - fm) * (x0 - xm1)) / (xp1 - xm1);
+}
+
+static double *zalloc(size_t n)
+{
+    double *p = (double *)calloc(n ? n : 1, sizeof(double));
+    if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    return p;
+}
+
+/* ------------------------------------------------------------------ */
+/* FD coefficients, t3dsdsc.f:336-606                                  */
+/* ------------------------------------------------------------------ */
+static void d2coef(const double *c, int j, double *A, double *B, double *C, double *D)
+{
+    /* t3dsdsc.f:337-364 (x), 375-392 (y), 401-420 (z); c is 1-based coords */
+    double p1 = V1(c, j + 1) - V1(c, j), m1 = V1(c, j) - V1(c, j - 1);
+    double p2 = V1(c, j + 2) - V1(c, j), m2 = V1(c, j) - V1(c, j - 2);
+    double p1s = p1 * p1, m2s = m2 * m2, m1s = m1 * m1, p2s = p2 * p2;
+    double f1 = p1s + p1 * m2;
+    double f2 = p1s * p1 - p1 * m2s;
+    double f3 = p1s * p1s + p1 * (m2s * m2);
+    double g1 = -m1s + m1 * m2;
+    double g2 = m1s * m1 - m1 * m2s;
+    double g3 = -(m1s * m1s) + m1 * (m2s * m2);
+    double h1 = p2s + p2 * m2;
+    double h2 = p2s * p2 - p2 * m2s;
+    double h3 = p2s * p2s + p2 * (m2s * m2);
+    double ca = (f1 * h3 - f3 * h1) * (g2 * h3 - g3 * h2) - (g1 * h3 - g3 * h1) * (f2 * h3 - f3 * h2);
+    (void)g1;
+    *A = 2. * h3 * (g2 * h3 - g3 * h2) / ca;
+    *B = 2. * h3 * (h2 * f3 - h3 * f2) / ca;
+    *C = 2. * h3 * (f2 * g3 - f3 * g2) / ca;
+    *D = -(p1 * *A + m1 * *B + p2 * *C) / m2;
+}
+
+static void d1coef(const double *c, int j, double *A, double *B, double *C, double *D)
+{
+    /* t3dsdsc.f:440-459 (x), 461-481 (y), 483-504 (z) */
+    double p1 = V1(c, j + 1) - V1(c, j), m1 = V1(c, j) - V1(c, j - 1);
+    double p2 = V1(c, j + 2) - V1(c, j), m2 = V1(c, j) - V1(c, j - 2);
+    double p1s = p1 * p1, m1s = m1 * m1, p2s = p2 * p2, m2s = m2 * m2;
+    double p14 = p1s * p1s, m14 = m1s * m1s, p24 = p2s * p2s, m23 = m2s * m2;
+    double f1 = p1 + p14 / m23;
+    double f2 = p1s - p14 / m2s;
+    double f3 = p1s * p1 + p14 / m2;
+    double g1 = m1 - m14 / m23;
+    double g2 = -m1s + m14 / m2s;
+    double g3 = m1s * m1 - m14 / m2;
+    double h1 = p2 + p24 / m23;
+    double h2 = p2s - p24 / m2s;
+    double h3 = p2s * p2 + p24 / m2;
+    double ca = (f1 * h3 - f3 * h1) * (g2 * h3 - g3 * h2) - (g1 * h3 - g3 * h1) * (f2 * h3 - f3 * h2);
+    *A = h3 * (g2 * h3 - g3 * h2) / ca;
+    *B = h3 * (h2 * f3 - h3 * f2) / ca;
+    *C = h3 * (f2 * g3 - f3 * g2) / ca;
+    *D = (p1s * *A - m1s * *B + p2s * *C) / m2s;
+}
+
+static void onesided(const double *c, int j, int dir, double *A, double *B, double *C)
+{
+    /* t3dsdsc.f:505-530 (dir=+1, axp) and 531-543 (dir=-1, axm); y/z alike */
+    double d1 = dir * (V1(c, j + dir) - V1(c, j));
+    double d2 = dir * (V1(c, j + 2 * dir) - V1(c, j));
+    double d3 = dir * (V1(c, j + 3 * dir) - V1(c, j));
+    double d1s = d1 * d1, d2s = d2 * d2, d3s = d3 * d3;
+    double f1 = d1 - d1s * d1 / d3s;
+    double f2 = d1s - d1s * d1 / d3;
+    double g1 = d2 - d2s * d2 / d3s;
+    double g2 = d2s - d2s * d2 / d3;
+    double ca = f1 * g2 - f2 * g1;
+    *A = g2 / ca;
+    *B = -f2 / ca;
+    *C = (1 - *A * d1 - *B * d2) / d3;
+}
+
+static void build_coeffs(orc *o)
+{
+    int j;
+    for (j = o->x_f; j <= o->x_l; j++) {
+        d2coef(o->xx, j, &V1(o->ax2, j), &V1(o->bx2, j), &V1(o->cx2, j), &V1(o->dx2, j));
+        d1coef(o->xx, j, &V1(o->ax1, j), &V1(o->bx1, j), &V1(o->cx1, j), &V1(o->dx1, j));
+    }
+    for (j = 3; j <= o->ny - 2; j++) {
+        d2coef(o->yy, j, &V1(o->ay2, j), &V1(o->by2, j), &V1(o->cy2, j), &V1(o->dy2, j));
+        d1coef(o->yy, j, &V1(o->ay1, j), &V1(o->by1, j), &V1(o->cy1, j), &V1(o->dy1, j));
+    }
+    for (j = o->z_f; j <= o->z_l; j++) {
+        d2coef(o->zz, j, &V1(o->az2, j), &V1(o->bz2, j), &V1(o->cz2, j), &V1(o->dz2, j));
+        d1coef(o->zz, j, &V1(o->az1, j), &V1(o->bz1, j), &V1(o->cz1, j), &V1(o->dz1, j));
+    }
+    for (j = o->xf; j <= o->xl - 3; j++) onesided(o->xx, j, +1, &V1(o->axp, j), &V1(o->bxp, j), &V1(o->cxp, j));
+    for (j = o->xf + 3; j <= o->xl; j++) onesided(o->xx, j, -1, &V1(o->axm, j), &V1(o->bxm, j), &V1(o->cxm, j));
+    for (j = 1; j <= o->ny - 3; j++) onesided(o->yy, j, +1, &V1(o->ayp, j), &V1(o->byp, j), &V1(o->cyp, j));
+    for (j = 4; j <= o->ny; j++) onesided(o->yy, j, -1, &V1(o->aym, j), &V1(o->bym, j), &V1(o->cym, j));
+    for (j = o->zf; j <= o->zl - 3; j++) onesided(o->zz, j, +1, &V1(o->azp, j), &V1(o->bzp, j), &V1(o->czp, j));
+    for (j = o->zf + 3; j <= o->zl; j++) onesided(o->zz, j, -1, &V1(o->azm, j), &V1(o->bzm, j), &V1(o->czm, j));
+}
+
+/* ------------------------------------------------------------------ */
+/* Boundary routines                                                   */
+/* ------------------------------------------------------------------ */
+/* periodic y, two ghost rows, whole extended x/z range (replaces the
+ * one-row copy at t3dsdsc.f:1813-1818, 2122-2127, 2606-2611, 2774-2779,
+ * 2938-2943) */
+static void periodic_y(orc *o, double *a, int nf)
+{
+    int m, jx, jz, my = o->ny;
+    for (m = 1; m <= nf; m++)
+#pragma omp parallel for private(jx)
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                FM(a, jx, 1, jz, m) = FM(a, jx, my - 3, jz, m);
+                FM(a, jx, 2, jz, m) = FM(a, jx, my - 2, jz, m);
+                FM(a, jx, my - 1, jz, m) = FM(a, jx, 3, jz, m);
+                FM(a, jx, my, jz, m) = FM(a, jx, 4, jz, m);
+            }
+}
+
+/* bndryc / bndryh / bndryE: t3dsdsc.f:2539-2695, 2699-2867, 2871-3027.
+ * The three routines are the same code on cur / xdh / efld. */
+static void bndry_aux(orc *o, double *a)
+{
+    int m, jx, jy, jz, my = o->ny, xl = o->xl, zl = o->zl;
+    if (!o->c.bcs) return;
+    /* mask t3dsdsc.f:2549-2557: rr1 > rcr everywhere in the Harris setup */
+    /* x: t3dsdsc.f:2558-2577 */
+    for (m = 1; m <= 3; m++)
+#pragma omp parallel for private(jy)
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jy = 1; jy <= my; jy++) {
+                FM(a, 4, jy, jz, m) = (V1(o->axp, 4) * FM(a, 5, jy, jz, m) + V1(o->bxp, 4) * FM(a, 6, jy, jz, m) +
+                                       V1(o->cxp, 4) * FM(a, 7, jy, jz, m)) /
+                                      (V1(o->axp, 4) + V1(o->bxp, 4) + V1(o->cxp, 4));
+                FM(a, 3, jy, jz, m) = (V1(o->axp, 3) * FM(a, 4, jy, jz, m) + V1(o->bxp, 3) * FM(a, 5, jy, jz, m) +
+                                       V1(o->cxp, 3) * FM(a, 6, jy, jz, m)) /
+                                      (V1(o->axp, 3) + V1(o->bxp, 3) + V1(o->cxp, 3));
+                FM(a, xl, jy, jz, m) = 0.;
+                FM(a, xl - 1, jy, jz, m) = 0.5 * FM(a, xl - 2, jy, jz, m);
+            }
+    /* y: t3dsdsc.f:2592-2612 */
+    if (o->c.periodic_y) {
+        periodic_y(o, a, 3);
+    } else {
+        for (m = 1; m <= 3; m++)
+#pragma omp parallel for private(jx)
+            for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+                for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                    FM(a, jx, 1, jz, m) = 0;
+                    FM(a, jx, 2, jz, m) = 0.5 * FM(a, jx, 3, jz, m);
+                    FM(a, jx, my, jz, m) = 0;
+                    FM(a, jx, my - 1, jz, m) = 0.5 * FM(a, jx, my - 2, jz, m);
+                }
+    }
+    /* z: t3dsdsc.f:2614-2644 */
+    for (m = 1; m <= 3; m++)
+#pragma omp parallel for private(jx)
+        for (jy = 1; jy <= my; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                FM(a, jx, jy, 3, m) = 0;
+                FM(a, jx, jy, 4, m) = 0.5 * FM(a, jx, jy, 5, m);
+            }
+    for (m = 1; m <= 3; m++)
+#pragma omp parallel for private(jx)
+        for (jy = 1; jy <= my; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                FM(a, jx, jy, zl, m) = 0;
+                FM(a, jx, jy, zl - 1, m) = 0.5 * FM(a, jx, jy, zl - 2, m);
+            }
+    /* message4 (t3dsdsc.f:2692) moves nothing on a single domain */
+}
+
+/* bndry: t3dsdsc.f:1677-1985 (dU/dt boundaries), Harris x-left rule */
+static void bndry(orc *o)
+{
+    int m, jx, jy, jz, my = o->ny, xl = o->xl, zl = o->zl;
+    double *a = o->xdif;
+    if (!o->c.bcs) return;
+    /* mask 1692-1707 never active; message4 1708 moves nothing */
+    for (m = 1; m <= 8; m++)
+#pragma omp parallel for private(jy)
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jy = 1; jy <= my; jy++) {
+                /* x-left: x-right rule 1780-1781 mirrored */
+                FM(a, 3, jy, jz, m) = 0;
+                FM(a, 4, jy, jz, m) = 0.5 * FM(a, 5, jy, jz, m);
+                /* x-right 1775-1783 */
+                FM(a, xl, jy, jz, m) = 0;
+                FM(a, xl - 1, jy, jz, m) = 0.5 * FM(a, xl - 2, jy, jz, m);
+            }
+    /* y 1800-1819 */
+    if (o->c.periodic_y) {
+        periodic_y(o, a, 8);
+    } else {
+        for (m = 1; m <= 8; m++)
+#pragma omp parallel for private(jx)
+            for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+                for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                    FM(a, jx, 1, jz, m) = 0.;
+                    FM(a, jx, 2, jz, m) = 0.5 * FM(a, jx, 3, jz, m);
+                    FM(a, jx, my, jz, m) = 0.;
+                    FM(a, jx, my - 1, jz, m) = 0.5 * FM(a, jx, my - 2, jz, m);
+                }
+    }
+    /* z 1821-1861 */
+    for (m = 1; m <= 8; m++)
+#pragma omp parallel for private(jx)
+        for (jy = 1; jy <= my; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                FM(a, jx, jy, 3, m) = 0.;
+                FM(a, jx, jy, 4, m) = 0.5 * FM(a, jx, jy, 5, m);
+            }
+    for (m = 1; m <= 8; m++)
+#pragma omp parallel for private(jx)
+        for (jy = 1; jy <= my; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                FM(a, jx, jy, zl, m) = 0.;
+                FM(a, jx, jy, zl - 1, m) = 0.5 * FM(a, jx, jy, zl - 2, m);
+            }
+    /* limiter off (mhd03.for:22); message4 1978 moves nothing */
+}
+
+/* bndrytt: t3dsdsc.f:1989-2284 (post-step perturbation boundaries) */
+static void bndrytt(orc *o, int ms, int me)
+{
+    int m, jx, jy, jz, my = o->ny, xl = o->xl, zl = o->zl;
+    double *a = o->xm;
+    if (!o->c.bcs) return;
+    /* x 2019-2046 */
+    for (m = ms; m <= me; m++)
+#pragma omp parallel for private(jy)
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jy = 1; jy <= my; jy++) {
+                FM(a, 4, jy, jz, m) = (V1(o->axp, 4) * FM(a, 5, jy, jz, m) + V1(o->bxp, 4) * FM(a, 6, jy, jz, m) +
+                                       V1(o->cxp, 4) * FM(a, 7, jy, jz, m)) /
+                                      (V1(o->axp, 4) + V1(o->bxp, 4) + V1(o->cxp, 4));
+                FM(a, 3, jy, jz, m) = (V1(o->axp, 3) * FM(a, 4, jy, jz, m) + V1(o->bxp, 3) * FM(a, 5, jy, jz, m) +
+                                       V1(o->cxp, 3) * FM(a, 6, jy, jz, m)) /
+                                      (V1(o->axp, 3) + V1(o->bxp, 3) + V1(o->cxp, 3));
+                FM(a, xl, jy, jz, m) = 0;
+                FM(a, xl - 1, jy, jz, m) = 0.5 * FM(a, xl - 2, jy, jz, m);
+            }
+    /* y 2108-2128: non-periodic rule runs jx = x_first .. x_last+2 */
+    if (o->c.periodic_y) {
+        periodic_y(o, a, me);   /* fields ms..me: all callers use 1..8 */
+    } else {
+        for (m = ms; m <= me; m++)
+#pragma omp parallel for private(jx)
+            for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+                for (jx = o->x_f; jx <= o->x_l + 2; jx++) {
+                    FM(a, jx, 1, jz, m) = 0;
+                    FM(a, jx, 2, jz, m) = 0.5 * FM(a, jx, 3, jz, m);
+                    FM(a, jx, my, jz, m) = 0;
+                    FM(a, jx, my - 1, jz, m) = 0.5 * FM(a, jx, my - 2, jz, m);
+                }
+    }
+    /* z 2130-2168: jx = x_first .. x_last+2 */
+    for (m = ms; m <= me; m++)
+#pragma omp parallel for private(jx)
+        for (jy = 1; jy <= my; jy++)
+            for (jx = o->x_f; jx <= o->x_l + 2; jx++) {
+                FM(a, jx, jy, 3, m) = 0;
+                FM(a, jx, jy, 4, m) = 0.5 * FM(a, jx, jy, 5, m);
+            }
+    for (m = ms; m <= me; m++)
+#pragma omp parallel for private(jx)
+        for (jy = 1; jy <= my; jy++)
+            for (jx = o->x_f; jx <= o->x_l + 2; jx++) {
+                FM(a, jx, jy, zl, m) = 0;
+                FM(a, jx, jy, zl - 1, m) = 0.5 * FM(a, jx, jy, zl - 2, m);
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* Smoothing sweeps, t3dsdsc.f:5611-6054                               */
+/* ------------------------------------------------------------------ */
+enum { COEF_RADIAL = 0, COEF_BX = 1, COEF_EIGHTH = 2 };
+
+/* coefficient of one sweep at (jx,jy,jz) */
+static inline double filt_coef(const orc *o, int kind, int jx, int jy, int jz)
+{
+    if (kind == COEF_BX) {
+        /* t3dsdsc.f:5636-5648: (1-caf0)*(1/32)*(1-tanh((x+3)/8)), caf0 = 0 */
+        return (1 - 0.0) * (1.0 / 32.0) * (1 - tanh((V1(o->xx, jx) + 3) / 8.));
+    } else if (kind == COEF_RADIAL) {
+        /* t3dsdsc.f:5659-5661 (avrg), 5895-5897 (avrgE), 5981-5983 (avrgc) */
+        double r = sqrt(V1(o->xx, jx) * V1(o->xx, jx) + V1(o->yy, jy) * V1(o->yy, jy) +
+                        V1(o->zz, jz) * V1(o->zz, jz));
+        return 1.0 / 32.0 + (1.0 / 8.0 - 1.0 / 32.0) * 0.5 * (1 + tanh((r - 3.) / 3.));
+    }
+    return 1.0 / 8.0;   /* avrgh t3dsdsc.f:6031 */
+}
+
+/* one sweep of field m of array a (nf fields): w = f + c*sum(difc), then
+ * copy back on the interior (t3dsdsc.f:5630-5680) */
+static void filter_pass(orc *o, double *a, int m, int kind)
+{
+    int jx, jy, jz;
+#pragma omp parallel for private(jy, jx)
+    for (jz = o->z_f; jz <= o->z_l; jz++)
+        for (jy = 3; jy <= o->ny - 2; jy++)
+            for (jx = o->x_f; jx <= o->x_l; jx++) {
+                double c = filt_coef(o, kind, jx, jy, jz);
+                double s = difc(FM(a, jx - 1, jy, jz, m), FM(a, jx, jy, jz, m), FM(a, jx + 1, jy, jz, m),
+                                V1(o->xx, jx - 1), V1(o->xx, jx), V1(o->xx, jx + 1)) +
+                           difc(FM(a, jx, jy - 1, jz, m), FM(a, jx, jy, jz, m), FM(a, jx, jy + 1, jz, m),
+                                V1(o->yy, jy - 1), V1(o->yy, jy), V1(o->yy, jy + 1)) +
+                           difc(FM(a, jx, jy, jz - 1, m), FM(a, jx, jy, jz, m), FM(a, jx, jy, jz + 1, m),
+                                V1(o->zz, jz - 1), V1(o->zz, jz), V1(o->zz, jz + 1));
+                F(o->w, jx, jy, jz) = FM(a, jx, jy, jz, m) + c * s;
+            }
+#pragma omp parallel for private(jy, jx)
+    for (jz = o->z_f; jz <= o->z_l; jz++)
+        for (jy = 3; jy <= o->ny - 2; jy++)
+            for (jx = o->x_f; jx <= o->x_l; jx++) FM(a, jx, jy, jz, m) = F(o->w, jx, jy, jz);
+}
+
+/* avrgc (5952-6004), avrgE (5866-5918), avrgh (6008-6054): kk passes on
+ * three fields, each pass followed by the field's boundary routine */
+static void avrg_aux(orc *o, double *a, int kind, int kk)
+{
+    int k, m;
+    if (!o->c.filters) return;
+    for (k = 1; k <= kk; k++) {
+        for (m = 1; m <= 3; m++) filter_pass(o, a, m, kind);
+        bndry_aux(o, a);
+    }
+}
+
+/* avrg (5611-5720) on xdif and avrgtt (5754-5862) on xm: fields 5..7 use
+ * the x-profile coefficient, the others the radial one; the second block
+ * of both routines is skipped by `goto 27` */
+static void avrg_state(orc *o, double *a, int kk, int post)
+{
+    int k, m;
+    if (!o->c.filters) return;
+    for (k = 1; k <= kk; k++) {
+        for (m = 1; m <= 8; m++) filter_pass(o, a, m, (m >= 5 && m <= 7) ? COEF_BX : COEF_RADIAL);
+        if (post) bndrytt(o, 1, 8); else bndry(o);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* current, t3dsdsc.f:2452-2500                                        */
+/* ------------------------------------------------------------------ */
+static void current_raw(orc *o)
+{
+    int jx, jy, jz;
+#pragma omp parallel for private(jy, jx)
+    for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+        for (jy = 1; jy <= o->ny; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                F(o->u, jx, jy, jz) = FM(o->x, jx, jy, jz, 5) - F(o->bxi, jx, jy, jz);
+                F(o->v, jx, jy, jz) = FM(o->x, jx, jy, jz, 6) - F(o->byi, jx, jy, jz);
+                F(o->w, jx, jy, jz) = FM(o->x, jx, jy, jz, 7) - F(o->bzi, jx, jy, jz);
+            }
+#define DX(a) d1fc(F(a, jx - 2, jy, jz), F(a, jx - 1, jy, jz), F(a, jx, jy, jz), F(a, jx + 1, jy, jz), \
+                   F(a, jx + 2, jy, jz), V1(o->ax1, jx), V1(o->bx1, jx), V1(o->cx1, jx), V1(o->dx1, jx))
+#define DY(a) d1fc(F(a, jx, jy - 2, jz), F(a, jx, jy - 1, jz), F(a, jx, jy, jz), F(a, jx, jy + 1, jz), \
+                   F(a, jx, jy + 2, jz), V1(o->ay1, jy), V1(o->by1, jy), V1(o->cy1, jy), V1(o->dy1, jy))
+#define DZ(a) d1fc(F(a, jx, jy, jz - 2), F(a, jx, jy, jz - 1), F(a, jx, jy, jz), F(a, jx, jy, jz + 1), \
+                   F(a, jx, jy, jz + 2), V1(o->az1, jz), V1(o->bz1, jz), V1(o->cz1, jz), V1(o->dz1, jz))
+#pragma omp parallel for private(jy, jx)
+    for (jz = o->z_f; jz <= o->z_l; jz++)
+        for (jy = 3; jy <= o->ny - 2; jy++)
+            for (jx = o->x_f; jx <= o->x_l; jx++) {
+                FM(o->cur, jx, jy, jz, 1) = DY(o->w) - DZ(o->v);
+                FM(o->cur, jx, jy, jz, 2) = DZ(o->u) - DX(o->w);
+                FM(o->cur, jx, jy, jz, 3) = DX(o->v) - DY(o->u);
+            }
+}
+
+static void current(orc *o)
+{
+    current_raw(o);
+    bndry_aux(o, o->cur);               /* 2493 */
+    avrg_aux(o, o->cur, COEF_RADIAL, 4); /* 2495 */
+}
+
+/* derivative helpers on one field of an 8/3-field array */
+#define DXM(a, m) d1fc(FM(a, jx - 2, jy, jz, m), FM(a, jx - 1, jy, jz, m), FM(a, jx, jy, jz, m), FM(a, jx + 1, jy, jz, m), \
+                       FM(a, jx + 2, jy, jz, m), V1(o->ax1, jx), V1(o->bx1, jx), V1(o->cx1, jx), V1(o->dx1, jx))
+#define DYM(a, m) d1fc(FM(a, jx, jy - 2, jz, m), FM(a, jx, jy - 1, jz, m), FM(a, jx, jy, jz, m), FM(a, jx, jy + 1, jz, m), \
+                       FM(a, jx, jy + 2, jz, m), V1(o->ay1, jy), V1(o->by1, jy), V1(o->cy1, jy), V1(o->dy1, jy))
+#define DZM(a, m) d1fc(FM(a, jx, jy, jz - 2, m), FM(a, jx, jy, jz - 1, m), FM(a, jx, jy, jz, m), FM(a, jx, jy, jz + 1, m), \
+                       FM(a, jx, jy, jz + 2, m), V1(o->az1, jz), V1(o->bz1, jz), V1(o->cz1, jz), V1(o->dz1, jz))
+#define D2X(a) d1fc(F(a, jx - 2, jy, jz), F(a, jx - 1, jy, jz), F(a, jx, jy, jz), F(a, jx + 1, jy, jz), \
+                    F(a, jx + 2, jy, jz), V1(o->ax2, jx), V1(o->bx2, jx), V1(o->cx2, jx), V1(o->dx2, jx))
+#define D2Y(a) d1fc(F(a, jx, jy - 2, jz), F(a, jx, jy - 1, jz), F(a, jx, jy, jz), F(a, jx, jy + 1, jz), \
+                    F(a, jx, jy + 2, jz), V1(o->ay2, jy), V1(o->by2, jy), V1(o->cy2, jy), V1(o->dy2, jy))
+#define D2Z(a) d1fc(F(a, jx, jy, jz - 2), F(a, jx, jy, jz - 1), F(a, jx, jy, jz), F(a, jx, jy, jz + 1), \
+                    F(a, jx, jy, jz + 2), V1(o->az2, jz), V1(o->bz2, jz), V1(o->cz2, jz), V1(o->dz2, jz))
+#define D2XM(a, m) d1fc(FM(a, jx - 2, jy, jz, m), FM(a, jx - 1, jy, jz, m), FM(a, jx, jy, jz, m), FM(a, jx + 1, jy, jz, m), \
+                        FM(a, jx + 2, jy, jz, m), V1(o->ax2, jx), V1(o->bx2, jx), V1(o->cx2, jx), V1(o->dx2, jx))
+#define D2YM(a, m) d1fc(FM(a, jx, jy - 2, jz, m), FM(a, jx, jy - 1, jz, m), FM(a, jx, jy, jz, m), FM(a, jx, jy + 1, jz, m), \
+                        FM(a, jx, jy + 2, jz, m), V1(o->ay2, jy), V1(o->by2, jy), V1(o->cy2, jy), V1(o->dy2, jy))
+#define D2ZM(a, m) d1fc(FM(a, jx, jy, jz - 2, m), FM(a, jx, jy, jz - 1, m), FM(a, jx, jy, jz, m), FM(a, jx, jy, jz + 1, m), \
+                        FM(a, jx, jy, jz + 2, m), V1(o->az2, jz), V1(o->bz2, jz), V1(o->cz2, jz), V1(o->dz2, jz))
+
+/* ------------------------------------------------------------------ */
+/* right, t3dsdsc.f:1297-1673                                          */
+/* ------------------------------------------------------------------ */
+static void flux(orc *o, int m)
+{
+    /* t3dsdsc.f:2304-2347 (m = 1..4 only are reachable, 1334) */
+    int jx, jy, jz;
+    double *X = o->x;
+#pragma omp parallel for private(jy, jx)
+    for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+        for (jy = 1; jy <= o->ny; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                double x1 = FM(X, jx, jy, jz, 1), x2 = FM(X, jx, jy, jz, 2), x3 = FM(X, jx, jy, jz, 3);
+                double x4 = FM(X, jx, jy, jz, 4), x8 = FM(X, jx, jy, jz, 8), pi0 = F(o->pri, jx, jy, jz);
+                double f, g, h;
+                if (m == 1) { f = x2; g = x3; h = x4; }
+                else if (m == 2) { f = x2 * x2 / x1 + (x8 - pi0); g = x2 * x3 / x1; h = x2 * x4 / x1; }
+                else if (m == 3) { f = x2 * x3 / x1; g = x3 * x3 / x1 + (x8 - pi0); h = x4 * x3 / x1; }
+                else { f = x2 * x4 / x1; g = x3 * x4 / x1; h = x4 * x4 / x1 + (x8 - pi0); }
+                F(o->fs, jx, jy, jz) = f; F(o->gs, jx, jy, jz) = g; F(o->hs, jx, jy, jz) = h;
+            }
+}
+
+static void right(orc *o, double t, int init)
+{
+    int m, jx, jy, jz;
+    double *X = o->x, *XD = o->xdif, *C = o->cur, *E = o->efld;
+    double eta = init ? 0.0 : o->etan;          /* etan is zero before foreta (t3dsdsc.f:840) */
+    double fmu = init ? 0.0 : o->fmu, frho = init ? 0.0 : o->frho, fpr = init ? 0.0 : o->fpr;
+    double gam = o->c.gamma;
+
+    current(o);                                                     /* 1332 */
+    for (m = 1; m <= 4; m++) {                                      /* 1334-1346 */
+        flux(o, m);
+#pragma omp parallel for private(jy, jx)
+        for (jz = o->z_f; jz <= o->z_l; jz++)
+            for (jy = 3; jy <= o->ny - 2; jy++)
+                for (jx = o->x_f; jx <= o->x_l; jx++)
+                    FM(XD, jx, jy, jz, m) = -DX(o->fs) - DY(o->gs) - DZ(o->hs);
+    }
+    /* 1348-1387 (totalE = .true.) */
+#pragma omp parallel for private(jy, jx)
+    for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+        for (jy = 1; jy <= o->ny; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                double x1 = FM(X, jx, jy, jz, 1), x2 = FM(X, jx, jy, jz, 2), x3 = FM(X, jx, jy, jz, 3);
+                double x4 = FM(X, jx, jy, jz, 4), x5 = FM(X, jx, jy, jz, 5), x6 = FM(X, jx, jy, jz, 6);
+                double x7 = FM(X, jx, jy, jz, 7);
+                double c1 = FM(C, jx, jy, jz, 1), c2 = FM(C, jx, jy, jz, 2), c3 = FM(C, jx, jy, jz, 3);
+                F(o->fs, jx, jy, jz) = x2 / x1;
+                F(o->gs, jx, jy, jz) = x3 / x1;
+                F(o->hs, jx, jy, jz) = x4 / x1;
+                FM(XD, jx, jy, jz, 2) = FM(XD, jx, jy, jz, 2) - FM(o->xresd, jx, jy, jz, 1) + c2 * x7 - c3 * x6;
+                FM(XD, jx, jy, jz, 3) = FM(XD, jx, jy, jz, 3) - FM(o->xresd, jx, jy, jz, 2) + c3 * x5 - c1 * x7;
+                FM(XD, jx, jy, jz, 4) = FM(XD, jx, jy, jz, 4) - FM(o->xresd, jx, jy, jz, 3) + c1 * x6 - c2 * x5;
+                FM(E, jx, jy, jz, 1) = (-x3 * x7 + x4 * x6) / x1 + eta * c1;
+                FM(E, jx, jy, jz, 2) = (-x4 * x5 + x2 * x7) / x1 + eta * c2;
+                FM(E, jx, jy, jz, 3) = (-x2 * x6 + x3 * x5) / x1 + eta * c3;
+            }
+    avrg_aux(o, E, COEF_RADIAL, 4);                                 /* 1389 */
+
+    if (o->c.hall) {                                                /* 1391-1433 */
+        double cht = o->c.di;                                       /* uniform d_i */
+#pragma omp parallel for private(jy, jx)
+        for (jz = o->z_f; jz <= o->z_l; jz++)
+            for (jy = 3; jy <= o->ny - 2; jy++)
+                for (jx = o->x_f; jx <= o->x_l; jx++) {
+                    double x1 = FM(X, jx, jy, jz, 1), x5 = FM(X, jx, jy, jz, 5), x6 = FM(X, jx, jy, jz, 6);
+                    double x7 = FM(X, jx, jy, jz, 7);
+                    double c1 = FM(C, jx, jy, jz, 1), c2 = FM(C, jx, jy, jz, 2), c3 = FM(C, jx, jy, jz, 3);
+                    FM(o->xdh, jx, jy, jz, 1) = cht * (c2 * x7 - c3 * x6 - DXM(X, 8)) / x1;
+                    FM(o->xdh, jx, jy, jz, 2) = cht * (c3 * x5 - c1 * x7 - DYM(X, 8)) / x1;
+                    FM(o->xdh, jx, jy, jz, 3) = cht * (c1 * x6 - c2 * x5 - DZM(X, 8)) / x1;
+                }
+        bndry_aux(o, o->xdh);                                       /* 1423 */
+        avrg_aux(o, o->xdh, COEF_EIGHTH, 5);                        /* 1424 */
+        for (m = 1; m <= 3; m++)                                    /* 1426-1432 */
+#pragma omp parallel for private(jy, jx)
+            for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+                for (jy = 1; jy <= o->ny; jy++)
+                    for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++)
+                        FM(E, jx, jy, jz, m) = FM(E, jx, jy, jz, m) + FM(o->xdh, jx, jy, jz, m);
+    }
+
+    /* 1435-1566: -curl E and the pressure equation */
+#pragma omp parallel for private(jy, jx)
+    for (jz = o->z_f; jz <= o->z_l; jz++)
+        for (jy = 3; jy <= o->ny - 2; jy++)
+            for (jx = o->x_f; jx <= o->x_l; jx++) {
+                double c1 = FM(C, jx, jy, jz, 1), c2 = FM(C, jx, jy, jz, 2), c3 = FM(C, jx, jy, jz, 3);
+                FM(XD, jx, jy, jz, 5) = -DYM(E, 3) + DZM(E, 2);
+                FM(XD, jx, jy, jz, 6) = -DZM(E, 1) + DXM(E, 3);
+                FM(XD, jx, jy, jz, 7) = -DXM(E, 2) + DYM(E, 1);
+                FM(XD, jx, jy, jz, 8) = -DXM(X, 8) * F(o->fs, jx, jy, jz) - DYM(X, 8) * F(o->gs, jx, jy, jz) -
+                                        DZM(X, 8) * F(o->hs, jx, jy, jz) +
+                                        (-DX(o->fs) - DY(o->gs) - DZ(o->hs)) * gam * FM(X, jx, jy, jz, 8) +
+                                        (gam - 1.) * eta * (c1 * c1 + c2 * c2 + c3 * c3);
+            }
+    /* 1585-1621 viscosity, 1623-1663 diffusion (constant coefficients) */
+    if (fmu != 0.0) {
+#pragma omp parallel for private(jy, jx)
+        for (jz = o->z_f; jz <= o->z_l; jz++)
+            for (jy = 3; jy <= o->ny - 2; jy++)
+                for (jx = o->x_f; jx <= o->x_l; jx++) {
+                    double af = D2X(o->fs) + D2Y(o->fs) + D2Z(o->fs);
+                    double ag = D2X(o->gs) + D2Y(o->gs) + D2Z(o->gs);
+                    double ah = D2X(o->hs) + D2Y(o->hs) + D2Z(o->hs);
+                    FM(XD, jx, jy, jz, 2) = FM(XD, jx, jy, jz, 2) + fmu * af;
+                    FM(XD, jx, jy, jz, 3) = FM(XD, jx, jy, jz, 3) + fmu * ag;
+                    FM(XD, jx, jy, jz, 4) = FM(XD, jx, jy, jz, 4) + fmu * ah;
+                    FM(XD, jx, jy, jz, 8) = FM(XD, jx, jy, jz, 8) +
+                        fmu * (F(o->fs, jx, jy, jz) * af + F(o->gs, jx, jy, jz) * ag + F(o->hs, jx, jy, jz) * ah);
+                }
+    }
+    if (frho != 0.0 || fpr != 0.0) {
+#pragma omp parallel for private(jy, jx)
+        for (jz = o->z_f; jz <= o->z_l; jz++)
+            for (jy = 3; jy <= o->ny - 2; jy++)
+                for (jx = o->x_f; jx <= o->x_l; jx++) {
+                    FM(XD, jx, jy, jz, 1) = FM(XD, jx, jy, jz, 1) +
+                        frho * (D2XM(X, 1) + D2YM(X, 1) + D2ZM(X, 1));
+                    FM(XD, jx, jy, jz, 8) = FM(XD, jx, jy, jz, 8) +
+                        fpr * (D2XM(X, 8) + D2YM(X, 8) + D2ZM(X, 8));
+                }
+    }
+    bndry(o);                                                       /* 1665 */
+    if (t != 0.) avrg_state(o, XD, 2, 0);                           /* 1669 */
+}
+
+/* ------------------------------------------------------------------ */
+/* stepon, t3dsdsc.f:1063-1185                                         */
+/* ------------------------------------------------------------------ */
+#define EXT_LOOP(body) do {                                                   \
+    int jx_, jy_, jz_;                                                        \
+    _Pragma("omp parallel for private(jy_, jx_)")                             \
+    for (jz_ = o->z_f - 2; jz_ <= o->z_l + 2; jz_++)                          \
+        for (jy_ = 1; jy_ <= o->ny; jy_++)                                    \
+            for (jx_ = o->x_f - 2; jx_ <= o->x_l + 2; jx_++) {                \
+                size_t q = I3(jx_, jy_, jz_); body                            \
+            }                                                                 \
+} while (0)
+
+static void stepon(orc *o, double time, double dt)
+{
+    int m;
+    size_t n3 = o->n3;
+    double *X = o->x, *XM = o->xm, *XF = o->xfold, *XD = o->xdif;
+    for (m = 0; m < 8; m++) EXT_LOOP(XF[m * n3 + q] = X[m * n3 + q];);       /* 1079-1084 */
+    right(o, time + dt / 2, 0);                                             /* 1086-1089 */
+    for (m = 0; m < 8; m++) EXT_LOOP(                                        /* 1090-1097 */
+        XM[m * n3 + q] = XF[m * n3 + q] + XD[m * n3 + q] * dt / 6.;
+        X[m * n3 + q] = XF[m * n3 + q] + XD[m * n3 + q] * dt / 2.;);
+    right(o, time + dt / 2, 0);                                             /* 1102-1104 */
+    for (m = 0; m < 8; m++) EXT_LOOP(                                        /* 1105-1111 */
+        XM[m * n3 + q] = XM[m * n3 + q] + XD[m * n3 + q] * dt / 3.;
+        X[m * n3 + q] = XF[m * n3 + q] + XD[m * n3 + q] * dt / 2.;);
+    right(o, time + dt, 0);                                                 /* 1113-1115 */
+    for (m = 0; m < 8; m++) EXT_LOOP(                                        /* 1116-1122 */
+        XM[m * n3 + q] = XM[m * n3 + q] + XD[m * n3 + q] * dt / 3.;
+        X[m * n3 + q] = XF[m * n3 + q] + XD[m * n3 + q] * dt;);
+    right(o, time + dt, 0);                                                 /* 1124-1126 */
+    for (m = 0; m < 8; m++) EXT_LOOP(                                        /* 1127-1132 */
+        X[m * n3 + q] = XM[m * n3 + q] + XD[m * n3 + q] * dt / 6.;);
+    /* 1135-1165: filter the perturbation x - eq */
+    EXT_LOOP(
+        XM[0 * n3 + q] = X[0 * n3 + q] - o->rhoi[q];
+        XM[1 * n3 + q] = X[1 * n3 + q];
+        XM[2 * n3 + q] = X[2 * n3 + q];
+        XM[3 * n3 + q] = X[3 * n3 + q];
+        XM[4 * n3 + q] = X[4 * n3 + q] - o->bxi[q];
+        XM[5 * n3 + q] = X[5 * n3 + q] - o->byi[q];
+        XM[6 * n3 + q] = X[6 * n3 + q] - o->bzi[q];
+        XM[7 * n3 + q] = X[7 * n3 + q] - o->pri[q];);
+    avrg_state(o, XM, 2, 1);                                                /* 1150 */
+    EXT_LOOP(
+        X[0 * n3 + q] = XM[0 * n3 + q] + o->rhoi[q];
+        X[1 * n3 + q] = XM[1 * n3 + q];
+        X[2 * n3 + q] = XM[2 * n3 + q];
+        X[3 * n3 + q] = XM[3 * n3 + q];
+        X[4 * n3 + q] = XM[4 * n3 + q] + o->bxi[q];
+        X[5 * n3 + q] = XM[5 * n3 + q] + o->byi[q];
+        X[6 * n3 + q] = XM[6 * n3 + q] + o->bzi[q];
+        X[7 * n3 + q] = XM[7 * n3 + q] + o->pri[q];);
+    current(o);                                                             /* 1171 */
+}
+
+/* ------------------------------------------------------------------ */
+/* public API                                                          */
+/* ------------------------------------------------------------------ */
+orc *orc_create(const orc_config *cfg, const double *xx, const double *yy, const double *zz,
+                const double *dx, const double *dy, const double *dz)
+{
+    orc *o = (orc *)calloc(1, sizeof(orc));
+    o->c = *cfg;
+    o->nx = cfg->mx + 4; o->ny = cfg->my; o->nz = cfg->mz + 4;
+    o->n3 = (size_t)o->nx * o->ny * o->nz;
+    o->xf = 3; o->xl = cfg->mx + 2; o->x_f = 5; o->x_l = cfg->mx;
+    o->zf = 3; o->zl = cfg->mz + 2; o->z_f = 5; o->z_l = cfg->mz;
+    o->xx = zalloc(o->nx); o->yy = zalloc(o->ny); o->zz = zalloc(o->nz);
+    o->dx = zalloc(o->nx); o->dy = zalloc(o->ny); o->dz = zalloc(o->nz);
+    memcpy(o->xx, xx, sizeof(double) * o->nx); memcpy(o->yy, yy, sizeof(double) * o->ny);
+    memcpy(o->zz, zz, sizeof(double) * o->nz);
+    memcpy(o->dx, dx, sizeof(double) * o->nx); memcpy(o->dy, dy, sizeof(double) * o->ny);
+    memcpy(o->dz, dz, sizeof(double) * o->nz);
+#define AX(p) o->p = zalloc(o->nx)
+#define AY(p) o->p = zalloc(o->ny)
+#define AZ(p) o->p = zalloc(o->nz)
+    AX(ax1); AX(bx1); AX(cx1); AX(dx1); AX(ax2); AX(bx2); AX(cx2); AX(dx2);
+    AY(ay1); AY(by1); AY(cy1); AY(dy1); AY(ay2); AY(by2); AY(cy2); AY(dy2);
+    AZ(az1); AZ(bz1); AZ(cz1); AZ(dz1); AZ(az2); AZ(bz2); AZ(cz2); AZ(dz2);
+    AX(axp); AX(bxp); AX(cxp); AX(axm); AX(bxm); AX(cxm);
+    AY(ayp); AY(byp); AY(cyp); AY(aym); AY(bym); AY(cym);
+    AZ(azp); AZ(bzp); AZ(czp); AZ(azm); AZ(bzm); AZ(czm);
+    o->x = zalloc(8 * o->n3); o->xm = zalloc(8 * o->n3); o->xfold = zalloc(8 * o->n3); o->xdif = zalloc(8 * o->n3);
+    o->rhoi = zalloc(o->n3); o->bxi = zalloc(o->n3); o->byi = zalloc(o->n3); o->bzi = zalloc(o->n3); o->pri = zalloc(o->n3);
+    o->xresd = zalloc(3 * o->n3); o->cur = zalloc(3 * o->n3); o->efld = zalloc(3 * o->n3); o->xdh = zalloc(3 * o->n3);
+    o->fs = zalloc(o->n3); o->gs = zalloc(o->n3); o->hs = zalloc(o->n3);
+    o->w = zalloc(o->n3); o->u = zalloc(o->n3); o->v = zalloc(o->n3);
+    build_coeffs(o);
+    /* foreta (6489-6511) and formu (6515-6532): constants */
+    o->etan = cfg->eta0; o->fmu = cfg->fmu0; o->frho = cfg->frho0; o->fpr = cfg->fpr0;
+    return o;
+}
+
+void orc_destroy(orc *o)
+{
+    if (!o) return;
+    double **p[] = {&o->xx, &o->yy, &o->zz, &o->dx, &o->dy, &o->dz, &o->ax1, &o->bx1, &o->cx1, &o->dx1,
+                    &o->ax2, &o->bx2, &o->cx2, &o->dx2, &o->ay1, &o->by1, &o->cy1, &o->dy1, &o->ay2,
+                    &o->by2, &o->cy2, &o->dy2, &o->az1, &o->bz1, &o->cz1, &o->dz1, &o->az2, &o->bz2,
+                    &o->cz2, &o->dz2, &o->axp, &o->bxp, &o->cxp, &o->axm, &o->bxm, &o->cxm, &o->ayp,
+                    &o->byp, &o->cyp, &o->aym, &o->bym, &o->cym, &o->azp, &o->bzp, &o->czp, &o->azm,
+                    &o->bzm, &o->czm, &o->x, &o->xm, &o->xfold, &o->xdif, &o->rhoi, &o->bxi, &o->byi,
+                    &o->bzi, &o->pri, &o->xresd, &o->cur, &o->efld, &o->xdh, &o->fs, &o->gs, &o->hs,
+                    &o->w, &o->u, &o->v};
+    for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(*p[i]);
+    free(o);
+}
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_equilibrium(orc *o, const double *rhoi, const double *bxi, const double *byi,
+                         const double *bzi, const double *pri)
+{
+    size_t b = sizeof(double) * o->n3;
+    memcpy(o->rhoi, rhoi, b); memcpy(o->bxi, bxi, b); memcpy(o->byi, byi, b);
+    memcpy(o->bzi, bzi, b); memcpy(o->pri, pri, b);
+}
+
+void orc_set_state(orc *o, const double *x) { memcpy(o->x, x, sizeof(double) * 8 * o->n3); }
+void orc_get_state(const orc *o, double *x) { memcpy(x, o->x, sizeof(double) * 8 * o->n3); }
+void orc_get_current(const orc *o, double *c) { memcpy(c, o->cur, sizeof(double) * 3 * o->n3); }
+void orc_get_xdif(const orc *o, double *c) { memcpy(c, o->xdif, sizeof(double) * 8 * o->n3); }
+void orc_get_efld(const orc *o, double *c) { memcpy(c, o->efld, sizeof(double) * 3 * o->n3); }
+void orc_get_xdh(const orc *o, double *c) { memcpy(c, o->xdh, sizeof(double) * 3 * o->n3); }
+void orc_get_residual(const orc *o, double *c) { memcpy(c, o->xresd, sizeof(double) * 3 * o->n3); }
+void orc_set_residual(orc *o, const double *c) { memcpy(o->xresd, c, sizeof(double) * 3 * o->n3); }
+
+/* coefficient export for the tests: which = 0..11 selects
+ * {ax1,bx1,cx1,dx1, ax2,bx2,cx2,dx2, axp,bxp,cxp, axm,bxm,cxm} for axis 0/1/2 */
+int orc_get_coeffs(const orc *o, int axis, double *out /* 14 * n */)
+{
+    int n = axis == 0 ? o->nx : axis == 1 ? o->ny : o->nz;
+    const double *src[3][14] = {
+        {o->ax1, o->bx1, o->cx1, o->dx1, o->ax2, o->bx2, o->cx2, o->dx2, o->axp, o->bxp, o->cxp, o->axm, o->bxm, o->cxm},
+        {o->ay1, o->by1, o->cy1, o->dy1, o->ay2, o->by2, o->cy2, o->dy2, o->ayp, o->byp, o->cyp, o->aym, o->bym, o->cym},
+        {o->az1, o->bz1, o->cz1, o->dz1, o->az2, o->bz2, o->cz2, o->dz2, o->azp, o->bzp, o->czp, o->azm, o->bzm, o->czm}};
+    for (int k = 0; k < 14; k++) memcpy(out + (size_t)k * n, src[axis][k], sizeof(double) * n);
+    return n;
+}
+
+/* initia t3dsdsc.f:816-834: xfold = x, right(0, 1e-9, 1) with eta = mu = 0,
+ * xresd = xdif(2..4) on the extended range */
+void orc_init_residual(orc *o)
+{
+    size_t n3 = o->n3;
+    int m;
+    memset(o->xresd, 0, sizeof(double) * 3 * n3);
+    for (m = 0; m < 8; m++) EXT_LOOP(o->xfold[m * n3 + q] = o->x[m * n3 + q];);
+    right(o, 0.0, 1);
+    for (m = 0; m < 3; m++) EXT_LOOP(o->xresd[m * n3 + q] = o->xdif[(m + 1) * n3 + q];);
+}
+
+/* one call of right(t) for the tests (t = 0 skips the final avrg as in 1669) */
+void orc_rhs(orc *o, double t) { right(o, t, 0); }
+void orc_current(orc *o) { current(o); }
+void orc_current_raw(orc *o) { current_raw(o); }
+void orc_step(orc *o, double time, double dt) { stepon(o, time, dt); }
+
+/* apply one boundary routine or sweep to a caller array (tests):
+ * kind 0 bndry(xdif), 1 bndrytt(xm), 2 bndry_aux(cur), 3 filter (nf fields
+ * of `a` with coefficient kind `ck`), a holds nf fields of n3 */
+void orc_apply(orc *o, int kind, double *a, int nf, int ck)
+{
+    size_t n3 = o->n3;
+    if (kind == 0) { memcpy(o->xdif, a, 8 * n3 * sizeof(double)); bndry(o); memcpy(a, o->xdif, 8 * n3 * sizeof(double)); }
+    else if (kind == 1) { memcpy(o->xm, a, 8 * n3 * sizeof(double)); bndrytt(o, 1, 8); memcpy(a, o->xm, 8 * n3 * sizeof(double)); }
+    else if (kind == 2) { bndry_aux(o, a); }
+    else { for (int m = 1; m <= nf; m++) filter_pass(o, a, m, ck); }
+}
+
+/* setdt t3dsdsc.f:4694-4810 (+ optional whistler speed) */
+double orc_compute_dt(const orc *o)
+{
+    double dtt = 100.;
+    double eta = o->etan, fmu = o->fmu, cfl = o->c.cfl, gam = o->c.gamma;
+    double cht = o->c.hall ? o->c.di : 0.0;
+    int jz;
+#pragma omp parallel for reduction(min : dtt)
+    for (jz = o->zf; jz <= o->z_l; jz++)
+        for (int jy = 1; jy <= o->ny - 1; jy++)
+            for (int jx = o->xf; jx <= o->x_l; jx++) {
+                double x1 = FM(o->x, jx, jy, jz, 1);
+                double vx = FM(o->x, jx, jy, jz, 2) / x1;
+                double vy = FM(o->x, jx, jy, jz, 3) / x1;
+                double vz = FM(o->x, jx, jy, jz, 4) / x1;
+                double b2 = FM(o->x, jx, jy, jz, 5) * FM(o->x, jx, jy, jz, 5) +
+                            FM(o->x, jx, jy, jz, 6) * FM(o->x, jx, jy, jz, 6) +
+                            FM(o->x, jx, jy, jz, 7) * FM(o->x, jx, jy, jz, 7);
+                double va2 = b2 / x1;
+                double cs2 = gam * FM(o->x, jx, jy, jz, 8) / x1;
+                double vf = sqrt(fabs(va2 + cs2));
+                double vpx = fabs(vx) + vf, vpy = fabs(vy) + vf, vpz = fabs(vz) + vf;
+                double ddx = V1(o->dx, jx), ddy = V1(o->dy, jy), ddz = V1(o->dz, jz);
+                if (o->c.whistler_dt && cht != 0.0) {
+                    double wb = 3.141592654 * cht * sqrt(b2) / x1;
+                    vpx = vpx + wb / fabs(ddx);
+                    vpy = vpy + wb / fabs(ddy);
+                    vpz = vpz + wb / fabs(ddz);
+                }
+                double dtx = fabs(ddx) / (vpx / cfl + (eta + fmu) / ddx / 2.);
+                double dty = fabs(ddy) / (vpy / cfl + (eta + fmu) / ddy / 2.);
+                double dtz = fabs(ddz) / (vpz / cfl + (eta + fmu) / ddz / 2.);
+                double dt1 = dtx < dty ? dtx : dty;
+                double dt2 = dtz < dt1 ? dtz : dt1;
+                if (dt2 < dtt) dtt = dt2;
+            }
+    return dtt;
+}
+
+/* diagn + funmax t3dsdsc.f:1189-1235, 6114-6147: extrema of the three
+ * current components on the plane jy = my/2+1 over jx,jz = 2..n-1 of the
+ * padded arrays (ghost columns 2 and n-1 hold 0 on a single domain).
+ * out = {cxmax, cxmin, cymax, cymin, czmax, czmin}; recomputes current. */
+void orc_diagn(orc *o, double *out)
+{
+    int jy = o->ny / 2 + 1;
+    current(o);
+    for (int m = 1; m <= 3; m++) {
+        double fmax = -1000000., fmin = 1000000.;
+        for (int jz = 2; jz <= o->nz - 1; jz++)
+            for (int jx = 2; jx <= o->nx - 1; jx++) {
+                double f = FM(o->cur, jx, jy, jz, m);
+                if (f > fmax) fmax = f;
+                if (f < fmin) fmin = f;
+            }
+        out[2 * (m - 1)] = fmax;
+        out[2 * (m - 1) + 1] = fmin;
+    }
+}
+
+/* diverB + bndrydiv t3dsdsc.f:2504-2535, 3031-3712 on the current state.
+ * The periodic-y branch uses the two-row copy.  Writes divB (n3). */
+void orc_divb(const orc *o, double *dv)
+{
+    int jx, jy, jz, my = o->ny, xl = o->xl, zl = o->zl;
+    memset(dv, 0, sizeof(double) * o->n3);
+    for (jz = o->z_f; jz <= o->z_l; jz++)
+        for (jy = 3; jy <= my - 2; jy++)
+            for (jx = o->x_f; jx <= o->x_l; jx++)
+                F(dv, jx, jy, jz) = DXM(o->x, 5) + DYM(o->x, 6) + DZM(o->x, 7);
+    for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+        for (jy = 1; jy <= my; jy++) {
+            F(dv, 4, jy, jz) = (V1(o->axp, 4) * F(dv, 5, jy, jz) + V1(o->bxp, 4) * F(dv, 6, jy, jz) +
+                                V1(o->cxp, 4) * F(dv, 7, jy, jz)) / (V1(o->axp, 4) + V1(o->bxp, 4) + V1(o->cxp, 4));
+            F(dv, 3, jy, jz) = (V1(o->axp, 3) * F(dv, 4, jy, jz) + V1(o->bxp, 3) * F(dv, 5, jy, jz) +
+                                V1(o->cxp, 3) * F(dv, 6, jy, jz)) / (V1(o->axp, 3) + V1(o->bxp, 3) + V1(o->cxp, 3));
+            F(dv, xl - 1, jy, jz) = (V1(o->axm, xl - 1) * F(dv, xl - 2, jy, jz) + V1(o->bxm, xl - 1) * F(dv, xl - 3, jy, jz) +
+                                     V1(o->cxm, xl - 1) * F(dv, xl - 4, jy, jz)) /
+                                    (V1(o->axm, xl - 1) + V1(o->bxm, xl - 1) + V1(o->cxm, xl - 1));
+            F(dv, xl, jy, jz) = (V1(o->axm, xl) * F(dv, xl - 1, jy, jz) + V1(o->bxm, xl) * F(dv, xl - 2, jy, jz) +
+                                 V1(o->cxm, xl) * F(dv, xl - 3, jy, jz)) / (V1(o->axm, xl) + V1(o->bxm, xl) + V1(o->cxm, xl));
+        }
+    if (o->c.periodic_y) {
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                F(dv, jx, 1, jz) = F(dv, jx, my - 3, jz);
+                F(dv, jx, 2, jz) = F(dv, jx, my - 2, jz);
+                F(dv, jx, my - 1, jz) = F(dv, jx, 3, jz);
+                F(dv, jx, my, jz) = F(dv, jx, 4, jz);
+            }
+    } else {
+        double aa1 = 5. / 2., bb1 = -2., cc1 = 1. / 2.;
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                F(dv, jx, 2, jz) = aa1 * F(dv, jx, 3, jz) + bb1 * F(dv, jx, 4, jz) + cc1 * F(dv, jx, 5, jz);
+                F(dv, jx, my - 1, jz) = aa1 * F(dv, jx, my - 2, jz) + bb1 * F(dv, jx, my - 3, jz) + cc1 * F(dv, jx, my - 4, jz);
+                F(dv, jx, 1, jz) = aa1 * F(dv, jx, 2, jz) + bb1 * F(dv, jx, 3, jz) + cc1 * F(dv, jx, 4, jz);
+                F(dv, jx, my, jz) = aa1 * F(dv, jx, my - 1, jz) + bb1 * F(dv, jx, my - 2, jz) + cc1 * F(dv, jx, my - 3, jz);
+            }
+    }
+    {
+        double aa1 = 5. / 2., bb1 = -2., cc1 = 1. / 2.;
+        for (jy = 1; jy <= my; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                F(dv, jx, jy, 4) = aa1 * F(dv, jx, jy, 5) + bb1 * F(dv, jx, jy, 6) + cc1 * F(dv, jx, jy, 7);
+                F(dv, jx, jy, 3) = aa1 * F(dv, jx, jy, 4) + bb1 * F(dv, jx, jy, 5) + cc1 * F(dv, jx, jy, 6);
+            }
+        for (jy = 1; jy <= my; jy++)
+            for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                F(dv, jx, jy, zl - 1) = aa1 * F(dv, jx, jy, zl - 2) + bb1 * F(dv, jx, jy, zl - 3) + cc1 * F(dv, jx, jy, zl - 4);
+                F(dv, jx, jy, zl) = aa1 * F(dv, jx, jy, zl - 1) + bb1 * F(dv, jx, jy, zl - 2) + cc1 * F(dv, jx, jy, zl - 3);
+            }
+    }
+}
+
+/* energy integrals (the reference only names `call energy`, t3dsdsc.f:844):
+ * out = {kinetic, magnetic, thermal, magnetic-perturbation}, each
+ * sum over unique physical points of the density times the trapezoid
+ * volume weight wx*wy*wz; y rows 3..my-2 when periodic (weight dy), 1..my
+ * otherwise (trapezoid). Summed in k, j, i order. */
+void orc_energy(const orc *o, double *out)
+{
+    double ek = 0, em = 0, et = 0, ep = 0;
+    int jy0 = o->c.periodic_y ? 3 : 1, jy1 = o->c.periodic_y ? o->ny - 2 : o->ny;
+    double gm1 = o->c.gamma - 1.0;
+    for (int jz = o->zf; jz <= o->zl; jz++) {
+        double wz = (jz == o->zf) ? 0.5 * (V1(o->zz, jz + 1) - V1(o->zz, jz))
+                  : (jz == o->zl) ? 0.5 * (V1(o->zz, jz) - V1(o->zz, jz - 1))
+                  : 0.5 * (V1(o->zz, jz + 1) - V1(o->zz, jz - 1));
+        for (int jy = jy0; jy <= jy1; jy++) {
+            double wy;
+            if (o->c.periodic_y) wy = V1(o->yy, jy + 1) - V1(o->yy, jy);
+            else wy = (jy == 1) ? 0.5 * (V1(o->yy, 2) - V1(o->yy, 1))
+                    : (jy == o->ny) ? 0.5 * (V1(o->yy, jy) - V1(o->yy, jy - 1))
+                    : 0.5 * (V1(o->yy, jy + 1) - V1(o->yy, jy - 1));
+            for (int jx = o->xf; jx <= o->xl; jx++) {
+                double wx = (jx == o->xf) ? 0.5 * (V1(o->xx, jx + 1) - V1(o->xx, jx))
+                          : (jx == o->xl) ? 0.5 * (V1(o->xx, jx) - V1(o->xx, jx - 1))
+                          : 0.5 * (V1(o->xx, jx + 1) - V1(o->xx, jx - 1));
+                double wv = wx * wy * wz;
+                double x1 = FM(o->x, jx, jy, jz, 1), x2 = FM(o->x, jx, jy, jz, 2), x3 = FM(o->x, jx, jy, jz, 3);
+                double x4 = FM(o->x, jx, jy, jz, 4), x5 = FM(o->x, jx, jy, jz, 5), x6 = FM(o->x, jx, jy, jz, 6);
+                double x7 = FM(o->x, jx, jy, jz, 7), x8 = FM(o->x, jx, jy, jz, 8);
+                double d5 = x5 - F(o->bxi, jx, jy, jz), d6 = x6 - F(o->byi, jx, jy, jz), d7 = x7 - F(o->bzi, jx, jy, jz);
+                ek += 0.5 * (x2 * x2 + x3 * x3 + x4 * x4) / x1 * wv;
+                em += 0.5 * (x5 * x5 + x6 * x6 + x7 * x7) * wv;
+                et += x8 / gm1 * wv;
+                ep += 0.5 * (d5 * d5 + d6 * d6 + d7 * d7) * wv;
+            }
+        }
+    }
+    out[0] = ek; out[1] = em; out[2] = et; out[3] = ep;
+}

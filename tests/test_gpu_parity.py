@@ -1,0 +1,137 @@
+"""Parity of the HIP step (through the C ABI) against the CPU oracle.
+
+Every test builds the same Harris case for both, runs the same sequence of
+calls and compares arrays.  Tolerances: the two sides evaluate the same
+expressions in the same order without FMA contraction; they differ only
+where the device tanh/sqrt differ from glibc by an ulp (radial smoothing
+coefficient), so single calls agree to ~1e-15 and the north-star bound is
+1e-10 relative L2 per field after 100 steps.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from dasha_amd import harris
+from tests.util import field_errors, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = 1e-12      # one operator call / one step
+NORTH = 1e-10      # relative L2 per field after 100 steps
+
+
+def _pair(case, **okw):
+    from oracle.oracle import Oracle
+    from dasha_amd.mhd import MHDStep
+    print(f"[parity] case={case.name} mx={case.mx} my={case.my} mz={case.mz} hall={case.hall} "
+          f"periodic_y={case.periodic_y}", flush=True)
+    return MHDStep(case), Oracle(case, **okw)
+
+
+CASES = {
+    "stretched_periodic_hall": lambda: harris.case_small(24, 8, 24),
+    "uniform_periodic_hall": lambda: harris.case_small(20, 8, 28, stretched=False, seed=11),
+    "stretched_nonperiodic_nohall": lambda: harris.case_small(24, 12, 20, periodic_y=False, di=0.0,
+                                                               seed=5),
+    "guide_field_visc": lambda: _visc_case(),
+}
+
+
+def _visc_case():
+    c = harris.case_small(22, 10, 24, guide=0.1, seed=9, di=0.2)
+    c.fmu0, c.frho0, c.fpr0 = 1e-3, 2e-3, 3e-3
+    return c
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_current_matches(name):
+    g, o = _pair(CASES[name]())
+    o.current()
+    errs = field_errors(g.get_current(), o.get_current())
+    print("current errs", errs)
+    assert max(errs) < TIGHT
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_residual_and_rhs_match(name):
+    g, o = _pair(CASES[name]())
+    g.init_residual()
+    o.init_residual()
+    e = field_errors(g.get_residual(), o.get_residual())
+    print("xresd errs", e)
+    assert max(e) < TIGHT
+    g.rhs(0.01)
+    o.rhs(0.01)
+    e = field_errors(g.get_xdif(), o.get_xdif())
+    print("xdif errs", e)
+    assert max(e) < TIGHT
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_dt_matches(name):
+    g, o = _pair(CASES[name]())
+    dg, do = g.compute_dt(), o.compute_dt()
+    assert abs(dg - do) <= 1e-14 * do
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_one_and_five_steps(name):
+    g, o = _pair(CASES[name]())
+    tg, dg = g.run(1)
+    to, do = o.run(1)
+    e = field_errors(g.get_state(), o.get_state())
+    print("1 step errs", e)
+    assert max(e) < TIGHT
+    tg, dg = g.run(4, time=tg, init=False)
+    to, do = o.run(4, time=to, init=False)
+    e = field_errors(g.get_state(), o.get_state())
+    print("5 step errs", e, dg, do)
+    assert max(e) < 1e-11
+    assert np.allclose(dg, do, rtol=1e-12, atol=0)
+
+
+def test_advance_matches_host_loop():
+    """device-resident dt loop (mhd_advance) == host setdt/stepon loop"""
+    c = harris.case_small(24, 8, 24, seed=3)
+    from dasha_amd.mhd import MHDStep
+    a, b = MHDStep(c), MHDStep(c)
+    a.init_residual(); b.init_residual()
+    ta, dla = 0.0, None
+    ta, dla, ms = a.advance(6, 0.0)
+    tb, dts = b.run(6, init=False)
+    assert abs(ta - tb) < 1e-14 and dla == dts[-1]
+    assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_diagnostics_match():
+    c = harris.case_small(24, 8, 24, seed=4)
+    g, o = _pair(c)
+    g.run(2); o.run(2)
+    d = g.diagnostics()
+    od = o.diagn()
+    oe = o.energy()
+    got = [d["cxmax"], d["cxmin"], d["cymax"], d["cymin"], d["czmax"], d["czmin"]]
+    assert np.allclose(got, od, rtol=1e-12, atol=1e-15)
+    assert np.allclose([d["e_kin"], d["e_mag"], d["e_th"], d["e_mag_pert"]], oe, rtol=1e-12)
+    dv_g, dv_o = g.get_divb(), o.divb()
+    assert rel_l2(dv_g, dv_o) < 1e-12
+    print("diag", d)
+
+
+def test_c1_100_steps_north_star():
+    """C1 (64x32x64 uniform Harris, eta 0.005, d_i 0.1): 100 RK4 steps,
+    relative L2 per field <= 1e-10, div B drift reported."""
+    from oracle import oracle as orc_mod
+    c = harris.case_c1()
+    g, o = _pair(c)
+    d0 = g.diagnostics()
+    tg, dg = g.run(100)
+    to, do = o.run(100)
+    e = field_errors(g.get_state(), o.get_state())
+    d1 = g.diagnostics()
+    print(f"C1 100 steps: time {tg:.6f}  rel L2 per field {['%.2e' % v for v in e]}", flush=True)
+    print(f"C1 div B max {d0['divb_max']:.3e} -> {d1['divb_max']:.3e}, rms {d0['divb_rms']:.3e} -> "
+          f"{d1['divb_rms']:.3e}; oracle threads {orc_mod.max_threads()}", flush=True)
+    assert max(e) <= NORTH
+    assert np.allclose(dg, do, rtol=1e-10, atol=0)

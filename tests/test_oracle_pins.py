@@ -1,0 +1,429 @@
+"""Pins of the CPU oracle (oracle/mhd_oracle.c) against mathematics.
+
+The reference ships no tests, fixtures or golden vectors, and its source does
+not compile with the only Fortran compiler in the image, so the oracle is
+checked against facts fixed independently of its code:
+
+* FD coefficients: closed forms on uniform meshes and the unique
+  polynomial-exact (Vandermonde/Fornberg) weights on stretched meshes;
+* smoothing sweep: the uniform-mesh second difference;
+* boundary rules: their defining properties (zero one-sided slope of a
+  cubic, 0 / half rules, periodic images, corner precedence);
+* raw right-hand side: symbolic (sympy) derivatives of polynomial fields;
+* composition of smoothing sweeps and pass counts: exact Fourier symbols of
+  a linear resistive/Hall mode far from the origin;
+* RK4 register scheme: textbook RK4 built from the oracle's own RHS calls;
+* whole step: exact equilibrium invariance, z-mirror symmetry,
+  y-translation invariance;
+* setdt: closed form on a uniform state and its index ranges.
+"""
+import math
+
+import numpy as np
+import pytest
+import sympy as sp
+
+from dasha_amd import harris
+
+orc = pytest.importorskip("oracle.oracle")
+
+
+def mk(case, **kw):
+    orc.build()
+    return orc.Oracle(case, **kw)
+
+
+# ---------------------------------------------------------------- coefficients
+def fd_weights(offsets, deriv):
+    """unique weights w with sum w_l * d_l**p / p! = delta(p, deriv)"""
+    n = len(offsets)
+    A = np.array([[d ** p / math.factorial(p) for d in offsets] for p in range(n)])
+    b = np.zeros(n)
+    b[deriv] = 1.0
+    return np.linalg.solve(A, b)
+
+
+def test_uniform_closed_forms():
+    c = harris.case_small(24, 8, 24, stretched=False, lx=8.0, lz=8.0)
+    o = mk(c)
+    h = 8.0 / 23
+    co = o.coeffs(0)
+    i = slice(5, 21)   # padded 0-based interior 4..mx-1 holds jx=5..mx
+    ax1, bx1, cx1, dx1, ax2, bx2, cx2, dx2, axp, bxp, cxp = (co[k][4:c.mx] for k in range(11))
+    np.testing.assert_allclose(ax1, 2 / (3 * h), rtol=1e-11)
+    np.testing.assert_allclose(bx1, 2 / (3 * h), rtol=1e-11)
+    np.testing.assert_allclose(cx1, -1 / (12 * h), rtol=1e-11)
+    np.testing.assert_allclose(dx1, -1 / (12 * h), rtol=1e-11)
+    np.testing.assert_allclose(ax2, 4 / (3 * h * h), rtol=1e-10)
+    np.testing.assert_allclose(bx2, -4 / (3 * h * h), rtol=1e-10)
+    np.testing.assert_allclose(cx2, -1 / (12 * h * h), rtol=1e-10)
+    np.testing.assert_allclose(dx2, 1 / (12 * h * h), rtol=1e-10)
+    # one-sided: axp = 3/h, bxp = -3/(2h), cxp = 1/(3h) at jx = 3, 4
+    np.testing.assert_allclose(co[8][2:4], 3 / h, rtol=1e-11)
+    np.testing.assert_allclose(co[9][2:4], -3 / (2 * h), rtol=1e-11)
+    np.testing.assert_allclose(co[10][2:4], 1 / (3 * h), rtol=1e-11)
+    del i
+
+
+@pytest.mark.parametrize("axis", [0, 2])
+def test_stretched_weights_are_polynomial_exact(axis):
+    c = harris.case_small(24, 8, 28, stretched=True, dmin=0.15)
+    o = mk(c)
+    co = o.coeffs(axis)
+    coord = c.xx if axis == 0 else c.zz
+    n = c.mx if axis == 0 else c.mz
+    for jx in range(5, n + 1):          # Fortran interior index
+        j = jx - 1
+        off = [coord[j + s] - coord[j] for s in (-2, -1, 0, 1, 2)]
+        for deriv, base in ((1, 0), (2, 4)):
+            a, b, cc, d = (co[base + t][j] for t in range(4))
+            # a(f1-f0)+b(f0-f-1)+c(f2-f0)+d(f0-f-2) as stencil weights
+            w = np.array([-d, -b, -a + b - cc + d, a, cc])
+            ref = fd_weights(off, deriv)
+            np.testing.assert_allclose(w, ref, rtol=1e-8, atol=1e-8 * np.abs(ref).max())
+    # one-sided third-order derivative weights (axp, axm)
+    for jx in range(3, n - 1):
+        j = jx - 1
+        off = [coord[j + s] - coord[j] for s in (0, 1, 2, 3)]
+        ref = fd_weights(off, 1)
+        a, b, cc = co[8][j], co[9][j], co[10][j]
+        np.testing.assert_allclose([-(a + b + cc), a, b, cc], ref, rtol=1e-8,
+                                   atol=1e-8 * np.abs(ref).max())
+    for jx in range(6, n + 3):
+        j = jx - 1
+        off = [coord[j - s] - coord[j] for s in (0, 1, 2, 3)]
+        ref = fd_weights(off, 1)
+        a, b, cc = co[11][j], co[12][j], co[13][j]
+        # axm form: f'(x) = -(axm(f-1 - f0) + ...): signs follow t3dsdsc.f:531-543
+        np.testing.assert_allclose([(a + b + cc), -a, -b, -cc], ref, rtol=1e-8,
+                                   atol=1e-8 * np.abs(ref).max())
+
+
+# ---------------------------------------------------------------- smoothing
+def test_sweep_is_second_difference_on_uniform_mesh():
+    c = harris.case_small(20, 8, 20, stretched=False)
+    o = mk(c)
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((3,) + o.shape3)
+    b = o.apply(3, a, nf=3, coef_kind=2)        # coefficient 1/8 (avrgh)
+    s = np.s_[4:-4, 2:-2, 4:-4]
+    lap = (a[:, 5:-3, 2:-2, 4:-4] + a[:, 3:-5, 2:-2, 4:-4] + a[:, 4:-4, 3:-1, 4:-4] + a[:, 4:-4, 1:-3, 4:-4]
+           + a[:, 4:-4, 2:-2, 5:-3] + a[:, 4:-4, 2:-2, 3:-5] - 6 * a[:, 4:-4, 2:-2, 4:-4])
+    np.testing.assert_allclose(b[(slice(None),) + s], a[(slice(None),) + s] + lap / 8, rtol=0, atol=1e-12)
+    # the x-profile coefficient (1/32)(1 - tanh((x+3)/8)) on the B components
+    b8 = o.apply(3, rng.standard_normal((8,) + o.shape3), nf=8, coef_kind=1)
+    assert b8.shape[0] == 8
+
+
+# ---------------------------------------------------------------- boundaries
+def _rand(o, nf, seed=1):
+    return np.random.default_rng(seed).standard_normal((nf,) + o.shape3)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_state_boundary_rules(periodic):
+    c = harris.case_small(20, 12, 20, periodic_y=periodic)
+    o = mk(c)
+    a = _rand(o, 8)
+    b = o.apply(0, a)
+    mx, my, mz = c.mx, c.my, c.mz
+    X = lambda jx: jx - 1
+    # z planes (applied last, all jx of the extended range, all jy)
+    assert np.all(b[:, X(3), :, X(3):X(mx + 3)] == 0)
+    np.testing.assert_array_equal(b[:, X(4), :, X(3):X(mx + 3)], 0.5 * b[:, X(5), :, X(3):X(mx + 3)])
+    np.testing.assert_array_equal(b[:, X(mz + 1), :, X(3):X(mx + 3)], 0.5 * b[:, X(mz), :, X(3):X(mx + 3)])
+    assert np.all(b[:, X(mz + 2), :, X(3):X(mx + 3)] == 0)
+    zi = slice(X(5), X(mz + 1))
+    if periodic:
+        xe = slice(X(3), X(mx + 3))
+        np.testing.assert_array_equal(b[:, zi, 0, xe], b[:, zi, my - 4, xe])
+        np.testing.assert_array_equal(b[:, zi, 1, xe], b[:, zi, my - 3, xe])
+        np.testing.assert_array_equal(b[:, zi, my - 2, xe], b[:, zi, 2, xe])
+        np.testing.assert_array_equal(b[:, zi, my - 1, xe], b[:, zi, 3, xe])
+    else:
+        assert np.all(b[:, zi, 0, X(3):X(mx + 3)] == 0)
+        np.testing.assert_array_equal(b[:, zi, 1, X(3):X(mx + 3)], 0.5 * b[:, zi, 2, X(3):X(mx + 3)])
+    yi = slice(2, my - 2)
+    # x rules on interior rows: mirrored x-right rule on the left
+    assert np.all(b[:, zi, yi, X(3)] == 0)
+    np.testing.assert_array_equal(b[:, zi, yi, X(4)], 0.5 * a[:, zi, yi, X(5)])
+    assert np.all(b[:, zi, yi, X(mx + 2)] == 0)
+    np.testing.assert_array_equal(b[:, zi, yi, X(mx + 1)], 0.5 * a[:, zi, yi, X(mx)])
+    # interior untouched, ghost columns untouched
+    np.testing.assert_array_equal(b[:, zi, yi, X(5):X(mx + 1)], a[:, zi, yi, X(5):X(mx + 1)])
+    np.testing.assert_array_equal(b[:, :, :, :2], a[:, :, :, :2])
+
+
+def _cubic_flat_at(x, x3, x4):
+    """cubic with zero slope at x3 and x4"""
+    return 0.7 + 2 * (x - x3) ** 3 - 3 * (x4 - x3) * (x - x3) ** 2
+
+
+@pytest.mark.parametrize("stretched", [False, True])
+def test_aux_boundary_extrapolation_recovers_flat_cubic(stretched):
+    c = harris.case_small(24, 8, 24, stretched=stretched)
+    o = mk(c)
+    x3, x4 = c.xx[2], c.xx[3]
+    prof = _cubic_flat_at(c.xx, x3, x4)
+    a = np.broadcast_to(prof, (3,) + o.shape3).copy()
+    truth = a.copy()
+    a[:, :, :, 2:4] = 123.0          # garbage at jx = 3, 4
+    b = o.apply(2, a)
+    zi, yi = slice(4, c.mz), slice(2, c.my - 2)
+    np.testing.assert_allclose(b[:, zi, yi, 2:4], truth[:, zi, yi, 2:4], rtol=1e-11, atol=1e-11)
+
+
+def test_post_step_boundary_corner_precedence():
+    """bndrytt, non-periodic y: the y and z rules skip jx = 3, 4, so the x-left
+    corner keeps the x extrapolation of the pre-fill face values
+    (t3dsdsc.f:2108-2168 loop from x_first)"""
+    c = harris.case_small(20, 12, 20, periodic_y=False)
+    o = mk(c)
+    a = _rand(o, 8, seed=5)
+    b = o.apply(1, a)
+    co = o.coeffs(0)
+    ax4, bx4, cx4 = co[8][3], co[9][3], co[10][3]
+    ax3, bx3, cx3 = co[8][2], co[9][2], co[10][2]
+    for (kz, jy) in [(2, 0), (2, 5), (3, 1), (c.mz + 1, c.my - 1), (7, 0)]:
+        f = a[:, kz, jy, :]
+        v4 = (ax4 * f[:, 4] + bx4 * f[:, 5] + cx4 * f[:, 6]) / (ax4 + bx4 + cx4)
+        v3 = (ax3 * v4 + bx3 * f[:, 4] + cx3 * f[:, 5]) / (ax3 + bx3 + cx3)
+        np.testing.assert_array_equal(b[:, kz, jy, 3], v4)
+        np.testing.assert_array_equal(b[:, kz, jy, 2], v3)
+    # away from jx = 3, 4 the z rule wins on the z planes
+    assert np.all(b[:, 2, :, 4:c.mx + 2] == 0)
+
+
+# ---------------------------------------------------------------- raw RHS
+def test_raw_rhs_matches_symbolic_derivatives():
+    """filters and boundary routines off: right() on polynomial fields equals
+    the analytic RHS (the 5-point weights are exact for degree <= 4)"""
+    X, Y, Z = sp.symbols("x y z")
+    rho = sp.Integer(2)
+    v = [sp.Rational(1, 10) + sp.Rational(2, 10) * Y, sp.Rational(3, 10) * Z - sp.Rational(1, 10) * X,
+         sp.Rational(2, 10) * X + sp.Rational(1, 10) * Y]
+    B = [1 + sp.Rational(1, 10) * Z ** 2 + sp.Rational(1, 20) * Y, sp.Rational(1, 10) * X * Z + sp.Rational(3, 10),
+         sp.Rational(3, 10) * Y - sp.Rational(1, 10) * X ** 2]
+    Bi = [sp.Rational(1, 2), sp.Integer(0), sp.Rational(1, 5)]
+    p = 1 + sp.Rational(1, 10) * X ** 2 + sp.Rational(1, 20) * Y * Z
+    pri = sp.Rational(4, 5)
+    eta, cht, gam = 0.01, 0.3, 5.0 / 3.0
+    grad = lambda f: [sp.diff(f, s) for s in (X, Y, Z)]
+    curl = lambda F: [sp.diff(F[2], Y) - sp.diff(F[1], Z), sp.diff(F[0], Z) - sp.diff(F[2], X),
+                      sp.diff(F[1], X) - sp.diff(F[0], Y)]
+    cross = lambda a, b: [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+    J = curl([B[i] - Bi[i] for i in range(3)])
+    m = [rho * vi for vi in v]
+    divv = sum(sp.diff(v[i], s) for i, s in enumerate((X, Y, Z)))
+    r1 = -sum(sp.diff(m[i], s) for i, s in enumerate((X, Y, Z)))
+    JxB = cross(J, B)
+    mom = []
+    for a in range(3):
+        fl = [rho * v[a] * v[b] + ((p - pri) if a == b else 0) for b in range(3)]
+        mom.append(-sum(sp.diff(fl[b], s) for b, s in enumerate((X, Y, Z))) + JxB[a])
+    gp = grad(p)
+    Et = [-cross(v, B)[i] + eta * J[i] + cht * (JxB[i] - gp[i]) / rho for i in range(3)]
+    ind = [-c_ for c_ in curl(Et)]
+    pres = -sum(v[i] * gp[i] for i in range(3)) - gam * p * divv + (gam - 1) * eta * sum(j * j for j in J)
+    exprs = [r1] + mom + ind + [pres]
+    fns = [sp.lambdify((X, Y, Z), e, "numpy") for e in exprs]
+
+    c = harris.case_small(24, 12, 24, stretched=True, dmin=0.25, di=cht, eta0=eta)
+    shape = (c.mz + 4, c.my, c.mx + 4)
+    xg = np.broadcast_to(c.xx[None, None, :], shape)
+    yg = np.broadcast_to(c.yy[None, :, None], shape)
+    zg = np.broadcast_to(c.zz[:, None, None], shape)
+    ev = lambda f: np.broadcast_to(sp.lambdify((X, Y, Z), f, "numpy")(xg, yg, zg), shape).astype(float)
+    x0 = np.zeros((8,) + shape)
+    x0[0] = ev(rho)
+    for i in range(3):
+        x0[1 + i] = ev(m[i])
+        x0[4 + i] = ev(B[i])
+    x0[7] = ev(p)
+    c.x0 = x0
+    c.rhoi = ev(rho).copy()
+    c.bxi, c.byi, c.bzi = (ev(b).copy() for b in Bi)
+    c.pri = ev(pri).copy()
+    o = mk(c, filters=False, bcs=False)
+    o.rhs(0.5)
+    xd = o.get_xdif()
+    # compare where every input of the chain was computed by a stencil
+    # (current/xdh are only stencil-computed on the interior)
+    s = np.s_[8:c.mz - 4, 6:c.my - 6, 8:c.mx - 4]
+    for k in range(8):
+        want = np.broadcast_to(fns[k](xg, yg, zg), shape)[s]
+        got = xd[k][s]
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9, err_msg=f"field {k + 1}")
+
+
+# ---------------------------------------------------------------- composition
+def _mode_case(mz=64, ny=8):
+    c = harris.harris_case("mode", 48, ny, mz, stretched=False, lx=20.0, lz=16.0, ly=4.0,
+                           x_offset=300.0, eta0=0.01, di=0.2, amp=0.0, noise=0.0)
+    return c
+
+
+def test_fourier_symbols_pin_sweep_counts():
+    """far from the origin every radial coefficient is exactly 1/8 and the
+    B-component one exactly 0; a z-mode dBy = eps cos(kz) on a uniform state
+    with background Bz0 must give
+      xdif(6) = -eta sigma^2 F^8 dBy      (avrgc x4, avrgE x4)
+      xdif(5) = -(cht Bz0/rho) sigma^2 F^9 eps cos(kz)   (avrgc x4, avrgh x5)
+    with sigma the 5-point derivative symbol and F = 1 + (2cos kh - 2)/8."""
+    c = _mode_case()
+    shape = (c.mz + 4, c.my, c.mx + 4)
+    h = 16.0 / (c.mz - 1)
+    kw = 2 * math.pi / 16.0 * 3
+    eps, bz0, rho0, p0 = 1e-9, 1.3, 1.7, 0.9
+    Z = np.broadcast_to(c.zz[:, None, None], shape)
+    x0 = np.zeros((8,) + shape)
+    x0[0] = rho0
+    x0[5] = eps * np.cos(kw * Z)
+    x0[6] = bz0
+    x0[7] = p0
+    c.x0 = x0
+    c.rhoi = np.full(shape, rho0)
+    c.bxi = np.zeros(shape); c.byi = np.zeros(shape); c.bzi = np.full(shape, bz0)
+    c.pri = np.full(shape, p0)
+    o = mk(c)
+    o.rhs(0.5)
+    xd = o.get_xdif()
+    sig = (4.0 / 3.0 * math.sin(kw * h) - 1.0 / 6.0 * math.sin(2 * kw * h)) / h
+    F = 1 + (2 * math.cos(kw * h) - 2) / 8.0
+    # boundary rules reach ~15 cells inwards: compare the centre only
+    s = np.s_[c.mz // 2 - 3:c.mz // 2 + 4, 3:c.my - 3, c.mx // 2 - 3:c.mx // 2 + 4]
+    cz = np.cos(kw * Z)[s]
+    np.testing.assert_allclose(xd[5][s], -c.eta0 * sig ** 2 * F ** 8 * eps * cz, rtol=1e-6, atol=1e-22)
+    np.testing.assert_allclose(xd[4][s], -(c.di * bz0 / rho0) * sig ** 2 * F ** 9 * eps * cz, rtol=1e-6,
+                               atol=1e-22)
+    # a wrong sweep count would change F's power: the check is sensitive
+    assert abs(F ** 8 / F ** 9 - 1) > 1e-2
+
+
+def _rk4_reference(o, x0, dt):
+    """textbook RK4 from four calls of the oracle's own right()"""
+    ks = []
+    for a in (0.0, 0.5, 0.5, 1.0):
+        xs = x0 if not ks else x0 + a * dt * ks[-1]
+        o.set_state(xs)
+        o.rhs(1.0)
+        ks.append(o.get_xdif())
+    return x0 + dt / 6.0 * (ks[0] + 2 * ks[1] + 2 * ks[2] + ks[3])
+
+
+def test_rk4_register_scheme_equals_textbook_rk4():
+    c = harris.case_small(20, 8, 20, seed=21)
+    a = mk(c, filters=True)
+    b = mk(c, filters=True)
+    a.init_residual()
+    b.init_residual()
+    dt = a.compute_dt()
+    a.step(0.0, dt)
+    got = a.get_state()
+    x1 = _rk4_reference(b, c.x0.copy(), dt)
+    # post-step: xm = x - eq; 2 x (sweep on 8 fields + bndrytt); x = xm + eq
+    eq = np.stack([c.rhoi, np.zeros_like(c.rhoi), np.zeros_like(c.rhoi), np.zeros_like(c.rhoi),
+                   c.bxi, c.byi, c.bzi, c.pri])
+    xm = x1 - eq
+    for _ in range(2):
+        xm_f = xm.copy()
+        for m in range(8):
+            xm_f[m] = b.apply(3, xm, nf=8, coef_kind=1 if 4 <= m <= 6 else 0)[m]
+        xm = b.apply(1, xm_f)
+    want = xm + eq
+    s = np.s_[:, 2:-2, :, 2:-2]
+    err = np.abs(got[s] - want[s]).max() / np.abs(want[s]).max()
+    assert err < 1e-13, err
+
+
+# ---------------------------------------------------------------- invariants
+def test_equilibrium_is_an_exact_fixed_point():
+    """J = curl(B - B_i), p - p_i in the momentum flux and the frozen residual
+    make an unperturbed equilibrium an exact fixed point (Hall off: the Hall
+    term uses the full pressure gradient, t3dsdsc.f:1409, which the x
+    boundary rules turn into a nonzero curl near the x faces)."""
+    c = harris.case_small(24, 8, 24, amp=0.0, noise=0.0, guide=0.1, di=0.0)
+    o = mk(c)
+    o.run(3)
+    np.testing.assert_array_equal(o.get_state(), c.x0)
+    # with Hall on the only change comes from the x faces inwards
+    c2 = harris.case_small(24, 8, 24, amp=0.0, noise=0.0, guide=0.1, di=0.2)
+    o2 = mk(c2)
+    o2.rhs(0.01)
+    xd = o2.get_xdif()
+    assert np.abs(xd[[0, 1, 2, 3, 7]]).max() == 0.0
+    # (the left faces extrapolate a constant: rounding only)
+    assert np.abs(xd[5][:, :, 2:c2.mx // 2 - 6]).max() < 1e-15
+    assert np.abs(xd[5]).max() > 1e-3
+
+
+def test_z_mirror_symmetry():
+    """reflection z -> -z: v polar (vz odd), B pseudovector (Bx, By odd);
+    holds with Hall on (quadrupolar By) on the symmetric stretched mesh"""
+    c = harris.case_small(24, 8, 24, noise=0.0, amp=0.1, guide=0.0)
+    o = mk(c)
+    o.run(3)
+    x = o.get_state()[:, 2:-2]
+    assert np.abs(x[5]).max() > 1e-6       # Hall has made a By to test
+    par = [1, 1, 1, -1, -1, -1, 1, 1]      # rho, mx, my, mz, Bx, By, Bz, p
+    for m in range(8):
+        a, b = x[m], par[m] * x[m][::-1]
+        scale = np.abs(a).max() + 1e-300
+        assert np.abs(a - b).max() / scale < 1e-11, m
+
+
+def test_y_translation_invariance():
+    c = harris.harris_case("far", 20, 8, 20, stretched=False, lx=10.0, lz=10.0, ly=4.0,
+                           x_offset=300.0, eta0=0.01, di=0.2, seed=4, noise=1e-3)
+    o1 = mk(c)
+    o1.run(2)
+    r1 = o1.get_state()
+    c2 = harris.harris_case("far", 20, 8, 20, stretched=False, lx=10.0, lz=10.0, ly=4.0,
+                            x_offset=300.0, eta0=0.01, di=0.2, seed=4, noise=1e-3)
+    ny, sh = 8, 3
+    core = c2.x0[:, :, 2:2 + ny]
+    rolled = np.roll(core, sh, axis=2)
+    c2.x0[:, :, 2:2 + ny] = rolled
+    c2.x0[:, :, 0:2] = rolled[:, :, ny - 2:ny]
+    c2.x0[:, :, ny + 2:ny + 4] = rolled[:, :, 0:2]
+    o2 = mk(c2)
+    o2.run(2)
+    r2 = o2.get_state()
+    a = np.roll(r1[:, :, 2:2 + ny], sh, axis=2)
+    b = r2[:, :, 2:2 + ny]
+    for m in range(8):
+        assert np.abs(a[m] - b[m]).max() <= 1e-12 * (np.abs(a[m]).max() + 1e-300), m
+
+
+# ---------------------------------------------------------------- setdt
+def test_setdt_closed_form_and_ranges():
+    c = harris.case_small(20, 12, 20, stretched=False, periodic_y=False, di=0.2, eta0=0.01)
+    shape = (c.mz + 4, c.my, c.mx + 4)
+    rho, vx, vy, vz, b, p = 1.5, 0.1, -0.2, 0.05, (0.3, 0.4, 0.5), 0.7
+    x0 = np.zeros((8,) + shape)
+    x0[0] = rho; x0[1] = rho * vx; x0[2] = rho * vy; x0[3] = rho * vz
+    x0[4], x0[5], x0[6] = b
+    x0[7] = p
+    c.x0 = x0
+    o = mk(c)
+    b2 = sum(t * t for t in b)
+    vf = math.sqrt(b2 / rho + c.gamma * p / rho)
+    wb = 3.141592654 * c.di * math.sqrt(b2) / rho
+    dts = []
+    for v_, d in ((vx, c.dx[4]), (vy, c.dy[3]), (vz, c.dz[4])):
+        vp = abs(v_) + vf + wb / abs(d)
+        dts.append(abs(d) / (vp / c.cfl + c.eta0 / d / 2))
+    assert abs(o.compute_dt() - min(dts)) <= 1e-15 * min(dts)
+    base = o.compute_dt()
+
+    def planted(jx, jy, jz):
+        x = x0.copy()
+        x[1, jz - 1, jy - 1, jx - 1] = rho * 50.0
+        o.set_state(x)
+        return o.compute_dt()
+    # setdt loops jz = zfirst..z_last, jy = 1..my-1, jx = xfirst..x_last
+    assert planted(3, 1, 3) < base
+    assert planted(c.mx, c.my - 1, c.mz) < base
+    assert planted(c.mx + 1, 5, 8) == base       # jx = xlast-1 excluded
+    assert planted(8, c.my, 8) == base           # jy = my excluded
+    assert planted(8, 5, c.mz + 1) == base       # jz = zlast-1 excluded
