@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 RK4 Hall-MHD step on one MI355X.
+
+Workload (default): the tanh-stretched 256x128x256 Harris sheet with Hall
+term (eta 0.003, d_i 0.2), the single-GPU HBM roofline configuration.  A
+"step" is one full RK4 step of the reference's stepon (four right-hand
+sides with every smoothing sweep and boundary fill, the post-step
+perturbation smoothing) plus its setdt, with the state resident in HBM.
+
+Prints ONE JSON line:
+  value          cell-updates/s = mx*ny*mz*steps / T (physical cells)
+  roofline       dominant kernel family: algorithmic bytes / HIP-event time
+  step_roofline  whole step: value * 6256 B (5104 B without Hall) / 8 TB/s
+  cpu_baseline   the CPU oracle (C restatement, OpenMP) on a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
+B_ALG_HALL = 6256           # algorithmic bytes per cell per RK4 step (Hall on)
+B_ALG_NOHALL = 5104         # d_i = 0
+
+
+def per_cell_bytes(case):
+    """algorithmic HBM bytes per interior cell of one launch of each kernel
+    family (each input field read once, each output written once)"""
+    hall = case.hall
+    return {
+        "curl": 8 * 9,                                  # B 3 + B_i 3 in, J 3 out
+        "smooth3": 8 * 6,                               # 3 in, 3 out
+        "smooth8": 8 * 16,                              # 8 in, 8 out
+        "rhs_flux_E_hall": 8 * (15 + 4 + 3 + (3 if hall else 0)),   # U8,p_i,J3,xresd3 in
+        "rhs_curlE_pressure": 8 * ((6 if hall else 3) + 5 + 3 + 4),  # E(+xdh), rho,m,p, J in; 4 out
+        "rk_update": 8 * 32,                            # stages 1-3: xdif,xfold,xm in; xm,x out
+        "post_add_eq": 8 * (8 + 5 + 8),
+        "setdt": 8 * 8,
+    }
+
+
+def cpu_baseline(case_name: str, steps: int, threads: int):
+    from dasha_amd import harris
+    from oracle import oracle
+    oracle.build()
+    oracle.set_threads(threads)
+    case = harris.CASES[case_name]()
+    o = oracle.Oracle(case)
+    o.init_residual()
+    dt = o.compute_dt()
+    t0 = time.perf_counter()
+    tt = 0.0
+    for _ in range(steps):
+        dt = o.compute_dt()
+        o.step(tt, dt)
+        tt += dt
+    el = time.perf_counter() - t0
+    return {"value": case.cells * steps / el, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} RK4 step(s) incl. setdt of {case.name} ({case.cells} cells) "
+                      f"by the C/OpenMP oracle, {el:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
+              f"(the step runs on one GPU's domain).", file=sys.stderr)
+        sys.exit(2)
+
+    from dasha_amd import harris
+    from dasha_amd.mhd import MHDStep
+
+    case = harris.CASES[args.config]()
+    t_setup = time.perf_counter()
+    g = MHDStep(case)
+    g.init_residual()
+    dt0 = g.compute_dt()
+    print(f"[bench] {case.name}: setup {time.perf_counter() - t_setup:.1f}s, dt0 {dt0:.4e}",
+          file=sys.stderr, flush=True)
+    # warm-up steps (untimed)
+    tm, _, _ = g.advance(args.warmup, 0.0, dt0)
+    # timed region: K steps of setdt+stepon, HIP events on the library stream
+    tm, dt_last, ms = g.advance(args.steps, tm, dt0)
+    ms_per_step = ms / max(args.steps, 1)
+    value = case.cells * args.steps / (ms / 1e3)
+    print(f"[bench] timed {args.steps} steps: {ms:.1f} ms ({ms_per_step:.3f} ms/step)",
+          file=sys.stderr, flush=True)
+
+    # per-kernel-family times (events around every launch) on a short run
+    roof = None
+    fam = {}
+    if not args.no_profile:
+        g.set_profiling(True)
+        tm, _, _ = g.advance(min(args.steps, 5), tm, dt0)
+        fam = g.kernel_times()
+        g.set_profiling(False)
+        pcb = per_cell_bytes(case)
+        interior = (case.mx - 4) * (case.my - 4) * (case.mz - 4)
+        ext = case.mx * case.my * case.mz
+        best = max(fam.items(), key=lambda kv: kv[1][0])
+        name, (tot_ms, calls) = best
+        cells = ext if name in ("rk_update", "post_add_eq", "rhs_flux_E_hall") else interior
+        if name in pcb and calls > 0:
+            avg_s = tot_ms / calls / 1e3
+            ach = pcb[name] * cells / avg_s
+            roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": ach / HBM_PEAK, "traffic": None, "kernel": name,
+                    "avg_launch_ms": tot_ms / calls,
+                    "alg_bytes_per_launch": pcb[name] * cells}
+    b_alg = B_ALG_HALL if case.hall else B_ALG_NOHALL
+    out = {
+        "metric": "FP64 cell-updates/s per RK4 step",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{case.name} Harris sheet, RK4 Hall MHD step (stepon+setdt)",
+                   "grid": [case.mx, case.ny, case.mz], "stretched": args.config != "C1",
+                   "hall": case.hall, "eta": case.eta0, "d_i": case.di, "cfl": case.cfl},
+        "roofline": roof,
+        "step_roofline": {"bytes_per_cell_step": b_alg,
+                          "achieved_GBps": value * b_alg / 1e9,
+                          "frac": value * b_alg / HBM_PEAK},
+        "kernel_families_ms": {k: {"total_ms": v[0], "calls": v[1]} for k, v in fam.items()},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

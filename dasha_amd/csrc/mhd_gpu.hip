@@ -727,7 +727,8 @@ struct mhd_handle {
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> pool;
+    size_t nev = 0;
     std::vector<int> ev_tag;
 };
 
@@ -790,14 +791,18 @@ int blocks(long n, int b) { return (int)((n + b - 1) / b); }
 
 static void set_err(mhd_handle *h, const std::string &s) { if (h) h->err = s; }
 
-// profiling helpers: events bracket launches of one family
+// profiling helpers: events bracket launches of one family; events come
+// from a pool that grows once and is reused
 static void prof_mark(mhd_handle *h, int tag)
 {
     if (!h->prof) return;
-    hipEvent_t e;
-    hipEventCreate(&e);
-    hipEventRecord(e, h->stream);
-    h->ev.push_back(e);
+    if (h->nev == h->pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        h->pool.push_back(e);
+    }
+    hipEvent_t e = h->pool[h->nev++];
+    (void)hipEventRecord(e, h->stream);
     h->ev_tag.push_back(tag);
 }
 
@@ -807,17 +812,16 @@ static const char *PT_NAMES[PT_N] = {"curl", "smooth3", "smooth8", "boundary", "
 
 static int prof_collect(mhd_handle *h)
 {
-    if (h->ev.empty()) return 0;
-    HCHK(hipEventSynchronize(h->ev.back()));
-    for (size_t t = 0; t + 1 < h->ev.size(); t += 2) {
+    if (h->nev == 0) return 0;
+    HCHK(hipEventSynchronize(h->pool[h->nev - 1]));
+    for (size_t t = 0; t + 1 < h->nev; t += 2) {
         float ms = 0;
-        hipEventElapsedTime(&ms, h->ev[t], h->ev[t + 1]);
+        (void)hipEventElapsedTime(&ms, h->pool[t], h->pool[t + 1]);
         int tag = h->ev_tag[t];
         h->pt[tag].ms += ms;
         h->pt[tag].calls++;
     }
-    for (auto e : h->ev) hipEventDestroy(e);
-    h->ev.clear();
+    h->nev = 0;
     h->ev_tag.clear();
     return 0;
 }
@@ -1350,8 +1354,8 @@ void mhd_destroy(mhd_handle *h)
                     h->eA, h->eB, h->hA, h->hB, h->red, h->dtv};
     for (auto p : ps)
         if (p) hipFree(p);
-    for (auto e : h->ev) hipEventDestroy(e);
-    if (h->stream) hipStreamDestroy(h->stream);
+    for (auto e : h->pool) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
 
