@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event run")
+    ap.add_argument("--copy-probe", action="store_true", help="also measure a plain device copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
@@ -146,6 +147,9 @@ def main():
         "kernel_families_ms": {k: {"total_ms": v[0], "calls": v[1]} for k, v in fam.items()},
         "cpu_baseline": None,
     }
+    if args.copy_probe:
+        from dasha_amd.mhd import copy_rate
+        out["copy_probe_GBps"] = copy_rate() / 1e9
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_steps, threads)

@@ -48,6 +48,8 @@ struct Geo {
     const double *x2a, *x2b, *x2c, *x2d, *y2a, *y2b, *y2c, *y2d, *z2a, *z2b, *z2c, *z2d;
     // smoothing spacings (xp1-x0, x0-xm1, xp1-xm1) per axis (t3dsdsc.f:5618)
     const double *xhp, *xhm, *xhs, *yhp, *yhm, *yhs, *zhp, *zhm, *zhs;
+    // difc weights 2*hp/hs, 2*hm/hs per axis
+    const double *xwp, *xwm, *ywp, *ywm, *zwp, *zwm;
     const double *xc, *yc, *zc;          // coordinates
     const double *dxs, *dys, *dzs;       // setdt spacings
     const double *cbx;                   // B-component smoothing coefficient per x
@@ -107,12 +109,28 @@ __device__ inline double difc(double fm, double f0, double fp, double hp, double
     return 2. * ((fp - f0) * hp - (f0 - fm) * hm) / hs;
 }
 
+// the same term with the per-axis weights wp = 2*hp/hs, wm = 2*hm/hs
+// precomputed (no division per cell; differs from difc by rounding only)
+__device__ inline double difw(double fm, double f0, double fp, double wp, double wm)
+{
+    return (fp - f0) * wp - (f0 - fm) * wm;
+}
+
 // radial smoothing coefficient, t3dsdsc.f:5659-5661 / 5895-5897 / 5981-5983
 __device__ inline double radial_coef(const Geo &g, int i, int j, int k)
 {
     double x = g.xc[i], y = g.yc[j], z = g.zc[k];
     double r = sqrt(x * x + y * y + z * z);
     return 1.0 / 32.0 + (1.0 / 8.0 - 1.0 / 32.0) * 0.5 * (1 + tanh((r - 3.) / 3.));
+}
+
+// the same coefficient through 1 + tanh(u) = 2 / (1 + exp(-2u)):
+// 1/32 + (3/32) / (1 + exp(-2(r-3)/3)); one exp and one division instead of
+// tanh (equal to rounding)
+__device__ inline double radial_coef_fast(double x2y2, double z)
+{
+    double r = sqrt(x2y2 + z * z);
+    return 1.0 / 32.0 + (3.0 / 32.0) / (1.0 + exp(-2.0 * (r - 3.) / 3.));
 }
 
 enum { KIND_RADIAL = 0, KIND_EIGHTH = 1, KIND_STATE = 2 };
@@ -193,6 +211,425 @@ __global__ void k_filter(Geo g, const double *__restrict__ A, Eq eq, double *__r
         B[q + m * g.S] = f0 + c * s;
     }
 }
+
+// ---------------------------------------------------------------------------
+// z-marching (2.5-D) tiles: a block owns a TX x TY column tile of the
+// extended x-y range and walks a chunk of interior z planes.  Values needed
+// at x/y neighbours sit in an LDS plane with a halo of R; values needed at
+// z neighbours sit in a per-thread register queue.  Each input value is
+// loaded once per plane (plus the tile halo).
+// ---------------------------------------------------------------------------
+constexpr int TX = 64;
+constexpr int TY = 8;
+constexpr int NT = TX * TY;
+
+struct Chunk {
+    int i, j, tx, ty, kb, ke;
+    bool valid, inner;
+};
+
+__device__ inline Chunk chunk_of(const Geo &g, int kc)
+{
+    Chunk c;
+    c.tx = threadIdx.x % TX;
+    c.ty = threadIdx.x / TX;
+    c.i = blockIdx.x * TX + c.tx;
+    c.j = blockIdx.y * TY + c.ty;
+    c.kb = 2 + blockIdx.z * kc;
+    c.ke = min(c.kb + kc, g.NZ - 2);
+    c.valid = c.i < g.NX && c.j < g.NY;
+    c.inner = c.i >= 2 && c.i <= g.NX - 3 && c.j >= 2 && c.j <= g.NY - 3;
+    return c;
+}
+
+// load helper for a smoothing source: buffer value or x - eq on the fly
+template <int SRC>
+__device__ inline double src_at(const double *__restrict__ a, const double *__restrict__ e, long q)
+{
+    if (SRC == SRC_XEQ && e) return a[q] - e[q];
+    return a[q];
+}
+
+template <int NF, int KIND, int SRC>
+__global__ __launch_bounds__(NT) void k_filter25(Geo g, const double *__restrict__ A, Eq eq,
+                                                  double *__restrict__ B, int kc)
+{
+    __shared__ double sp[NF][TY + 2][TX + 2];
+    Chunk c = chunk_of(g, kc);
+    if (c.kb >= c.ke) return;
+    const long S = g.S, sy = g.sy, sz = g.sz;
+    const long q0 = (long)c.i + sy * c.j;
+    double zm[NF], z0[NF], zp[NF];
+    if (c.valid) {
+#pragma unroll
+        for (int m = 0; m < NF; m++) {
+            zm[m] = src_at<SRC>(A + m * S, eq.f[m], q0 + sz * (c.kb - 1));
+            z0[m] = src_at<SRC>(A + m * S, eq.f[m], q0 + sz * c.kb);
+        }
+    }
+    const double wpx = c.inner ? g.xwp[c.i] : 0, wmx = c.inner ? g.xwm[c.i] : 0;
+    const double wpy = c.inner ? g.ywp[c.j] : 0, wmy = c.inner ? g.ywm[c.j] : 0;
+    const double cb = (KIND == KIND_STATE && c.inner) ? g.cbx[c.i] : 0.0;
+    const double xy2 = c.inner ? g.xc[c.i] * g.xc[c.i] + g.yc[c.j] * g.yc[c.j] : 0.0;
+    // halo role of this thread (x edges and y edges of the tile)
+    const int hx = c.tx == 0 ? -1 : (c.tx == TX - 1 ? 1 : 0);
+    const int hy = c.ty == 0 ? -1 : (c.ty == TY - 1 ? 1 : 0);
+    const bool hxok = hx != 0 && c.j < g.NY && c.i + hx >= 0 && c.i + hx < g.NX;
+    const bool hyok = hy != 0 && c.i < g.NX && c.j + hy >= 0 && c.j + hy < g.NY;
+    for (int k = c.kb; k < c.ke; k++) {
+        const long qk = q0 + sz * k;
+        if (c.valid) {
+#pragma unroll
+            for (int m = 0; m < NF; m++) {
+                zp[m] = src_at<SRC>(A + m * S, eq.f[m], qk + sz);
+                sp[m][c.ty + 1][c.tx + 1] = z0[m];
+            }
+        }
+        if (hxok) {
+#pragma unroll
+            for (int m = 0; m < NF; m++) sp[m][c.ty + 1][c.tx + 1 + hx] = src_at<SRC>(A + m * S, eq.f[m], qk + hx);
+        }
+        if (hyok) {
+#pragma unroll
+            for (int m = 0; m < NF; m++) sp[m][c.ty + 1 + hy][c.tx + 1] = src_at<SRC>(A + m * S, eq.f[m], qk + hy * sy);
+        }
+        __syncthreads();
+        if (c.inner) {
+            const double wpz = g.zwp[k], wmz = g.zwm[k];
+            const double crad = (KIND == KIND_EIGHTH) ? 1.0 / 8.0 : radial_coef_fast(xy2, g.zc[k]);
+#pragma unroll
+            for (int m = 0; m < NF; m++) {
+                double f0 = z0[m];
+                double s = difw(sp[m][c.ty + 1][c.tx], f0, sp[m][c.ty + 1][c.tx + 2], wpx, wmx) +
+                           difw(sp[m][c.ty][c.tx + 1], f0, sp[m][c.ty + 2][c.tx + 1], wpy, wmy) +
+                           difw(zm[m], f0, zp[m], wpz, wmz);
+                double cc = (KIND == KIND_STATE && m >= 4 && m <= 6) ? cb : crad;
+                B[qk + m * S] = f0 + cc * s;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NF; m++) { zm[m] = z0[m]; z0[m] = zp[m]; }
+    }
+}
+
+// curl(B - B_i) on the interior (t3dsdsc.f:2465-2491), z-marching
+__global__ __launch_bounds__(NT) void k_curl25(Geo g, const double *__restrict__ X, const double *__restrict__ bxi,
+                                               const double *__restrict__ byi, const double *__restrict__ bzi,
+                                               double *__restrict__ cur, int kc)
+{
+    __shared__ double sp[3][TY + 4][TX + 4];
+    Chunk c = chunk_of(g, kc);
+    if (c.kb >= c.ke) return;
+    const long S = g.S, sy = g.sy, sz = g.sz;
+    const long q0 = (long)c.i + sy * c.j;
+    const double *b5 = X + 4 * S, *b6 = X + 5 * S, *b7 = X + 6 * S;
+    double uq[5], vq[5];   // z queue of u = Bx - bxi, v = By - byi (planes k-2..k+2)
+    if (c.valid) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            long q = q0 + sz * (c.kb - 2 + t);
+            uq[t] = b5[q] - bxi[q];
+            vq[t] = b6[q] - byi[q];
+        }
+    }
+    const int hx = c.tx < 2 ? -2 : (c.tx >= TX - 2 ? 2 : 0);
+    const int hy = c.ty < 2 ? -2 : (c.ty >= TY - 2 ? 2 : 0);
+    const bool hxok = hx != 0 && c.j < g.NY && c.i + hx >= 0 && c.i + hx < g.NX;
+    const bool hyok = hy != 0 && c.i < g.NX && c.j + hy >= 0 && c.j + hy < g.NY;
+    double ax[4], ay[4];
+    if (c.inner) {
+        ax[0] = g.x1a[c.i]; ax[1] = g.x1b[c.i]; ax[2] = g.x1c[c.i]; ax[3] = g.x1d[c.i];
+        ay[0] = g.y1a[c.j]; ay[1] = g.y1b[c.j]; ay[2] = g.y1c[c.j]; ay[3] = g.y1d[c.j];
+    }
+    for (int k = c.kb; k < c.ke; k++) {
+        const long qk = q0 + sz * k;
+        if (c.valid) {
+            long q = qk + 2 * sz;
+            uq[4] = b5[q] - bxi[q];
+            vq[4] = b6[q] - byi[q];
+            sp[0][c.ty + 2][c.tx + 2] = uq[2];
+            sp[1][c.ty + 2][c.tx + 2] = vq[2];
+            sp[2][c.ty + 2][c.tx + 2] = b7[qk] - bzi[qk];
+        }
+        if (hxok) {
+            long q = qk + hx;
+            sp[0][c.ty + 2][c.tx + 2 + hx] = b5[q] - bxi[q];
+            sp[1][c.ty + 2][c.tx + 2 + hx] = b6[q] - byi[q];
+            sp[2][c.ty + 2][c.tx + 2 + hx] = b7[q] - bzi[q];
+        }
+        if (hyok) {
+            long q = qk + hy * sy;
+            sp[0][c.ty + 2 + hy][c.tx + 2] = b5[q] - bxi[q];
+            sp[1][c.ty + 2 + hy][c.tx + 2] = b6[q] - byi[q];
+            sp[2][c.ty + 2 + hy][c.tx + 2] = b7[q] - bzi[q];
+        }
+        __syncthreads();
+        if (c.inner) {
+            const int X0 = c.tx + 2, Y0 = c.ty + 2;
+            double az[4] = {g.z1a[k], g.z1b[k], g.z1c[k], g.z1d[k]};
+#define DXS(f) fd5(sp[f][Y0][X0 - 2], sp[f][Y0][X0 - 1], sp[f][Y0][X0], sp[f][Y0][X0 + 1], sp[f][Y0][X0 + 2], ax[0], ax[1], ax[2], ax[3])
+#define DYS(f) fd5(sp[f][Y0 - 2][X0], sp[f][Y0 - 1][X0], sp[f][Y0][X0], sp[f][Y0 + 1][X0], sp[f][Y0 + 2][X0], ay[0], ay[1], ay[2], ay[3])
+            double wy = DYS(2), wx = DXS(2), vx = DXS(1), uy = DYS(0);
+            double vz = fd5(vq[0], vq[1], vq[2], vq[3], vq[4], az[0], az[1], az[2], az[3]);
+            double uz = fd5(uq[0], uq[1], uq[2], uq[3], uq[4], az[0], az[1], az[2], az[3]);
+#undef DXS
+#undef DYS
+            cur[qk] = wy - vz;
+            cur[qk + S] = uz - wx;
+            cur[qk + 2 * S] = vx - uy;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 4; t++) { uq[t] = uq[t + 1]; vq[t] = vq[t + 1]; }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// right, part 1 as a z-marching tile.  Per point the momentum flux tensor
+// (t3dsdsc.f:2311-2346) is formed once:
+//   T22 = x2*x2/x1 + (p - p_i), T23 = x2*x3/x1, T24 = x2*x4/x1,
+//   T33 = x3*x3/x1 + (p - p_i), T34 = x4*x3/x1, T44 = x4*x4/x1 + (p - p_i)
+// and xdif(1..4) = -(d/dx, d/dy, d/dz) of (x2,x3,x4), (T22,T23,T24),
+// (T23,T33,T34), (T24,T34,T44): the same values the reference's flux(m)
+// arrays hold.  LDS holds {x2,x3,T22,T23,T24,T33,T34,p} of plane k with a
+// halo of 2; the register queue holds {x4,T24,T34,T44,p} of planes k-2..k+2.
+// ---------------------------------------------------------------------------
+struct Rec {
+    double x2, x3, x4, t22, t23, t24, t33, t34, t44, p;
+};
+
+__device__ inline Rec rec_at(const double *__restrict__ X, const double *__restrict__ pri, long S, long q)
+{
+    Rec r;
+    double x1 = X[q];
+    r.x2 = X[q + S]; r.x3 = X[q + 2 * S]; r.x4 = X[q + 3 * S]; r.p = X[q + 7 * S];
+    double dp = r.p - pri[q];
+    r.t22 = r.x2 * r.x2 / x1 + dp;
+    r.t23 = r.x2 * r.x3 / x1;
+    r.t24 = r.x2 * r.x4 / x1;
+    r.t33 = r.x3 * r.x3 / x1 + dp;
+    r.t34 = r.x4 * r.x3 / x1;
+    r.t44 = r.x4 * r.x4 / x1 + dp;
+    return r;
+}
+
+enum { LX2 = 0, LX3, LT22, LT23, LT24, LT33, LT34, LP, NLA };
+
+__device__ inline void rec_to_lds(double (*sp)[TY + 4][TX + 4], int y, int x, const Rec &r)
+{
+    sp[LX2][y][x] = r.x2; sp[LX3][y][x] = r.x3; sp[LT22][y][x] = r.t22; sp[LT23][y][x] = r.t23;
+    sp[LT24][y][x] = r.t24; sp[LT33][y][x] = r.t33; sp[LT34][y][x] = r.t34; sp[LP][y][x] = r.p;
+}
+
+__device__ inline void e_point(const double *__restrict__ X, const double *__restrict__ cur, double eta, long S,
+                               long q, double *__restrict__ E)
+{
+    double x1 = X[q], x2 = X[q + S], x3 = X[q + 2 * S], x4 = X[q + 3 * S];
+    double x5 = X[q + 4 * S], x6 = X[q + 5 * S], x7 = X[q + 6 * S];
+    E[q] = (-x3 * x7 + x4 * x6) / x1 + eta * cur[q];
+    E[q + S] = (-x4 * x5 + x2 * x7) / x1 + eta * cur[q + S];
+    E[q + 2 * S] = (-x2 * x6 + x3 * x5) / x1 + eta * cur[q + 2 * S];
+}
+
+__global__ __launch_bounds__(NT) void k_rhs_a25(Geo g, const double *__restrict__ X, const double *__restrict__ pri,
+                                                const double *__restrict__ cur, const double *__restrict__ xresd,
+                                                double eta, int hall, double cht, double *__restrict__ XD,
+                                                double *__restrict__ E, double *__restrict__ XH, int kc)
+{
+    __shared__ double sp[NLA][TY + 4][TX + 4];
+    Chunk c = chunk_of(g, kc);
+    if (c.kb >= c.ke) return;
+    const long S = g.S, sy = g.sy, sz = g.sz;
+    const long q0 = (long)c.i + sy * c.j;
+    // E on the planes outside the interior z range (first / last chunk)
+    if (c.valid) {
+        if (c.kb == 2) { e_point(X, cur, eta, S, q0, E); e_point(X, cur, eta, S, q0 + sz, E); }
+        if (c.ke == g.NZ - 2) {
+            e_point(X, cur, eta, S, q0 + sz * (g.NZ - 2), E);
+            e_point(X, cur, eta, S, q0 + sz * (g.NZ - 1), E);
+        }
+    }
+    // queues: z fields for planes k-2..k+2, x-y fields for planes k..k+2
+    double qx4[5], qt24[5], qt34[5], qt44[5], qp[5];
+    double yx2[3], yx3[3], yt22[3], yt23[3], yt33[3];
+    if (c.valid) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            Rec r = rec_at(X, pri, S, q0 + sz * (c.kb - 2 + t));
+            qx4[t] = r.x4; qt24[t] = r.t24; qt34[t] = r.t34; qt44[t] = r.t44; qp[t] = r.p;
+            if (t >= 2) {
+                yx2[t - 2] = r.x2; yx3[t - 2] = r.x3; yt22[t - 2] = r.t22; yt23[t - 2] = r.t23; yt33[t - 2] = r.t33;
+            }
+        }
+    }
+    const int hx = c.tx < 2 ? -2 : (c.tx >= TX - 2 ? 2 : 0);
+    const int hy = c.ty < 2 ? -2 : (c.ty >= TY - 2 ? 2 : 0);
+    const bool hxok = hx != 0 && c.j < g.NY && c.i + hx >= 0 && c.i + hx < g.NX;
+    const bool hyok = hy != 0 && c.i < g.NX && c.j + hy >= 0 && c.j + hy < g.NY;
+    double ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
+    if (c.inner) {
+        ax[0] = g.x1a[c.i]; ax[1] = g.x1b[c.i]; ax[2] = g.x1c[c.i]; ax[3] = g.x1d[c.i];
+        ay[0] = g.y1a[c.j]; ay[1] = g.y1b[c.j]; ay[2] = g.y1c[c.j]; ay[3] = g.y1d[c.j];
+    }
+    for (int k = c.kb; k < c.ke; k++) {
+        const long qk = q0 + sz * k;
+        if (c.valid) {
+            Rec r = rec_at(X, pri, S, qk + 2 * sz);
+            qx4[4] = r.x4; qt24[4] = r.t24; qt34[4] = r.t34; qt44[4] = r.t44; qp[4] = r.p;
+            yx2[2] = r.x2; yx3[2] = r.x3; yt22[2] = r.t22; yt23[2] = r.t23; yt33[2] = r.t33;
+            const int y = c.ty + 2, x = c.tx + 2;
+            sp[LX2][y][x] = yx2[0]; sp[LX3][y][x] = yx3[0]; sp[LT22][y][x] = yt22[0]; sp[LT23][y][x] = yt23[0];
+            sp[LT24][y][x] = qt24[2]; sp[LT33][y][x] = yt33[0]; sp[LT34][y][x] = qt34[2]; sp[LP][y][x] = qp[2];
+            e_point(X, cur, eta, S, qk, E);
+        }
+        if (hxok) rec_to_lds(sp, c.ty + 2, c.tx + 2 + hx, rec_at(X, pri, S, qk + hx));
+        if (hyok) rec_to_lds(sp, c.ty + 2 + hy, c.tx + 2, rec_at(X, pri, S, qk + hy * sy));
+        __syncthreads();
+        if (c.inner) {
+            const int X0 = c.tx + 2, Y0 = c.ty + 2;
+            double az[4] = {g.z1a[k], g.z1b[k], g.z1c[k], g.z1d[k]};
+#define DXL(f) fd5(sp[f][Y0][X0 - 2], sp[f][Y0][X0 - 1], sp[f][Y0][X0], sp[f][Y0][X0 + 1], sp[f][Y0][X0 + 2], ax[0], ax[1], ax[2], ax[3])
+#define DYL(f) fd5(sp[f][Y0 - 2][X0], sp[f][Y0 - 1][X0], sp[f][Y0][X0], sp[f][Y0 + 1][X0], sp[f][Y0 + 2][X0], ay[0], ay[1], ay[2], ay[3])
+#define DZQ(a) fd5(a[0], a[1], a[2], a[3], a[4], az[0], az[1], az[2], az[3])
+            double xd0 = -DXL(LX2) - DYL(LX3) - DZQ(qx4);
+            double xd1 = -DXL(LT22) - DYL(LT23) - DZQ(qt24);
+            double xd2 = -DXL(LT23) - DYL(LT33) - DZQ(qt34);
+            double xd3 = -DXL(LT24) - DYL(LT34) - DZQ(qt44);
+            double x1 = X[qk], x5 = X[qk + 4 * S], x6 = X[qk + 5 * S], x7 = X[qk + 6 * S];
+            double c1 = cur[qk], c2 = cur[qk + S], c3 = cur[qk + 2 * S];
+            xd1 = xd1 - xresd[qk] + c2 * x7 - c3 * x6;
+            xd2 = xd2 - xresd[qk + S] + c3 * x5 - c1 * x7;
+            xd3 = xd3 - xresd[qk + 2 * S] + c1 * x6 - c2 * x5;
+            XD[qk] = xd0; XD[qk + S] = xd1; XD[qk + 2 * S] = xd2; XD[qk + 3 * S] = xd3;
+            if (hall) {
+                XH[qk] = cht * (c2 * x7 - c3 * x6 - DXL(LP)) / x1;
+                XH[qk + S] = cht * (c3 * x5 - c1 * x7 - DYL(LP)) / x1;
+                XH[qk + 2 * S] = cht * (c1 * x6 - c2 * x5 - DZQ(qp)) / x1;
+            }
+#undef DZQ
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            qx4[t] = qx4[t + 1]; qt24[t] = qt24[t + 1]; qt34[t] = qt34[t + 1]; qt44[t] = qt44[t + 1]; qp[t] = qp[t + 1];
+        }
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            yx2[t] = yx2[t + 1]; yx3[t] = yx3[t + 1]; yt22[t] = yt22[t + 1]; yt23[t] = yt23[t + 1]; yt33[t] = yt33[t + 1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// right, part 2 as a z-marching tile: -curl(E + xdh) and the pressure
+// equation (t3dsdsc.f:1426-1566), optional viscosity/diffusion (1585-1663).
+// Per point e = E + xdh and v = m / rho are formed once.
+// LDS: {e1,e2,e3,p,vx,vy[,vz,rho]}; queue: {e1,e2,p,vz[,vx,vy,rho]}.
+// ---------------------------------------------------------------------------
+enum { LE1 = 0, LE2, LE3, LPP, LVX, LVY, LVZ, LRH };
+
+template <bool VD>
+__global__ __launch_bounds__(NT) void k_rhs_b25(Geo g, const double *__restrict__ X, const double *__restrict__ cur,
+                                                const double *__restrict__ E, const double *__restrict__ XH,
+                                                double eta, double gam, double fmu, double frho, double fpr,
+                                                double *__restrict__ XD, int kc)
+{
+    constexpr int NL = VD ? 8 : 6;
+    __shared__ double sp[NL][TY + 4][TX + 4];
+    Chunk c = chunk_of(g, kc);
+    if (c.kb >= c.ke) return;
+    const long S = g.S, sy = g.sy, sz = g.sz;
+    const long q0 = (long)c.i + sy * c.j;
+    auto ept = [&](long q, int m) { return XH ? E[q + m * S] + XH[q + m * S] : E[q + m * S]; };
+    double qe1[5], qe2[5], qp[5], qvz[5], qvx[5], qvy[5], qrh[5];
+    auto load_q = [&](int t, long q) {
+        double r = X[q];
+        qe1[t] = ept(q, 0); qe2[t] = ept(q, 1); qp[t] = X[q + 7 * S]; qvz[t] = X[q + 3 * S] / r;
+        if (VD) { qvx[t] = X[q + S] / r; qvy[t] = X[q + 2 * S] / r; qrh[t] = r; }
+    };
+    auto to_lds = [&](int y, int x, long q) {
+        double r = X[q];
+        sp[LE1][y][x] = ept(q, 0); sp[LE2][y][x] = ept(q, 1); sp[LE3][y][x] = ept(q, 2);
+        sp[LPP][y][x] = X[q + 7 * S]; sp[LVX][y][x] = X[q + S] / r; sp[LVY][y][x] = X[q + 2 * S] / r;
+        if (VD) { sp[LVZ][y][x] = X[q + 3 * S] / r; sp[LRH][y][x] = r; }
+    };
+    if (c.valid) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) load_q(t, q0 + sz * (c.kb - 2 + t));
+    }
+    const int hx = c.tx < 2 ? -2 : (c.tx >= TX - 2 ? 2 : 0);
+    const int hy = c.ty < 2 ? -2 : (c.ty >= TY - 2 ? 2 : 0);
+    const bool hxok = hx != 0 && c.j < g.NY && c.i + hx >= 0 && c.i + hx < g.NX;
+    const bool hyok = hy != 0 && c.i < g.NX && c.j + hy >= 0 && c.j + hy < g.NY;
+    double ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
+    if (c.inner) {
+        ax[0] = g.x1a[c.i]; ax[1] = g.x1b[c.i]; ax[2] = g.x1c[c.i]; ax[3] = g.x1d[c.i];
+        ay[0] = g.y1a[c.j]; ay[1] = g.y1b[c.j]; ay[2] = g.y1c[c.j]; ay[3] = g.y1d[c.j];
+        if (VD) {
+            bx[0] = g.x2a[c.i]; bx[1] = g.x2b[c.i]; bx[2] = g.x2c[c.i]; bx[3] = g.x2d[c.i];
+            by[0] = g.y2a[c.j]; by[1] = g.y2b[c.j]; by[2] = g.y2c[c.j]; by[3] = g.y2d[c.j];
+        }
+    }
+    for (int k = c.kb; k < c.ke; k++) {
+        const long qk = q0 + sz * k;
+        if (c.valid) {
+            load_q(4, qk + 2 * sz);
+            to_lds(c.ty + 2, c.tx + 2, qk);
+        }
+        if (hxok) to_lds(c.ty + 2, c.tx + 2 + hx, qk + hx);
+        if (hyok) to_lds(c.ty + 2 + hy, c.tx + 2, qk + hy * sy);
+        __syncthreads();
+        if (c.inner) {
+            const int X0 = c.tx + 2, Y0 = c.ty + 2;
+            double az[4] = {g.z1a[k], g.z1b[k], g.z1c[k], g.z1d[k]};
+#define DZQ(a) fd5(a[0], a[1], a[2], a[3], a[4], az[0], az[1], az[2], az[3])
+            double xd5 = -DYL(LE3) + DZQ(qe2);
+            double xd6 = -DZQ(qe1) + DXL(LE3);
+            double xd7 = -DXL(LE2) + DYL(LE1);
+            double fs0 = sp[LVX][Y0][X0], gs0 = sp[LVY][Y0][X0], hs0 = qvz[2], p0 = qp[2];
+            double dvx = DXL(LVX), dvy = DYL(LVY), dvz = DZQ(qvz);
+            double c1 = cur[qk], c2 = cur[qk + S], c3 = cur[qk + 2 * S];
+            double xd8 = -DXL(LPP) * fs0 - DYL(LPP) * gs0 - DZQ(qp) * hs0 + (-dvx - dvy - dvz) * gam * p0 +
+                         (gam - 1.) * eta * (c1 * c1 + c2 * c2 + c3 * c3);
+#undef DZQ
+            if (VD) {
+                double bz[4] = {g.z2a[k], g.z2b[k], g.z2c[k], g.z2d[k]};
+#define D2X(f) fd5(sp[f][Y0][X0 - 2], sp[f][Y0][X0 - 1], sp[f][Y0][X0], sp[f][Y0][X0 + 1], sp[f][Y0][X0 + 2], bx[0], bx[1], bx[2], bx[3])
+#define D2Y(f) fd5(sp[f][Y0 - 2][X0], sp[f][Y0 - 1][X0], sp[f][Y0][X0], sp[f][Y0 + 1][X0], sp[f][Y0 + 2][X0], by[0], by[1], by[2], by[3])
+#define D2Z(a) fd5(a[0], a[1], a[2], a[3], a[4], bz[0], bz[1], bz[2], bz[3])
+                if (fmu != 0.0) {
+                    double af = D2X(LVX) + D2Y(LVX) + D2Z(qvx);
+                    double ag = D2X(LVY) + D2Y(LVY) + D2Z(qvy);
+                    double ah = D2X(LVZ) + D2Y(LVZ) + D2Z(qvz);
+                    XD[qk + S] = XD[qk + S] + fmu * af;
+                    XD[qk + 2 * S] = XD[qk + 2 * S] + fmu * ag;
+                    XD[qk + 3 * S] = XD[qk + 3 * S] + fmu * ah;
+                    xd8 = xd8 + fmu * (fs0 * af + gs0 * ag + hs0 * ah);
+                }
+                if (frho != 0.0 || fpr != 0.0) {
+                    XD[qk] = XD[qk] + frho * (D2X(LRH) + D2Y(LRH) + D2Z(qrh));
+                    xd8 = xd8 + fpr * (D2X(LPP) + D2Y(LPP) + D2Z(qp));
+                }
+#undef D2X
+#undef D2Y
+#undef D2Z
+            }
+            XD[qk + 4 * S] = xd5;
+            XD[qk + 5 * S] = xd6;
+            XD[qk + 6 * S] = xd7;
+            XD[qk + 7 * S] = xd8;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            qe1[t] = qe1[t + 1]; qe2[t] = qe2[t + 1]; qp[t] = qp[t + 1]; qvz[t] = qvz[t + 1];
+            if (VD) { qvx[t] = qvx[t + 1]; qvy[t] = qvy[t + 1]; qrh[t] = qrh[t + 1]; }
+        }
+    }
+}
+#undef DXL
+#undef DYL
 
 // ---------------------------------------------------------------------------
 // boundary fills: bndryc/bndryE/bndryh (BC_AUX, t3dsdsc.f:2539-3027),
@@ -477,6 +914,12 @@ __global__ void k_copy3(long S, long n, const double *__restrict__ A, double *__
 
 __global__ void k_set(double *p, double v) { *p = v; }
 
+// plain streaming copy (reference rate for the roofline figures)
+__global__ void k_copy(long n, const double2 *__restrict__ a, double2 *__restrict__ b)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) b[t] = a[t];
+}
+
 // ---------------------------------------------------------------------------
 // setdt, t3dsdsc.f:4709-4808: minimum over jz = 3..mz, jy = 1..my-1,
 // jx = 3..mx  (k = 0..NZ-3, j = 0..NY-2, i = 0..NX-3)
@@ -724,6 +1167,8 @@ struct mhd_handle {
     double *dwx = nullptr, *dwy = nullptr, *dwz = nullptr;
     std::string err;
     bool have_eq = false;
+    int kc = 16;                           // z planes per tile
+    bool naive = false;                    // one-point-per-thread kernels (A/B checks)
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -867,6 +1312,11 @@ static void launch_bc(mhd_handle *h, int type, int nf, double *B, const double *
 
 static const dim3 BLK(64, 4, 1);
 
+static dim3 tile_grid(const mhd_handle *h)
+{
+    return dim3((unsigned)blocks(h->NX, TX), (unsigned)blocks(h->NY, TY), (unsigned)blocks(h->NZ - 4, h->kc));
+}
+
 // kk smoothing passes on a 3-field array with its aux boundary fill;
 // returns the buffer holding the result
 static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk)
@@ -874,10 +1324,18 @@ static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk)
     Eq none = eq_state(h, false);
     for (int p = 0; p < kk; p++) {
         PROF_BEGIN(PT_FILT3);
-        if (kind == KIND_RADIAL)
-            hipLaunchKernelGGL((k_filter<3, KIND_RADIAL, SRC_BUF>), int_grid(h, BLK), BLK, 0, h->stream, h->g, a, none, b);
-        else
-            hipLaunchKernelGGL((k_filter<3, KIND_EIGHTH, SRC_BUF>), int_grid(h, BLK), BLK, 0, h->stream, h->g, a, none, b);
+        if (h->naive) {
+            if (kind == KIND_RADIAL)
+                hipLaunchKernelGGL((k_filter<3, KIND_RADIAL, SRC_BUF>), int_grid(h, BLK), BLK, 0, h->stream, h->g, a, none, b);
+            else
+                hipLaunchKernelGGL((k_filter<3, KIND_EIGHTH, SRC_BUF>), int_grid(h, BLK), BLK, 0, h->stream, h->g, a, none, b);
+        } else if (kind == KIND_RADIAL) {
+            hipLaunchKernelGGL((k_filter25<3, KIND_RADIAL, SRC_BUF>), tile_grid(h), dim3(NT), 0, h->stream, h->g, a, none, b,
+                               h->kc);
+        } else {
+            hipLaunchKernelGGL((k_filter25<3, KIND_EIGHTH, SRC_BUF>), tile_grid(h), dim3(NT), 0, h->stream, h->g, a, none, b,
+                               h->kc);
+        }
         PROF_END(PT_FILT3);
         launch_bc(h, BC_AUX, 3, b, nullptr, nullptr, none);
         std::swap(a, b);
@@ -890,8 +1348,12 @@ static void current(mhd_handle *h, const double *X)
 {
     Eq none = eq_state(h, false);
     PROF_BEGIN(PT_CURL);
-    hipLaunchKernelGGL(k_curl, int_grid(h, BLK), BLK, 0, h->stream, h->g, X, h->eq + h->S, h->eq + 2 * h->S,
-                       h->eq + 3 * h->S, h->cA);
+    if (h->naive)
+        hipLaunchKernelGGL(k_curl, int_grid(h, BLK), BLK, 0, h->stream, h->g, X, h->eq + h->S, h->eq + 2 * h->S,
+                           h->eq + 3 * h->S, h->cA);
+    else
+        hipLaunchKernelGGL(k_curl25, tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->eq + h->S, h->eq + 2 * h->S,
+                           h->eq + 3 * h->S, h->cA, h->kc);
     PROF_END(PT_CURL);
     launch_bc(h, BC_AUX, 3, h->cA, nullptr, nullptr, none);
     h->cur = smooth_aux(h, h->cA, h->cB, KIND_RADIAL, 4);
@@ -907,8 +1369,12 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init)
     current(h, X);
     dim3 gext((unsigned)blocks(h->NX, BLK.x), (unsigned)blocks(h->NY, BLK.y), (unsigned)h->NZ);
     PROF_BEGIN(PT_RHSA);
-    hipLaunchKernelGGL(k_rhs_a, gext, BLK, 0, h->stream, h->g, X, h->eq + 4 * h->S, h->cur, h->xresd, eta,
-                       c.hall, c.di, h->xdA, h->eA, h->hA);
+    if (h->naive)
+        hipLaunchKernelGGL(k_rhs_a, gext, BLK, 0, h->stream, h->g, X, h->eq + 4 * h->S, h->cur, h->xresd, eta,
+                           c.hall, c.di, h->xdA, h->eA, h->hA);
+    else
+        hipLaunchKernelGGL(k_rhs_a25, tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->eq + 4 * h->S, h->cur, h->xresd,
+                           eta, c.hall, c.di, h->xdA, h->eA, h->hA, h->kc);
     PROF_END(PT_RHSA);
     double *E = smooth_aux(h, h->eA, h->eB, KIND_RADIAL, 4);
     double *XH = nullptr;
@@ -917,16 +1383,28 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init)
         XH = smooth_aux(h, h->hA, h->hB, KIND_EIGHTH, 5);
     }
     PROF_BEGIN(PT_RHSB);
-    hipLaunchKernelGGL(k_rhs_b, int_grid(h, BLK), BLK, 0, h->stream, h->g, X, h->cur, E, XH, eta, c.gamma, fmu,
-                       frho, fpr, h->xdA);
+    bool vd = fmu != 0.0 || frho != 0.0 || fpr != 0.0;
+    if (h->naive)
+        hipLaunchKernelGGL(k_rhs_b, int_grid(h, BLK), BLK, 0, h->stream, h->g, X, h->cur, E, XH, eta, c.gamma, fmu,
+                           frho, fpr, h->xdA);
+    else if (vd)
+        hipLaunchKernelGGL((k_rhs_b25<true>), tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->cur, E, XH, eta, c.gamma,
+                           fmu, frho, fpr, h->xdA, h->kc);
+    else
+        hipLaunchKernelGGL((k_rhs_b25<false>), tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->cur, E, XH, eta,
+                           c.gamma, fmu, frho, fpr, h->xdA, h->kc);
     PROF_END(PT_RHSB);
     launch_bc(h, BC_STATE, 8, h->xdA, nullptr, nullptr, none);
     double *a = h->xdA, *b = h->xdB;
     if (smooth_xdif) {
         for (int p = 0; p < 2; p++) {
             PROF_BEGIN(PT_FILT8);
-            hipLaunchKernelGGL((k_filter<8, KIND_STATE, SRC_BUF>), int_grid(h, BLK), BLK, 0, h->stream, h->g, a,
-                               none, b);
+            if (h->naive)
+                hipLaunchKernelGGL((k_filter<8, KIND_STATE, SRC_BUF>), int_grid(h, BLK), BLK, 0, h->stream, h->g, a,
+                                   none, b);
+            else
+                hipLaunchKernelGGL((k_filter25<8, KIND_STATE, SRC_BUF>), tile_grid(h), dim3(NT), 0, h->stream, h->g, a,
+                                   none, b, h->kc);
             PROF_END(PT_FILT8);
             launch_bc(h, BC_STATE, 8, b, nullptr, nullptr, none);
             std::swap(a, b);
@@ -957,14 +1435,22 @@ static void step_body(mhd_handle *h)
     // perturbation smoothing (1135-1165): xm = x - eq; avrgtt x2; x = xm + eq
     Eq sub = eq_state(h, true);
     PROF_BEGIN(PT_FILT8);
-    hipLaunchKernelGGL((k_filter<8, KIND_STATE, SRC_XEQ>), int_grid(h, BLK), BLK, 0, h->stream, h->g, h->x, sub,
-                       h->xm);
+    if (h->naive)
+        hipLaunchKernelGGL((k_filter<8, KIND_STATE, SRC_XEQ>), int_grid(h, BLK), BLK, 0, h->stream, h->g, h->x, sub,
+                           h->xm);
+    else
+        hipLaunchKernelGGL((k_filter25<8, KIND_STATE, SRC_XEQ>), tile_grid(h), dim3(NT), 0, h->stream, h->g, h->x, sub,
+                           h->xm, h->kc);
     PROF_END(PT_FILT8);
     launch_bc(h, BC_TT, 8, h->xm, nullptr, h->x, sub);
     Eq none = eq_state(h, false);
     PROF_BEGIN(PT_FILT8);
-    hipLaunchKernelGGL((k_filter<8, KIND_STATE, SRC_BUF>), int_grid(h, BLK), BLK, 0, h->stream, h->g, h->xm, none,
-                       h->xdB);
+    if (h->naive)
+        hipLaunchKernelGGL((k_filter<8, KIND_STATE, SRC_BUF>), int_grid(h, BLK), BLK, 0, h->stream, h->g, h->xm, none,
+                           h->xdB);
+    else
+        hipLaunchKernelGGL((k_filter25<8, KIND_STATE, SRC_BUF>), tile_grid(h), dim3(NT), 0, h->stream, h->g, h->xm, none,
+                           h->xdB, h->kc);
     PROF_END(PT_FILT8);
     launch_bc(h, BC_TT, 8, h->xdB, h->xm, nullptr, none);
     PROF_BEGIN(PT_POST);
@@ -1036,21 +1522,24 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     for (int j = 0; j < NY; j++) { Y[j] = yy[j]; DYs[j] = dy[j]; }
     for (int k = 0; k < NZ; k++) { Z[k] = zz[k + 2]; DZs[k] = dz[k + 2]; }
     // coefficient arrays: 8 (d1,d2) + 3 (spacings) per axis
-    std::vector<double> cx(11 * NX, 0.0), cy(11 * NY, 0.0), cz(11 * NZ, 0.0);
+    std::vector<double> cx(13 * NX, 0.0), cy(13 * NY, 0.0), cz(13 * NZ, 0.0);
     for (int i = 2; i <= NX - 3; i++) {
         coef_d1(X.data(), i, &cx[0 * NX + i], &cx[1 * NX + i], &cx[2 * NX + i], &cx[3 * NX + i]);
         coef_d2(X.data(), i, &cx[4 * NX + i], &cx[5 * NX + i], &cx[6 * NX + i], &cx[7 * NX + i]);
         cx[8 * NX + i] = X[i + 1] - X[i]; cx[9 * NX + i] = X[i] - X[i - 1]; cx[10 * NX + i] = X[i + 1] - X[i - 1];
+        cx[11 * NX + i] = 2. * cx[8 * NX + i] / cx[10 * NX + i]; cx[12 * NX + i] = 2. * cx[9 * NX + i] / cx[10 * NX + i];
     }
     for (int j = 2; j <= NY - 3; j++) {
         coef_d1(Y.data(), j, &cy[0 * NY + j], &cy[1 * NY + j], &cy[2 * NY + j], &cy[3 * NY + j]);
         coef_d2(Y.data(), j, &cy[4 * NY + j], &cy[5 * NY + j], &cy[6 * NY + j], &cy[7 * NY + j]);
         cy[8 * NY + j] = Y[j + 1] - Y[j]; cy[9 * NY + j] = Y[j] - Y[j - 1]; cy[10 * NY + j] = Y[j + 1] - Y[j - 1];
+        cy[11 * NY + j] = 2. * cy[8 * NY + j] / cy[10 * NY + j]; cy[12 * NY + j] = 2. * cy[9 * NY + j] / cy[10 * NY + j];
     }
     for (int k = 2; k <= NZ - 3; k++) {
         coef_d1(Z.data(), k, &cz[0 * NZ + k], &cz[1 * NZ + k], &cz[2 * NZ + k], &cz[3 * NZ + k]);
         coef_d2(Z.data(), k, &cz[4 * NZ + k], &cz[5 * NZ + k], &cz[6 * NZ + k], &cz[7 * NZ + k]);
         cz[8 * NZ + k] = Z[k + 1] - Z[k]; cz[9 * NZ + k] = Z[k] - Z[k - 1]; cz[10 * NZ + k] = Z[k + 1] - Z[k - 1];
+        cz[11 * NZ + k] = 2. * cz[8 * NZ + k] / cz[10 * NZ + k]; cz[12 * NZ + k] = 2. * cz[9 * NZ + k] / cz[10 * NZ + k];
     }
     std::vector<double> cbx(NX);
     for (int i = 0; i < NX; i++) cbx[i] = (1 - 0.0) * (1.0 / 32.0) * (1 - tanh((X[i] + 3) / 8.));
@@ -1079,13 +1568,13 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     const double *G0 = h->geo;
     g.x1a = G0 + ocx; g.x1b = g.x1a + NX; g.x1c = g.x1b + NX; g.x1d = g.x1c + NX;
     g.x2a = g.x1d + NX; g.x2b = g.x2a + NX; g.x2c = g.x2b + NX; g.x2d = g.x2c + NX;
-    g.xhp = g.x2d + NX; g.xhm = g.xhp + NX; g.xhs = g.xhm + NX;
+    g.xhp = g.x2d + NX; g.xhm = g.xhp + NX; g.xhs = g.xhm + NX; g.xwp = g.xhs + NX; g.xwm = g.xwp + NX;
     g.y1a = G0 + ocy; g.y1b = g.y1a + NY; g.y1c = g.y1b + NY; g.y1d = g.y1c + NY;
     g.y2a = g.y1d + NY; g.y2b = g.y2a + NY; g.y2c = g.y2b + NY; g.y2d = g.y2c + NY;
-    g.yhp = g.y2d + NY; g.yhm = g.yhp + NY; g.yhs = g.yhm + NY;
+    g.yhp = g.y2d + NY; g.yhm = g.yhp + NY; g.yhs = g.yhm + NY; g.ywp = g.yhs + NY; g.ywm = g.ywp + NY;
     g.z1a = G0 + ocz; g.z1b = g.z1a + NZ; g.z1c = g.z1b + NZ; g.z1d = g.z1c + NZ;
     g.z2a = g.z1d + NZ; g.z2b = g.z2a + NZ; g.z2c = g.z2b + NZ; g.z2d = g.z2c + NZ;
-    g.zhp = g.z2d + NZ; g.zhm = g.zhp + NZ; g.zhs = g.zhm + NZ;
+    g.zhp = g.z2d + NZ; g.zhm = g.zhp + NZ; g.zhs = g.zhm + NZ; g.zwp = g.zhs + NZ; g.zwm = g.zwp + NZ;
     g.xc = G0 + oX; g.yc = G0 + oY; g.zc = G0 + oZ;
     g.dxs = G0 + oDX; g.dys = G0 + oDY; g.dzs = G0 + oDZ;
     g.cbx = G0 + ocb;
@@ -1116,6 +1605,8 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     HCHK(hipMemset(h->dtv, 0, 8 * sizeof(double)));
     h->xd = h->xdA;
     h->cur = h->cA;
+    if (const char *e = getenv("MHD_KC")) h->kc = std::max(1, atoi(e));
+    if (const char *e = getenv("MHD_NAIVE")) h->naive = atoi(e) != 0;
     h->pt.resize(PT_N);
     for (int t = 0; t < PT_N; t++) h->pt[t] = Prof{PT_NAMES[t], 0.0, 0};
     *out = h;
@@ -1342,6 +1833,37 @@ int mhd_kernel_times(mhd_handle *h, int max, const char **names, double *ms, int
         if (calls) calls[t] = h->pt[t].calls;
     }
     return n;
+}
+
+int mhd_copy_rate(long bytes, int iters, double *bytes_per_s)
+{
+    if (bytes < (1 << 20) || iters < 1 || !bytes_per_s) return MHD_ERR_ARG;
+    long n2 = bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n2 * 16) != hipSuccess) return MHD_ERR_NOMEM;
+    if (hipMalloc(&b, n2 * 16) != hipSuccess) { (void)hipFree(a); return MHD_ERR_NOMEM; }
+    hipStream_t st;
+    if (hipStreamCreate(&st) != hipSuccess) { (void)hipFree(a); (void)hipFree(b); return MHD_ERR_HIP; }
+    (void)hipMemsetAsync(a, 0, n2 * 16, st);
+    int nb = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, st, n2, a, b);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, st);
+    for (int t = 0; t < iters; t++) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, st, n2, a, b);
+    (void)hipEventRecord(e1, st);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess || ms <= 0) return MHD_ERR_HIP;
+    *bytes_per_s = 2.0 * 16.0 * n2 * iters / (ms * 1e-3);
+    return MHD_OK;
 }
 
 const char *mhd_last_error(const mhd_handle *h) { return h ? h->err.c_str() : "null handle"; }

@@ -25,8 +25,8 @@ ERRORS = {-1: "bad argument", -2: "HIP error", -3: "out of device memory",
 EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state",
            "mhd_init_residual", "mhd_compute_dt", "mhd_step", "mhd_advance", "mhd_get_current",
            "mhd_diagnostics", "mhd_get_divb", "mhd_rhs", "mhd_get_xdif", "mhd_get_residual",
-           "mhd_set_residual", "mhd_set_profiling", "mhd_kernel_times", "mhd_last_error",
-           "mhd_destroy")
+           "mhd_set_residual", "mhd_set_profiling", "mhd_kernel_times", "mhd_copy_rate",
+           "mhd_last_error", "mhd_destroy")
 
 
 class MHDConfig(C.Structure):
@@ -76,11 +76,22 @@ def load_library(path: str = LIBPATH):
     L.mhd_rhs.argtypes = [P, C.c_double]
     L.mhd_set_profiling.argtypes = [P, I]
     L.mhd_kernel_times.argtypes = [P, I, C.POINTER(C.c_char_p), D, C.POINTER(I)]
+    L.mhd_copy_rate.argtypes = [C.c_long, I, D]
     L.mhd_last_error.argtypes = [P]
     L.mhd_last_error.restype = C.c_char_p
     L.mhd_destroy.argtypes = [P]
     _lib = L
     return L
+
+
+def copy_rate(nbytes: int = 2 << 30, iters: int = 10) -> float:
+    """read+write bytes/s of a plain device copy (library kernel)"""
+    L = load_library()
+    v = C.c_double()
+    rc = L.mhd_copy_rate(int(nbytes), int(iters), C.byref(v))
+    if rc != MHD_OK:
+        raise MHDError(rc, "copy probe failed")
+    return v.value
 
 
 def _p(a: np.ndarray):

@@ -99,8 +99,8 @@ int mhd_get_divb(mhd_handle *h, double *divb);
 
 /* stage-level access for parity checks of the RHS (right, t3dsdsc.f:1297-1673):
  * mhd_rhs runs one right(t) on the current state (t == 0 skips the final
- * smoothing as t3dsdsc.f:1669 does); the getters read the arrays right()
- * leaves behind: xdif (8 fields), efld and xdh (3 fields, smoothed). */
+ * smoothing as t3dsdsc.f:1669 does); mhd_get_xdif reads the xdif array
+ * (8 fields) right() leaves behind; xresd is the frozen momentum residual. */
 int mhd_rhs(mhd_handle *h, double t);
 int mhd_get_xdif(mhd_handle *h, double *xdif);
 int mhd_get_residual(mhd_handle *h, double *xresd);
@@ -110,6 +110,10 @@ int mhd_set_residual(mhd_handle *h, const double *xresd);
  * when enabled (names are stable strings; returns the count written) */
 int mhd_set_profiling(mhd_handle *h, int on);
 int mhd_kernel_times(mhd_handle *h, int max, const char **names, double *ms, int *calls);
+
+/* measured read+write rate of a plain device copy of `bytes` (no handle;
+ * the reference point for achieved-bandwidth figures) */
+int mhd_copy_rate(long bytes, int iters, double *bytes_per_s);
 
 const char *mhd_last_error(const mhd_handle *h);
 void mhd_destroy(mhd_handle *h);
