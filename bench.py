@@ -36,7 +36,9 @@ def per_cell_bytes(case):
     return {
         "curl": 8 * 9,                                  # B 3 + B_i 3 in, J 3 out
         "smooth3": 8 * 6,                               # 3 in, 3 out
+        "smooth3_eighth": 8 * 6,
         "smooth8": 8 * 16,                              # 8 in, 8 out
+        "smooth8_post": 8 * 16,
         "rhs_flux_E_hall": 8 * (15 + 4 + 3 + (3 if hall else 0)),   # U8,p_i,J3,xresd3 in
         "rhs_curlE_pressure": 8 * ((6 if hall else 3) + 5 + 3 + 4),  # E(+xdh), rho,m,p, J in; 4 out
         "rk_update": 8 * 32,                            # stages 1-3: xdif,xfold,xm in; xm,x out
