@@ -1,0 +1,170 @@
+! mhd_driver.f90 -- Fortran host program for the MI355X step.
+!
+! Keeps the time loop, guards and outputs of the reference's program main
+! (t3dsdsc.f:26-207) and calls the GPU step through the C ABI:
+!   setdt  -> mhd_compute_dt      stepon -> mhd_step
+!   diagn  -> mhd_diagnostics     recrd  -> mhd_get_state + restart record
+! Inputs in the run directory (written by dasha_amd.rundir):
+!   mhd.nml      namelist /run/ (grid sizes, physics flags and constants,
+!                nstop, nst, ncase, dtime, dnstep)
+!   grid.bin     xx, yy, zz, dx, dy, dz
+!   equil.bin    rhoi, bxi, byi, bzi, pri
+!   mhd<nst><000> restart record with the initial state (readin format)
+! Outputs: fort.20 (dt, time, nstep per step, t3dsdsc.f:180-181),
+! data/stepnm (nstep, time, dt, t3dsdsc.f:157-169), current.dat (diagn,
+! t3dsdsc.f:1228-1230), restart files mhd<nst><000> (recrd) and, at the
+! end of the run, the final state in the same format as mhdfinal.  Exit
+! status: 0 done, 2 dt collapse (t3dsdsc.f:142-154), 3 current blow-up
+! (t3dsdsc.f:1232), 1 any other error.  The reference's other snapshot
+! writers (recrd2/recrd3 plot files) are not part of the step and are not
+! written.
+program mhd_driver
+  use iso_c_binding
+  use mhd_gpu
+  implicit none
+  integer :: mx, my, mz, nstop, nst, ncase, dnstep, nend, nint
+  logical :: periody, hall, whistler
+  real(c_double) :: di, eta0, fmu0, frho0, fpr0, gamma, cfl, dtime
+  namelist /run/ mx, my, mz, periody, hall, whistler, di, eta0, fmu0, frho0, fpr0, gamma, cfl, &
+       nstop, nst, ncase, dtime, dnstep, nend, nint
+  type(mhd_config) :: cfg
+  type(mhd_diag) :: dg
+  type(c_ptr) :: h
+  real(c_double), allocatable :: xx(:), yy(:), zz(:), dx(:), dy(:), dz(:)
+  real(c_double), allocatable :: rhoi(:,:,:), bxi(:,:,:), byi(:,:,:), bzi(:,:,:), pri(:,:,:)
+  real(c_double), allocatable :: x(:,:,:,:)
+  real(c_double) :: time, dt, dt0, timein
+  integer :: nstep, nstin, rc, u, ios, ncase_in, nst_in
+  ! step at which output file nst was written (common /run/ nstp, mhd01.for:28)
+  integer :: nstp(0:10000)
+  logical :: exist
+
+  periody = .true.; hall = .true.; whistler = .true.
+  di = 0; eta0 = 0; fmu0 = 0; frho0 = 0; fpr0 = 0; gamma = 5.d0/3.d0; cfl = 0.4d0
+  nstop = 10; nst = 1; ncase = 1; dtime = 1.d0; dnstep = 10; nend = 5011; nint = 1
+  nstp = 0
+  open(newunit=u, file='mhd.nml', status='old', action='read', iostat=ios)
+  if (ios /= 0) call fail('cannot open mhd.nml', 1)
+  read(u, nml=run)
+  close(u)
+
+  allocate(xx(mx+4), yy(my), zz(mz+4), dx(mx+4), dy(my), dz(mz+4))
+  allocate(rhoi(mx+4,my,mz+4), bxi(mx+4,my,mz+4), byi(mx+4,my,mz+4), bzi(mx+4,my,mz+4), &
+       pri(mx+4,my,mz+4), x(mx+4,my,mz+4,8))
+  open(newunit=u, file='grid.bin', form='unformatted', status='old', action='read')
+  read(u) xx
+  read(u) yy
+  read(u) zz
+  read(u) dx
+  read(u) dy
+  read(u) dz
+  close(u)
+  open(newunit=u, file='equil.bin', form='unformatted', status='old', action='read')
+  read(u) rhoi
+  read(u) bxi
+  read(u) byi
+  read(u) bzi
+  read(u) pri
+  close(u)
+  ! readin (t3dsdsc.f:4822-4828)
+  open(newunit=u, file='mhd'//cn(nst)//cn(0), form='unformatted', status='old', action='read')
+  read(u) ncase_in, nstep, time, nst_in
+  read(u) x
+  close(u)
+
+  cfg%mx = mx; cfg%my = my; cfg%mz = mz
+  cfg%periodic_y = merge(1, 0, periody); cfg%hall = merge(1, 0, hall)
+  cfg%whistler_dt = merge(1, 0, whistler)
+  cfg%di = di; cfg%eta0 = eta0; cfg%fmu0 = fmu0; cfg%frho0 = frho0; cfg%fpr0 = fpr0
+  cfg%gamma = gamma; cfg%cfl = cfl
+  rc = mhd_create(cfg, xx, yy, zz, dx, dy, dz, h)
+  if (rc /= MHD_OK) call fail('mhd_create failed', 1)
+  call check(mhd_set_equilibrium(h, rhoi, bxi, byi, bzi, pri))
+  call check(mhd_set_state(h, x))
+  ! initia's frozen residual (t3dsdsc.f:816-834)
+  call check(mhd_init_residual(h))
+  call check(mhd_compute_dt(h, dt))
+  dt0 = dt
+  timein = time
+  nstin = nst
+  inquire(file='data/.', exist=exist)
+  if (.not. exist) call execute_command_line('mkdir -p data')
+
+  do
+     ! setdt (t3dsdsc.f:139-140)
+     call check(mhd_compute_dt(h, dt))
+     write(*,*) 'main', nstep, time, dt
+     if (dt < 0.1d0*dt0) then
+        ! t3dsdsc.f:142-154: snapshot and stop
+        nst = nst + nint
+        call recrd()
+        call mhd_destroy(h)
+        stop 2
+     end if
+     open(newunit=u, file='data/stepnm', position='append', action='write')
+     write(u,*) nstep, time, dt
+     close(u)
+     call check(mhd_step(h, time, dt))
+     ! diagn every 10 steps inside stepon (t3dsdsc.f:1174-1182)
+     if (mod(nstep, 10) == 0) then
+        rc = mhd_diagnostics(h, dg)
+        open(newunit=u, file='current.dat', position='append', action='write')
+        write(u,'(7(1x,e13.4))') dg%cxmax, dg%cxmin, dg%cymax, dg%cymin, dg%czmax, dg%czmin, time
+        close(u)
+        if (rc == MHD_ERR_BLOWUP) then
+           call mhd_destroy(h)
+           stop 3
+        end if
+        if (rc /= MHD_OK) call fail(trim(mhd_last_error(h)), 1)
+     end if
+     nstep = nstep + 1
+     time = time + dt
+     write(20,'(2(1x,f10.5),1x,i5)') dt, time, nstep
+     ! output cadence (t3dsdsc.f:185-199)
+     if (abs(time - timein - (nst - nstin + 1)*dtime) <= dt .and. (nstep - nstp(nst)) >= dnstep) then
+        nstp(nst) = nstep
+        nst = nst + nint
+        if (mod(nst, 3) == 1) call recrd()
+     end if
+     if (nstep > nstop) exit
+     if (nst >= nend) exit
+  end do
+  ! final state, in the restart format, under its own name
+  call check(mhd_get_state(h, x))
+  open(newunit=u, file='mhdfinal', form='unformatted', status='replace', action='write')
+  write(u) ncase, nstep, time, nst
+  write(u) x
+  close(u)
+  call mhd_destroy(h)
+
+contains
+
+  subroutine recrd()
+    ! t3dsdsc.f:4840-4861
+    integer :: v
+    call check(mhd_get_state(h, x))
+    open(newunit=v, file='mhd'//cn(nst)//cn(0), form='unformatted', status='replace', action='write')
+    write(v) ncase, nstep, time, nst
+    write(v) x
+    close(v)
+  end subroutine recrd
+
+  subroutine check(r)
+    integer(c_int), intent(in) :: r
+    if (r /= MHD_OK) call fail(trim(mhd_last_error(h)), 1)
+  end subroutine check
+
+  subroutine fail(msg, code)
+    character(len=*), intent(in) :: msg
+    integer, intent(in) :: code
+    write(0,*) 'mhd_driver: ', msg, ' (status', code, ')'
+    stop 1
+  end subroutine fail
+
+  character(len=3) function cn(n)
+    ! t3dsdsc.f:5416-5436
+    integer, intent(in) :: n
+    write(cn, '(i3.3)') n
+  end function cn
+
+end program mhd_driver

@@ -1,0 +1,79 @@
+"""Run-directory files in the reference's formats, for the Fortran driver.
+
+* restart files ``mhd<nst:3><rank:3>``: Fortran unformatted sequential,
+  record 1 = ``ncase, nstep, time, nst`` (int32, int32, float64, int32),
+  record 2 = ``x(mxpr, my, mzpr, 8)`` float64 (t3dsdsc.f:4840-4861 recrd,
+  4814-4836 readin, 5416-5436 cn);
+* ``grid.bin``: one record each of xx, yy, zz, dx, dy, dz (the arrays the
+  reference builds in input/gridpntnew2, t3dsdsc.f:248-313);
+* ``equil.bin``: one record each of rhoi, bxi, byi, bzi, pri (initia,
+  t3dsdsc.f:806-814);
+* ``mhd.nml``: namelist /run/ with the parameters the reference takes from
+  mhd01.for / mhd03.for and the main-loop constants (t3dsdsc.f:88-92).
+
+Record markers are 4-byte little-endian lengths (the flang and gfortran
+default).
+"""
+from __future__ import annotations
+
+import os
+import struct
+
+import numpy as np
+
+
+def _write_record(f, payload: bytes):
+    n = struct.pack("<i", len(payload))
+    f.write(n + payload + n)
+
+
+def _read_record(f) -> bytes:
+    (n,) = struct.unpack("<i", f.read(4))
+    data = f.read(n)
+    (m,) = struct.unpack("<i", f.read(4))
+    if m != n:
+        raise ValueError("corrupt Fortran record")
+    return data
+
+
+def cn(n: int) -> str:
+    """three-digit file number (t3dsdsc.f:5416-5436)"""
+    if not 0 <= n <= 999:
+        raise ValueError("file numbers run 0..999")
+    return f"{n:03d}"
+
+
+def write_restart(path: str, x: np.ndarray, *, ncase: int, nstep: int, time: float, nst: int):
+    with open(path, "wb") as f:
+        _write_record(f, struct.pack("<iidi", ncase, nstep, time, nst))
+        _write_record(f, np.ascontiguousarray(x, dtype="<f8").tobytes())
+
+
+def read_restart(path: str, shape):
+    with open(path, "rb") as f:
+        ncase, nstep, time, nst = struct.unpack("<iidi", _read_record(f))
+        x = np.frombuffer(_read_record(f), dtype="<f8").reshape(shape).copy()
+    return dict(ncase=ncase, nstep=nstep, time=time, nst=nst), x
+
+
+def write_case(rundir: str, case, *, nstop: int, nst: int = 1, ncase: int = 1, dtime: float = 1.0,
+               dnstep: int = 10):
+    """Write grid.bin, equil.bin, mhd.nml and the initial restart file."""
+    os.makedirs(rundir, exist_ok=True)
+    with open(os.path.join(rundir, "grid.bin"), "wb") as f:
+        for a in (case.xx, case.yy, case.zz, case.dx, case.dy, case.dz):
+            _write_record(f, np.ascontiguousarray(a, dtype="<f8").tobytes())
+    with open(os.path.join(rundir, "equil.bin"), "wb") as f:
+        for a in (case.rhoi, case.bxi, case.byi, case.bzi, case.pri):
+            _write_record(f, np.ascontiguousarray(a, dtype="<f8").tobytes())
+    b = lambda v: ".true." if v else ".false."
+    with open(os.path.join(rundir, "mhd.nml"), "w") as f:
+        f.write("&run\n")
+        f.write(f" mx={case.mx}, my={case.my}, mz={case.mz},\n")
+        f.write(f" periody={b(case.periodic_y)}, hall={b(case.hall)}, whistler={b(case.whistler_dt)},\n")
+        f.write(f" di={case.di!r}, eta0={case.eta0!r}, fmu0={case.fmu0!r}, frho0={case.frho0!r}, "
+                f"fpr0={case.fpr0!r},\n")
+        f.write(f" gamma={case.gamma!r}, cfl={case.cfl!r},\n")
+        f.write(f" nstop={nstop}, nst={nst}, ncase={ncase}, dtime={dtime!r}, dnstep={dnstep}\n/\n")
+    write_restart(os.path.join(rundir, "mhd" + cn(nst) + cn(0)), case.x0, ncase=ncase, nstep=0,
+                  time=0.0, nst=nst)

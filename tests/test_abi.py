@@ -1,0 +1,139 @@
+"""The C ABI (include/mhd_gpu.h) without a GPU: the library loads, exports
+every declared entry point, and the struct layouts agree between the C
+header, the ctypes mirror and the Fortran bind(C) module.  No compute
+call is made here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from dasha_amd import mhd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mhd_gpu.h")
+
+
+def _declared():
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    return set(re.findall(r"\b(mhd_\w+)\s*\(", text))
+
+
+def _lib():
+    if not os.path.exists(mhd.LIBPATH):
+        pytest.skip("libmhd_gpu.so not built")
+    return mhd.load_library()
+
+
+def test_header_matches_python_export_list():
+    assert _declared() == set(mhd.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib()
+    missing = [n for n in sorted(_declared()) if not hasattr(L, n)]
+    assert not missing
+    nm = subprocess.run(["nm", "-D", "--defined-only", mhd.LIBPATH], capture_output=True,
+                        text=True, check=True).stdout
+    dyn = set(re.findall(r"\sT\s(mhd_\w+)$", nm, flags=re.M))
+    assert _declared() <= dyn
+    # the C ABI exports only plain C names
+    assert not [n for n in dyn if n.startswith("_Z") and "mhd_" in n]
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    saved = mhd._lib
+    mhd._lib = None
+    try:
+        with pytest.raises(RuntimeError, match="missing"):
+            mhd.load_library(str(tmp_path / "nope.so"))
+    finally:
+        mhd._lib = saved
+
+
+_C_LAYOUT = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "mhd_gpu.h"
+#define F(T, m) printf(#T " " #m " %zu\n", offsetof(T, m))
+int main(void) {
+  printf("mhd_config size %zu\nmhd_diag size %zu\n", sizeof(mhd_config), sizeof(mhd_diag));
+  F(mhd_config, mx); F(mhd_config, my); F(mhd_config, mz); F(mhd_config, periodic_y);
+  F(mhd_config, hall); F(mhd_config, whistler_dt); F(mhd_config, di); F(mhd_config, eta0);
+  F(mhd_config, fmu0); F(mhd_config, frho0); F(mhd_config, fpr0); F(mhd_config, gamma);
+  F(mhd_config, cfl);
+  F(mhd_diag, cxmax); F(mhd_diag, cxmin); F(mhd_diag, cymax); F(mhd_diag, cymin);
+  F(mhd_diag, czmax); F(mhd_diag, czmin); F(mhd_diag, e_kin); F(mhd_diag, e_mag);
+  F(mhd_diag, e_th); F(mhd_diag, e_mag_pert); F(mhd_diag, divb_max); F(mhd_diag, divb_rms);
+  return 0;
+}
+"""
+
+
+def test_struct_layout_c_vs_ctypes(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(_C_LAYOUT)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = {}
+    for ln in out.splitlines():
+        t, m, v = ln.split()
+        got[(t, m)] = int(v)
+    assert got[("mhd_config", "size")] == C.sizeof(mhd.MHDConfig)
+    assert got[("mhd_diag", "size")] == C.sizeof(mhd.MHDDiag)
+    for name, _ in mhd.MHDConfig._fields_:
+        assert got[("mhd_config", name)] == getattr(mhd.MHDConfig, name).offset, name
+    for name, _ in mhd.MHDDiag._fields_:
+        assert got[("mhd_diag", name)] == getattr(mhd.MHDDiag, name).offset, name
+
+
+_F_LAYOUT = """
+program layout
+  use iso_c_binding
+  use mhd_gpu
+  implicit none
+  type(mhd_config), target :: c
+  type(mhd_diag), target :: d
+  print '(a,i0)', 'config ', c_sizeof(c)
+  print '(a,i0)', 'diag ', c_sizeof(d)
+  print '(a,i0)', 'di ', transfer(c_loc(c%di), 0_c_intptr_t) - transfer(c_loc(c), 0_c_intptr_t)
+  print '(a,i0)', 'cfl ', transfer(c_loc(c%cfl), 0_c_intptr_t) - transfer(c_loc(c), 0_c_intptr_t)
+  print '(a,i0)', 'divb_rms ', transfer(c_loc(d%divb_rms), 0_c_intptr_t) - &
+       transfer(c_loc(d), 0_c_intptr_t)
+end program layout
+"""
+
+
+def test_struct_layout_fortran_module(tmp_path):
+    from dasha_amd import build
+    fc = build.FLANG if os.path.exists(build.FLANG) else None
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    _lib()
+    mod = os.path.join(ROOT, "dasha_amd", "fortran", "mhd_gpu_mod.f90")
+    src = tmp_path / "layout.f90"
+    src.write_text(_F_LAYOUT)
+    exe = tmp_path / "layout"
+    libdir = os.path.dirname(mhd.LIBPATH)
+    subprocess.run([fc, "-J", str(tmp_path), mod, str(src), "-o", str(exe), f"-L{libdir}",
+                    "-lmhd_gpu", f"-Wl,-rpath,{libdir}"], check=True, cwd=str(tmp_path))
+    out = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                   check=True).stdout.splitlines())
+    assert int(out["config"]) == C.sizeof(mhd.MHDConfig)
+    assert int(out["diag"]) == C.sizeof(mhd.MHDDiag)
+    assert int(out["di"]) == mhd.MHDConfig.di.offset
+    assert int(out["cfl"]) == mhd.MHDConfig.cfl.offset
+    assert int(out["divb_rms"]) == mhd.MHDDiag.divb_rms.offset
+
+
+def test_product_never_imports_the_oracle():
+    """the package under dasha_amd/ has no route to the CPU oracle"""
+    pkg = os.path.join(ROOT, "dasha_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".f90")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert not re.search(r"\boracle\b|mhd_oracle", text), f

@@ -1,0 +1,97 @@
+"""The Fortran host program (dasha_amd/fortran/mhd_driver.f90) over the C ABI.
+
+CPU tests cover the run-directory formats and the driver's input side: on a
+machine without a GPU the driver reads mhd.nml, grid.bin, equil.bin and the
+restart record, then stops at mhd_create with its "mhd_create failed"
+message (any format mismatch stops it earlier with a runtime I/O error).
+The GPU test runs the driver for 12 steps and requires its final state to
+be bit-identical to the same setdt/stepon sequence driven from Python
+through the same library, with the fort.20 and current.dat cadence of
+t3dsdsc.f:171-181 and 1174-1182.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from dasha_amd import harris, rundir
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "dasha_amd", "fortran", "mhd_driver")
+LIB = os.path.join(ROOT, "dasha_amd", "libmhd_gpu.so")
+
+
+def _driver():
+    if not os.path.exists(DRIVER):
+        if not os.path.exists(LIB):
+            pytest.skip("libmhd_gpu.so not built")
+        from dasha_amd import build
+        if build.build_fortran() is None:
+            pytest.skip("no Fortran compiler")
+    return DRIVER
+
+
+def test_restart_record_roundtrip(tmp_path):
+    x = np.random.default_rng(0).standard_normal((8, 6, 5, 7))
+    p = str(tmp_path / ("mhd" + rundir.cn(3) + rundir.cn(0)))
+    rundir.write_restart(p, x, ncase=2, nstep=17, time=1.25, nst=3)
+    assert os.path.basename(p) == "mhd003000"
+    # record 1 is 20 bytes between 4-byte markers, record 2 the raw array
+    assert os.path.getsize(p) == 4 + 20 + 4 + 4 + x.nbytes + 4
+    hdr, y = rundir.read_restart(p, x.shape)
+    assert hdr == dict(ncase=2, nstep=17, time=1.25, nst=3)
+    assert np.array_equal(x, y)
+
+
+def test_cn_range():
+    assert rundir.cn(0) == "000" and rundir.cn(999) == "999"
+    with pytest.raises(ValueError):
+        rundir.cn(1000)
+
+
+def test_driver_reads_run_directory(tmp_path):
+    drv = _driver()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present: covered by the GPU test")
+    except ImportError:
+        pass
+    case = harris.case_small(12, 4, 12)
+    rundir.write_case(str(tmp_path), case, nstop=3)
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="")
+    r = subprocess.run([drv], cwd=str(tmp_path), capture_output=True, text=True, timeout=120,
+                       env=env)
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert "mhd_create failed" in r.stderr, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_matches_python_loop(tmp_path):
+    from dasha_amd.mhd import MHDStep
+    drv = _driver()
+    case = harris.case_small(24, 8, 24)
+    nstop = 11                       # the loop exits once nstep > nstop: 12 steps
+    rundir.write_case(str(tmp_path), case, nstop=nstop, dtime=1e9)
+    print(f"[driver] case={case.name} steps={nstop + 1}", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "110", drv], cwd=str(tmp_path),
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    hdr, xf = rundir.read_restart(str(tmp_path / "mhdfinal"), case.x0.shape)
+    assert hdr["nstep"] == nstop + 1
+
+    g = MHDStep(case)
+    t, dts = g.run(nstop + 1)
+    xg = g.get_state()
+    # x(mx+4, my, mz+4, 8) in Fortran order is the C array [8][mz+4][my][mx+4]
+    assert np.array_equal(xf, xg)
+    assert hdr["time"] == t
+
+    lines = (tmp_path / "fort.20").read_text().split("\n")
+    rows = [ln.split() for ln in lines if ln.strip()]
+    assert [int(r_[2]) for r_ in rows] == list(range(1, nstop + 2))
+    assert np.allclose([float(r_[0]) for r_ in rows], dts, atol=6e-6)
+    cur = [ln.split() for ln in (tmp_path / "current.dat").read_text().split("\n") if ln.strip()]
+    assert len(cur) == 2                              # nstep 0 and 10
+    assert all(len(c) == 7 for c in cur)
