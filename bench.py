@@ -30,21 +30,36 @@ B_ALG_NOHALL = 5104         # d_i = 0
 
 
 def per_cell_bytes(case):
-    """algorithmic HBM bytes per interior cell of one launch of each kernel
-    family (each input field read once, each output written once)"""
+    """algorithmic HBM bytes per physical cell of one launch of each kernel
+    family: every input field read once, every output field written once"""
     hall = case.hall
     return {
-        "curl": 8 * 9,                                  # B 3 + B_i 3 in, J 3 out
-        "smooth3": 8 * 6,                               # 3 in, 3 out
-        "smooth3_eighth": 8 * 6,
-        "smooth8": 8 * 16,                              # 8 in, 8 out
-        "smooth8_post": 8 * 16,
-        "rhs_flux_E_hall": 8 * (15 + 4 + 3 + (3 if hall else 0)),   # U8,p_i,J3,xresd3 in
-        "rhs_curlE_pressure": 8 * ((6 if hall else 3) + 5 + 3 + 4),  # E(+xdh), rho,m,p, J in; 4 out
-        "rk_update": 8 * 32,                            # stages 1-3: xdif,xfold,xm in; xm,x out
+        "curl": 8 * (6 + 3),                            # B 3 + B_i 3 in, J 3 out
+        "smooth3": 8 * (3 + 3),                         # avrgc / avrgE pass
+        "smooth3_eighth": 8 * (3 + 3),                  # avrgh pass
+        "smooth8": 8 * (8 + 8),                         # avrg pass 1
+        # avrg pass 2 fused with the RK register update, mean of the 4 stages:
+        # (xdif+xfold -> xm+x), 2 x (xdif+xfold+xm -> xm+x), (xdif+xm -> x)
+        "rk_update": 8 * (32 + 40 + 40 + 24) // 4,
+        "rhs_flux_E_hall": 8 * (15 + 10) if hall else 8 * (15 + 7),   # U8,p_i,J3,xresd3 -> xdif1-4,E3(,xdh3)
+        "rhs_curlE_pressure": 8 * ((18 if hall else 15) + 8),         # E3(,xdh3),U5,J3,xdif1-4 -> xdif8
+        "smooth8_post": 8 * ((8 + 5 + 8) + (8 + 8)) // 2,             # mean of the two avrgtt passes
         "post_add_eq": 8 * (8 + 5 + 8),
         "setdt": 8 * 8,
     }
+
+
+def traffic_from_profiles(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE scaled by the copy-kernel factors), when a
+    committed profile summary for this workload exists"""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes_per_launch"]
 
 
 def cpu_baseline(case_name: str, steps: int, threads: int):
@@ -76,7 +91,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event run")
     ap.add_argument("--copy-probe", action="store_true", help="also measure a plain device copy")
     args = ap.parse_args()
@@ -104,6 +119,15 @@ def main():
     print(f"[bench] timed {args.steps} steps: {ms:.1f} ms ({ms_per_step:.3f} ms/step)",
           file=sys.stderr, flush=True)
 
+    # the state crossing PCIe (restart / output points only, never per step)
+    t0 = time.perf_counter()
+    xs = g.get_state()
+    t1 = time.perf_counter()
+    g.set_state(xs)
+    t2 = time.perf_counter()
+    host_io = {"state_bytes": xs.nbytes, "get_state_ms": (t1 - t0) * 1e3, "set_state_ms": (t2 - t1) * 1e3}
+    del xs
+
     # per-kernel-family times (events around every launch) on a short run
     roof = None
     fam = {}
@@ -113,18 +137,15 @@ def main():
         fam = g.kernel_times()
         g.set_profiling(False)
         pcb = per_cell_bytes(case)
-        interior = (case.mx - 4) * (case.my - 4) * (case.mz - 4)
-        ext = case.mx * case.my * case.mz
         best = max(fam.items(), key=lambda kv: kv[1][0])
         name, (tot_ms, calls) = best
-        cells = ext if name in ("rk_update", "post_add_eq", "rhs_flux_E_hall") else interior
         if name in pcb and calls > 0:
             avg_s = tot_ms / calls / 1e3
-            ach = pcb[name] * cells / avg_s
+            alg = pcb[name] * case.cells
+            ach = alg / avg_s
             roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                    "frac": ach / HBM_PEAK, "traffic": None, "kernel": name,
-                    "avg_launch_ms": tot_ms / calls,
-                    "alg_bytes_per_launch": pcb[name] * cells}
+                    "frac": ach / HBM_PEAK, "traffic": traffic_from_profiles(name, args.config),
+                    "kernel": name, "avg_launch_ms": tot_ms / calls, "alg_bytes_per_launch": alg}
     b_alg = B_ALG_HALL if case.hall else B_ALG_NOHALL
     out = {
         "metric": "FP64 cell-updates/s per RK4 step",
@@ -147,6 +168,7 @@ def main():
                           "achieved_GBps": value * b_alg / 1e9,
                           "frac": value * b_alg / HBM_PEAK},
         "kernel_families_ms": {k: {"total_ms": v[0], "calls": v[1]} for k, v in fam.items()},
+        "host_io": host_io,
         "cpu_baseline": None,
     }
     if args.copy_probe:

@@ -31,7 +31,9 @@ struct Geo {
     const double *xc, *yc, *zc;          // coordinates
     const double *dxs, *dys, *dzs;       // setdt spacings
     const double *cbx;                   // B-component smoothing coefficient per x
+    const double *crad;                  // radial smoothing coefficient per point (0 off the interior)
     double ax3, bx3, cx3, ax4, bx4, cx4; // axp at jx = 3, 4 (x-left extrapolation)
+    double dx3, dx4, rx3, rx4;           // their sums (ax+bx)+cx and the rounded reciprocals
     double axmL1, bxmL1, cxmL1, axmL, bxmL, cxmL; // axm at xlast-1, xlast
 };
 
@@ -39,6 +41,15 @@ __host__ __device__ inline long IDX(const Geo &g, int i, int j, int k)
 {
     return (long)i + g.sy * j + g.sz * k;
 }
+
+// element e of a field with a wave-uniform base: the byte offset fits 32
+// bits (fields hold fewer than 2^29 points, checked at create), so the access
+// can use a scalar base and a 32-bit vector offset
+__device__ inline const double &fld(const double *base, unsigned e)
+{
+    return *(const double *)((const char *)base + (e << 3));
+}
+__device__ inline double *fldp(double *base, unsigned e) { return (double *)((char *)base + (e << 3)); }
 
 // physical row of logical row j (periodic image map; identity otherwise)
 __device__ inline int rmap(const Geo &g, int j)
@@ -201,15 +212,35 @@ __device__ inline int xslot(const Geo &g, int i)
     return i == 2 ? 0 : i == 3 ? 1 : i == 4 ? 2 : i == g.NX - 3 ? 3 : -1;
 }
 
+// a / b correctly rounded from the rounded reciprocal y = 1/b: one Newton
+// correction of a*y with an exact FMA residual (Markstein); equal to the
+// IEEE quotient for every finite a, b without overflow or underflow
+__device__ inline double divc(double a, double b, double y)
+{
+    const double q0 = a * y;
+    return fma(fma(-q0, b, a), y, q0);
+}
+
+// x-left extrapolation of the auxiliary fills (bndryc/E/h, t3dsdsc.f:2569-2572
+// and the like): f(4) = (ax f5 + bx f6 + cx f7) / (ax + bx + cx), then f(3)
+__device__ inline double xleft1(const Geo &g, double p2, double p3, double p4)
+{
+    return divc(g.ax4 * p2 + g.bx4 * p3 + g.cx4 * p4, g.dx4, g.rx4);
+}
+__device__ inline double xleft0(const Geo &g, double x1, double p2, double p3)
+{
+    return divc(g.ax3 * x1 + g.bx3 * p2 + g.cx3 * p3, g.dx3, g.rx3);
+}
+
 template <int TYPE>
 __device__ inline double xrule(const Geo &g, int i, double p2, double p3, double p4, double pL)
 {
     if (i == g.NX - 1) return 0.0;
     if (i == g.NX - 2) return 0.5 * pL;
     if (TYPE == BC_STATE) return i == 0 ? 0.0 : 0.5 * p2;
-    double x1 = (g.ax4 * p2 + g.bx4 * p3 + g.cx4 * p4) / (g.ax4 + g.bx4 + g.cx4);
+    double x1 = xleft1(g, p2, p3, p4);
     if (i == 1) return x1;
-    return (g.ax3 * x1 + g.bx3 * p2 + g.cx3 * p3) / (g.ax3 + g.bx3 + g.cx3);
+    return xleft0(g, x1, p2, p3);
 }
 
 __device__ inline bool xface(const Geo &g, int i) { return i < 2 || i > g.NX - 3; }

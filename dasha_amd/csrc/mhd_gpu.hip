@@ -63,6 +63,19 @@ __global__ void k_add_eq(long S, long n, const double *__restrict__ XM, Eq eq, d
 
 __global__ void k_set(double *p, double v) { *p = v; }
 
+// radial smoothing coefficient of every interior point (geometry only), the
+// same expression the sweeps evaluate in place
+__global__ void k_crad(Geo g, double *__restrict__ c)
+{
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.S) return;
+    int i = (int)(q % g.NX);
+    long r = q / g.NX;
+    int j = (int)(r % g.NY), k = (int)(r / g.NY);
+    bool in = i >= 2 && i <= g.NX - 3 && j >= 2 && j <= g.NY - 3 && k >= 2 && k <= g.NZ - 3;
+    c[q] = in ? radial_coef(g.xc[i] * g.xc[i] + g.yc[j] * g.yc[j], g.zc[k]) : 0.0;
+}
+
 // plain streaming copy (reference rate for the roofline figures)
 __global__ void k_copy(long n, const double *__restrict__ a, double *__restrict__ b)
 {
@@ -313,6 +326,7 @@ struct mhd_handle {
     double *xd = nullptr;                  // buffer holding the last xdif
     double *cur = nullptr;                 // buffer holding the last current
     double *red = nullptr;                 // reduction scratch
+    double *crad = nullptr;                // radial smoothing coefficient table
     double *dtv = nullptr;                 // device dt slots
     int nred = 0;
     std::vector<double> wx, wy, wz;        // energy weights (host copy)
@@ -320,8 +334,8 @@ struct mhd_handle {
     std::string err;
     bool have_eq = false;
     int kc = 16;                           // z planes per tile chunk
-    int fpb = 1;                           // fields per block of the production sweep
-    int sweep1 = 0;                        // sweep variant (A/B): 0 z-unrolled, 1 per-field march, 2 point, 3 multi-field march
+    int fuse_rk = 1;                       // RK update fused into the last avrg pass
+    int zs = 1;                            // pairs of smoothing passes in one temporal-blocked kernel
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -451,68 +465,51 @@ static dim3 tile_grid(const mhd_handle *h)
     return dim3((unsigned)blocks(h->NX, TX), (unsigned)blocks(nyt, TY), (unsigned)blocks(h->NZ - 4, h->kc));
 }
 
-static int nchunks(const mhd_handle *h) { return blocks(h->NZ - 4, h->kc); }
-
-static dim3 z_grid(const mhd_handle *h, int nfblocks)
+static dim3 z_grid(const mhd_handle *h, int nfblocks, int uz)
 {
     int nyt = h->g.yhi - h->g.ylo + 1;
-    return dim3((unsigned)blocks(h->NX, TX), (unsigned)blocks(nyt, BY), (unsigned)(blocks(h->NZ, UZ) * nfblocks));
+    return dim3((unsigned)blocks(h->NX, TX), (unsigned)blocks(nyt, BY), (unsigned)(blocks(h->NZ, uz) * nfblocks));
 }
 
-static dim3 pt_grid(const mhd_handle *h)
+// one smoothing sweep a -> b (RKS < 0) or the last avrg pass of RK stage
+// RKS fused with its register update (b unused).  All NF fields in one
+// block and 4 planes per thread measured fastest on MI355X at 256x128x256
+// (against one field per block, 8 planes, XCD-contiguous block order).
+template <int NF, int KIND, int TYPE, int RKS>
+static void launch_sweep(mhd_handle *h, const double *a, double *b)
 {
-    int nyt = h->g.yhi - h->g.ylo + 1;
-    return dim3((unsigned)blocks(h->NX, TX), (unsigned)blocks(nyt, PY), (unsigned)h->NZ);
+    RKArgs rk{h->dtv, h->xfold, h->xm, h->x};
+    hipLaunchKernelGGL((k_sweepz<NF, KIND, TYPE, NF, SWEEP_UZ, RKS>), z_grid(h, 1, SWEEP_UZ), dim3(TX * BY), 0,
+                       h->stream, h->g, a, b, rk, 0);
 }
 
-static dim3 sweep_grid(const mhd_handle *h, int nf)
+// NP passes a -> b in one temporal-blocked kernel (one field per block);
+// with RKS >= 0 the last pass ends in RK stage RKS (b unused)
+template <int NF, int NP, int KIND, int TYPE, int RKS>
+static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
 {
-    int nyt = h->g.yhi - h->g.ylo + 1;
-    return dim3((unsigned)blocks(h->NX, TX), (unsigned)blocks(nyt, SY), (unsigned)(nchunks(h) * nf));
+    RKArgs rk{h->dtv, h->xfold, h->xm, h->x};
+    const int W = TX - 2 * NP, H = ZB - 2 * NP;
+    const int nyt = h->g.yhi - h->g.ylo + 1;
+    const int nch = blocks(h->NZ - 4, ZKC);
+    dim3 grid((unsigned)blocks(h->NX, W), (unsigned)blocks(nyt, H), (unsigned)(nch * NF));
+    hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, nch);
 }
 
 // kk smoothing sweeps (+ fills) on a 3-field array; returns the result buffer
 static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk)
 {
-    for (int p = 0; p < kk; p++) {
-        int tag = kind == KIND_RADIAL ? PT_FILT3 : PT_FILT3E;
+    const int tag = kind == KIND_RADIAL ? PT_FILT3 : PT_FILT3E;
+    while (kk > 0) {
         PROF_BEGIN(tag);
-        if (h->sweep1 == 0) {
-            if (h->fpb == 1) {
-                if (kind == KIND_RADIAL)
-                    hipLaunchKernelGGL((k_sweepz<3, KIND_RADIAL, BC_AUX, 1>), z_grid(h, 3), dim3(TX * BY), 0, h->stream,
-                                       h->g, a, b);
-                else
-                    hipLaunchKernelGGL((k_sweepz<3, KIND_EIGHTH, BC_AUX, 1>), z_grid(h, 3), dim3(TX * BY), 0, h->stream,
-                                       h->g, a, b);
-            } else {
-                if (kind == KIND_RADIAL)
-                    hipLaunchKernelGGL((k_sweepz<3, KIND_RADIAL, BC_AUX, 3>), z_grid(h, 1), dim3(TX * BY), 0, h->stream,
-                                       h->g, a, b);
-                else
-                    hipLaunchKernelGGL((k_sweepz<3, KIND_EIGHTH, BC_AUX, 3>), z_grid(h, 1), dim3(TX * BY), 0, h->stream,
-                                       h->g, a, b);
-            }
-        } else if (h->sweep1 == 2) {
-            if (kind == KIND_RADIAL)
-                hipLaunchKernelGGL((k_sweep_pt<3, KIND_RADIAL, BC_AUX>), pt_grid(h), dim3(TX * PY), 0, h->stream, h->g,
-                                   a, b);
-            else
-                hipLaunchKernelGGL((k_sweep_pt<3, KIND_EIGHTH, BC_AUX>), pt_grid(h), dim3(TX * PY), 0, h->stream, h->g,
-                                   a, b);
-        } else if (h->sweep1) {
-            if (kind == KIND_RADIAL)
-                hipLaunchKernelGGL((k_sweep<KIND_RADIAL, BC_AUX>), sweep_grid(h, 3), dim3(TX * SY), 0, h->stream, h->g,
-                                   a, b, h->kc, nchunks(h));
-            else
-                hipLaunchKernelGGL((k_sweep<KIND_EIGHTH, BC_AUX>), sweep_grid(h, 3), dim3(TX * SY), 0, h->stream, h->g,
-                                   a, b, h->kc, nchunks(h));
-        } else if (kind == KIND_RADIAL) {
-            hipLaunchKernelGGL((k_sweepn<3, KIND_RADIAL, BC_AUX>), sweep_grid(h, 1), dim3(TX * SY), 0, h->stream, h->g,
-                               a, b, h->kc);
+        if (h->zs && kk >= 2) {
+            if (kind == KIND_RADIAL) launch_zsmooth<3, 2, KIND_RADIAL, BC_AUX, -1>(h, a, b);
+            else launch_zsmooth<3, 2, KIND_EIGHTH, BC_AUX, -1>(h, a, b);
+            kk -= 2;
         } else {
-            hipLaunchKernelGGL((k_sweepn<3, KIND_EIGHTH, BC_AUX>), sweep_grid(h, 1), dim3(TX * SY), 0, h->stream, h->g,
-                               a, b, h->kc);
+            if (kind == KIND_RADIAL) launch_sweep<3, KIND_RADIAL, BC_AUX, -1>(h, a, b);
+            else launch_sweep<3, KIND_EIGHTH, BC_AUX, -1>(h, a, b);
+            kk -= 1;
         }
         PROF_END(tag);
         std::swap(a, b);
@@ -531,7 +528,8 @@ static void current(mhd_handle *h, const double *X)
 }
 
 // right (t3dsdsc.f:1297-1673) of state X; result in h->xd
-static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init)
+// with rks >= 0 the last sweep performs RK stage rks (nothing is left in h->xd)
+static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int rks)
 {
     const mhd_config &c = h->cfg;
     double eta = init ? 0.0 : c.eta0, fmu = init ? 0.0 : c.fmu0;
@@ -554,27 +552,36 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init)
                            c.gamma, fmu, frho, fpr, h->xdA, h->kc);
     PROF_END(PT_RHSB);
     double *a = h->xdA, *b = h->xdB;
-    if (smooth_xdif) {
-        for (int p = 0; p < 2; p++) {
-            PROF_BEGIN(PT_FILT8);
-            if (h->sweep1 == 0 && h->fpb == 1)
-                hipLaunchKernelGGL((k_sweepz<8, KIND_STATE, BC_STATE, 1>), z_grid(h, 8), dim3(TX * BY), 0, h->stream,
-                                   h->g, a, b);
-            else if (h->sweep1 == 0)
-                hipLaunchKernelGGL((k_sweepz<8, KIND_STATE, BC_STATE, 8>), z_grid(h, 1), dim3(TX * BY), 0, h->stream,
-                                   h->g, a, b);
-            else if (h->sweep1 == 2)
-                hipLaunchKernelGGL((k_sweep_pt<8, KIND_STATE, BC_STATE>), pt_grid(h), dim3(TX * PY), 0, h->stream,
-                                   h->g, a, b);
-            else if (h->sweep1)
-                hipLaunchKernelGGL((k_sweep<KIND_STATE, BC_STATE>), sweep_grid(h, 8), dim3(TX * SY), 0, h->stream,
-                                   h->g, a, b, h->kc, nchunks(h));
-            else
-                hipLaunchKernelGGL((k_sweepn<8, KIND_STATE, BC_STATE>), sweep_grid(h, 1), dim3(TX * SY), 0, h->stream,
-                                   h->g, a, b, h->kc);
-            PROF_END(PT_FILT8);
-            std::swap(a, b);
+    if (smooth_xdif && h->zs) {
+        // avrg twice (t3dsdsc.f:1668-1671) in one temporal-blocked kernel;
+        // with rks >= 0 it also performs the RK stage update
+        PROF_BEGIN(rks >= 0 ? PT_RK : PT_FILT8);
+        switch (rks) {
+        case 0: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 0>(h, a, nullptr); break;
+        case 1: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 1>(h, a, nullptr); break;
+        case 2: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 2>(h, a, nullptr); break;
+        case 3: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 3>(h, a, nullptr); break;
+        default: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, -1>(h, a, b); std::swap(a, b); break;
         }
+        PROF_END(rks >= 0 ? PT_RK : PT_FILT8);
+        if (rks >= 0) a = nullptr;
+    } else if (smooth_xdif) {
+        // avrg twice (t3dsdsc.f:1668-1671); with rks >= 0 the second pass
+        // also performs the RK stage update
+        PROF_BEGIN(PT_FILT8);
+        launch_sweep<8, KIND_STATE, BC_STATE, -1>(h, a, b);
+        PROF_END(PT_FILT8);
+        std::swap(a, b);
+        PROF_BEGIN(rks >= 0 ? PT_RK : PT_FILT8);
+        switch (rks) {
+        case 0: launch_sweep<8, KIND_STATE, BC_STATE, 0>(h, a, nullptr); break;
+        case 1: launch_sweep<8, KIND_STATE, BC_STATE, 1>(h, a, nullptr); break;
+        case 2: launch_sweep<8, KIND_STATE, BC_STATE, 2>(h, a, nullptr); break;
+        case 3: launch_sweep<8, KIND_STATE, BC_STATE, 3>(h, a, nullptr); break;
+        default: launch_sweep<8, KIND_STATE, BC_STATE, -1>(h, a, b); std::swap(a, b); break;
+        }
+        PROF_END(rks >= 0 ? PT_RK : PT_FILT8);
+        if (rks >= 0) a = nullptr;
     }
     h->xd = a;
 }
@@ -600,11 +607,14 @@ static void launch_bc_tt(mhd_handle *h, double *B, const double *A, const double
 static void step_body(mhd_handle *h)
 {
     std::swap(h->x, h->xfold);   // xfold = x (1079-1084) by renaming
-    rhs(h, h->xfold, true, false);
-    rk(h, 0);
-    for (int s = 1; s < 4; s++) {
-        rhs(h, h->x, true, false);
-        rk(h, s);
+    for (int s = 0; s < 4; s++) {
+        const double *X = s == 0 ? h->xfold : h->x;
+        if (h->fuse_rk) {
+            rhs(h, X, true, false, s);
+        } else {
+            rhs(h, X, true, false, -1);
+            rk(h, s);
+        }
     }
     // perturbation smoothing (1135-1165): xm = x - eq; avrgtt x2; x = xm + eq
     Eq sub = eq_state(h, true);
@@ -682,6 +692,8 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     if (!cfg || !xx || !yy || !zz || !dx || !dy || !dz || !out) return MHD_ERR_ARG;
     *out = nullptr;
     if (cfg->mx < 8 || cfg->mz < 8 || cfg->my < 8) return MHD_ERR_ARG;
+    // one field must stay below 2^29 points (32-bit byte offsets inside a field)
+    if ((long)cfg->mx * cfg->my * cfg->mz >= (1L << 29)) return MHD_ERR_ARG;
     if (!(cfg->gamma > 1.0) || !(cfg->cfl > 0.0)) return MHD_ERR_ARG;
     if (cfg->periodic_y && cfg->my < 8) return MHD_ERR_ARG;
     {
@@ -770,6 +782,8 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     h->dwx = h->geo + owx; h->dwy = h->geo + owy; h->dwz = h->geo + owz;
     coef_side(X.data(), 0, +1, &g.ax3, &g.bx3, &g.cx3);
     coef_side(X.data(), 1, +1, &g.ax4, &g.bx4, &g.cx4);
+    g.dx3 = g.ax3 + g.bx3 + g.cx3; g.rx3 = 1.0 / g.dx3;
+    g.dx4 = g.ax4 + g.bx4 + g.cx4; g.rx4 = 1.0 / g.dx4;
     coef_side(X.data(), NX - 2, -1, &g.axmL1, &g.bxmL1, &g.cxmL1);
     coef_side(X.data(), NX - 1, -1, &g.axmL, &g.bxmL, &g.cxmL);
 
@@ -787,6 +801,10 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
         if (hipMalloc(p, 3 * fb) != hipSuccess) { set_err(h, "device allocation failed"); *out = h; return MHD_ERR_NOMEM; }
         HCHK(hipMemset(*p, 0, 3 * fb));
     }
+    if (hipMalloc(&h->crad, fb) != hipSuccess) { set_err(h, "device allocation failed"); *out = h; return MHD_ERR_NOMEM; }
+    g.crad = h->crad;
+    hipLaunchKernelGGL(k_crad, dim3(blocks(h->S, 256)), dim3(256), 0, h->stream, g, h->crad);
+    HCHK(hipGetLastError());
     h->nred = 1024;
     long nrows = (long)NY * NZ * 4;
     HCHK(hipMalloc(&h->red, std::max<long>(4096, nrows) * sizeof(double)));
@@ -795,8 +813,8 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     h->xd = h->xdA;
     h->cur = h->cA;
     if (const char *e = getenv("MHD_KC")) h->kc = std::max(1, atoi(e));
-    if (const char *e = getenv("MHD_SWEEP1")) h->sweep1 = atoi(e);
-    if (const char *e = getenv("MHD_FPB")) h->fpb = atoi(e);
+    if (const char *e = getenv("MHD_FUSE_RK")) h->fuse_rk = atoi(e);
+    if (const char *e = getenv("MHD_ZS")) h->zs = atoi(e);
     h->pt.resize(PT_N);
     for (int t = 0; t < PT_N; t++) h->pt[t] = Prof{PT_NAMES[t], 0.0, 0};
     *out = h;
@@ -833,7 +851,7 @@ int mhd_init_residual(mhd_handle *h)
     if (!h) return MHD_ERR_ARG;
     if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
     HCHK(hipMemsetAsync(h->xresd, 0, 3 * h->S * sizeof(double), h->stream));
-    rhs(h, h->x, false, true);
+    rhs(h, h->x, false, true, -1);
     HCHK(hipMemcpyAsync(h->xresd, h->xd + h->S, 3 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     HCHK(hipStreamSynchronize(h->stream));
     HCHK(hipGetLastError());
@@ -843,7 +861,7 @@ int mhd_init_residual(mhd_handle *h)
 int mhd_rhs(mhd_handle *h, double t)
 {
     if (!h) return MHD_ERR_ARG;
-    rhs(h, h->x, t != 0.0, false);
+    rhs(h, h->x, t != 0.0, false, -1);
     HCHK(hipStreamSynchronize(h->stream));
     HCHK(hipGetLastError());
     return MHD_OK;
@@ -852,6 +870,7 @@ int mhd_rhs(mhd_handle *h, double t)
 int mhd_get_xdif(mhd_handle *h, double *xdif)
 {
     if (!h || !xdif) return MHD_ERR_ARG;
+    if (!h->xd) { set_err(h, "no xdif: the last call was a step (its final sweep updates the state in place)"); return MHD_ERR_ARG; }
     return download_fields(h, xdif, h->xd, 8);
 }
 
@@ -1063,7 +1082,7 @@ void mhd_destroy(mhd_handle *h)
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
     double *ps[] = {h->geo, h->x, h->xfold, h->xm, h->xdA, h->xdB, h->eq, h->xresd, h->cA, h->cB,
-                    h->eA, h->eB, h->hA, h->hB, h->red, h->dtv};
+                    h->eA, h->eB, h->hA, h->hB, h->red, h->dtv, h->crad};
     for (auto p : ps)
         if (p) hipFree(p);
     for (auto e : h->pool) (void)hipEventDestroy(e);

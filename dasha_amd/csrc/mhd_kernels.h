@@ -110,435 +110,104 @@ __global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X
 }
 
 // ---------------------------------------------------------------------------
-// one smoothing sweep + its boundary fill on NF fields:
-// avrgc/avrgE/avrgh (3 fields, BC_AUX) and avrg on xdif (8 fields,
-// BC_STATE; the B components use the x-profile coefficient)
-// t3dsdsc.f:5611-5720, 5866-5918, 5952-6004, 6008-6054
+// lane helpers of the row-per-wavefront sweeps
 // ---------------------------------------------------------------------------
-template <int NF, int KIND, int TYPE>
-__global__ __launch_bounds__(NT) void k_smooth(Geo g, const double *__restrict__ A, double *__restrict__ B, int kc)
+// value held by lane l of the wavefront (l uniform over the wavefront)
+__device__ inline double lane_of(double v, int l)
 {
-    __shared__ double sp[NF][TY + 2][TX + 2];
-    __shared__ BCScratch<NF> bs;
-    BlockPos bp = block_pos();
-    Tile t = tile_of(g, kc, bp);
-    if (t.kb >= t.ke) return;
-    const long S = g.S, sz = g.sz;
-    const bool ld = loadable(g, t.i, t.j);
-    const long q0 = ld ? (long)t.i + g.sy * rmap(g, t.j) : 0;
-    double zm[NF], z0[NF], zp[NF];
-    if (ld) {
-#pragma unroll
-        for (int m = 0; m < NF; m++) {
-            zm[m] = A[m * S + q0 + sz * (t.kb - 1)];
-            z0[m] = A[m * S + q0 + sz * t.kb];
-        }
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// x-face values of a row from the lanes holding columns 2, 3, 4 (left) and
+// NX-3 (right); i0 = column of lane 0.  Every lane of the wavefront must
+// call; only windows that hold a face pay for its rule.
+template <int TYPE>
+__device__ inline double row_xrule_rl(const Geo &g, int i, int i0, bool apply, double v)
+{
+    double out = v;
+    if (i0 + TX > g.NX - 2) {       // right face in the window
+        const double pL = lane_of(v, min(max(g.NX - 3 - i0, 0), TX - 1));
+        if (apply && i == g.NX - 1) out = 0.0;
+        else if (apply && i == g.NX - 2) out = 0.5 * pL;
     }
-    const double wpx = t.inner ? g.xwp[t.i] : 0, wmx = t.inner ? g.xwm[t.i] : 0;
-    const double wpy = t.inner ? g.ywp[t.j] : 0, wmy = t.inner ? g.ywm[t.j] : 0;
-    const double cb = (KIND == KIND_STATE && t.inner) ? g.cbx[t.i] : 0.0;
-    const double xy2 = t.inner ? g.xc[t.i] * g.xc[t.i] + g.yc[t.j] * g.yc[t.j] : 0.0;
-    double v2[NF], vL[NF];
-    for (int k = t.kb; k < t.ke; k++) {
-        const long kz = sz * k;
-        if (ld) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) zp[m] = A[m * S + q0 + kz + sz];
-        }
-        load_plane<1, NF>(g, t, sp, kz, [&](int m, long q) { return A[m * S + q]; });
-        __syncthreads();
-        double v[NF];
-        if (t.inner) {
-            const double wpz = g.zwp[k], wmz = g.zwm[k];
-            const double crad = (KIND == KIND_EIGHTH) ? 1.0 / 8.0 : radial_coef(xy2, g.zc[k]);
-#pragma unroll
-            for (int m = 0; m < NF; m++) {
-                double f0 = z0[m];
-                double s = difw(sp[m][t.ty + 1][t.tx], f0, sp[m][t.ty + 1][t.tx + 2], wpx, wmx) +
-                           difw(sp[m][t.ty][t.tx + 1], f0, sp[m][t.ty + 2][t.tx + 1], wpy, wmy) +
-                           difw(zm[m], f0, zp[m], wpz, wmz);
-                double cc = (KIND == KIND_STATE && m >= 4 && m <= 6) ? cb : crad;
-                v[m] = f0 + cc * s;
-            }
+    if (i0 < 2) {                   // left face in the window
+        const int l2 = min(max(2 - i0, 0), TX - 3);
+        const double p2 = lane_of(v, l2);
+        if (TYPE == BC_STATE) {
+            if (apply && i == 0) out = 0.0;
+            else if (apply && i == 1) out = 0.5 * p2;
         } else {
-#pragma unroll
-            for (int m = 0; m < NF; m++) v[m] = 0.0;
-        }
-        plane_rules<NF, TYPE>(g, t, bs, v);
-        if (t.own) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) B[m * S + q0 + kz] = v[m];
-        }
-        if (k == 2) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) v2[m] = v[m];
-        }
-        if (k == g.NZ - 3) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) vL[m] = v[m];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < NF; m++) { zm[m] = z0[m]; z0[m] = zp[m]; }
-    }
-    zrules<NF>(g, t, B, v2, vL);
-}
-
-// ---------------------------------------------------------------------------
-// Sync-free smoothing sweep: one field per block, one tile row per 64-lane
-// wavefront.  x neighbours come from the neighbouring lanes (the two tile
-// edge lanes load their outer neighbour), y neighbours are read from L1/L2,
-// z neighbours from the register queue.  The x-face rules use lane
-// shuffles within the row; the non-periodic image rows evaluate the sweep at
-// row 2 (NY-3) themselves.  No LDS and no block barrier.
-// ---------------------------------------------------------------------------
-constexpr int SY = 8;            // rows (wavefronts) per block
-
-struct RowTile {
-    int lane, i, j, nomx, nomy, kb, ke;
-    bool valid, own;
-};
-
-__device__ inline RowTile rowtile_of(const Geo &g, int kc, const BlockPos &bp, int chunk)
-{
-    RowTile t;
-    t.lane = threadIdx.x % TX;
-    int row = threadIdx.x / TX;
-    t.nomx = bp.bx * TX;
-    int i0 = g.NX >= TX ? min(t.nomx, g.NX - TX) : 0;
-    int nyt = g.yhi - g.ylo + 1;
-    t.nomy = g.ylo + bp.by * SY;
-    int j0 = nyt >= SY ? min(t.nomy, g.yhi + 1 - SY) : g.ylo;
-    t.i = i0 + t.lane;
-    t.j = j0 + row;
-    t.valid = t.i < g.NX && t.j <= g.yhi;
-    t.own = t.valid && t.i >= t.nomx && t.j >= t.nomy;
-    t.kb = 2 + chunk * kc;
-    t.ke = min(t.kb + kc, g.NZ - 2);
-    return t;
-}
-
-__device__ inline double shfl(double v, int src) { return __shfl(v, src, TX); }
-
-// x-face values of a row from the lanes holding columns 2, 3, 4, NX-3
-// x-face values of a row (lane shuffles; every lane of the row must call)
-template <int TYPE>
-__device__ inline double row_xrule_lanes(const Geo &g, int i, int i0, bool valid, double v)
-{
-    int l2 = 2 - i0, lL = g.NX - 3 - i0;
-    double p2 = shfl(v, l2 >= 0 && l2 < TX ? l2 : 0);
-    double p3 = shfl(v, l2 + 1 >= 0 && l2 + 1 < TX ? l2 + 1 : 0);
-    double p4 = shfl(v, l2 + 2 >= 0 && l2 + 2 < TX ? l2 + 2 : 0);
-    double pL = shfl(v, lL >= 0 && lL < TX ? lL : 0);
-    if (valid && xface(g, i)) return xrule<TYPE>(g, i, p2, p3, p4, pL);
-    return v;
-}
-
-template <int TYPE>
-__device__ inline double row_xrule(const Geo &g, const RowTile &t, int i0, double v)
-{
-    int l2 = 2 - i0, lL = g.NX - 3 - i0;
-    double p2 = shfl(v, l2 >= 0 && l2 < TX ? l2 : 0);
-    double p3 = shfl(v, l2 + 1 >= 0 && l2 + 1 < TX ? l2 + 1 : 0);
-    double p4 = shfl(v, l2 + 2 >= 0 && l2 + 2 < TX ? l2 + 2 : 0);
-    double pL = shfl(v, lL >= 0 && lL < TX ? lL : 0);
-    if (t.valid && xface(g, t.i)) return xrule<TYPE>(g, t.i, p2, p3, p4, pL);
-    return v;
-}
-
-template <int KIND, int TYPE>
-__global__ __launch_bounds__(TX * SY) void k_sweep(Geo g, const double *__restrict__ A, double *__restrict__ B,
-                                                   int kc, int nchunk)
-{
-    BlockPos bp = block_pos();
-    const int m = bp.bz / nchunk;
-    RowTile t = rowtile_of(g, kc, bp, bp.bz % nchunk);
-    if (t.kb >= t.ke) return;
-    const int i0 = t.i - t.lane;
-    const long S = g.S, sz = g.sz, sy = g.sy;
-    const double *a = A + m * S;
-    double *b = B + m * S;
-    const bool inner = t.valid && t.i >= 2 && t.i <= g.NX - 3 && t.j >= 2 && t.j <= g.NY - 3;
-    const bool image = t.valid && yimage(g, t.j);
-    // the column this thread evaluates the sweep at: its own, or row 2 /
-    // NY-3 for a non-periodic image row
-    const int je = image ? (t.j < 2 ? 2 : g.NY - 3) : t.j;
-    const bool ecol = t.i < g.NX && (t.valid || !g.periodic ? true : true);
-    const bool einner = t.valid && t.i >= 2 && t.i <= g.NX - 3;   // x-interior column, row je interior
-    const long qc = (long)min(t.i, g.NX - 1) + sy * rmap(g, je);  // column of evaluation
-    const long qm = (long)min(t.i, g.NX - 1) + sy * rmap(g, je - 1);
-    const long qp = (long)min(t.i, g.NX - 1) + sy * rmap(g, je + 1);
-    const bool ld = t.i < g.NX && (g.periodic || (je >= 0 && je < g.NY));
-    (void)ecol;
-    double cb = 0.0, wpx = 0, wmx = 0, wpy = 0, wmy = 0, xy2 = 0;
-    if (einner) {
-        wpx = g.xwp[t.i]; wmx = g.xwm[t.i]; wpy = g.ywp[je]; wmy = g.ywm[je];
-        xy2 = g.xc[t.i] * g.xc[t.i] + g.yc[je] * g.yc[je];
-        if (KIND == KIND_STATE) cb = g.cbx[t.i];
-    }
-    const bool bcoef = KIND == KIND_STATE && m >= 4 && m <= 6;
-    // edge lanes' outer x neighbour
-    const int ex = t.lane == 0 ? -1 : (t.lane == TX - 1 ? 1 : 0);
-    const bool exok = ex != 0 && t.i + ex >= 0 && t.i + ex < g.NX;
-    double zm = 0, z0 = 0, zp = 0;
-    if (ld) { zm = a[qc + sz * (t.kb - 1)]; z0 = a[qc + sz * t.kb]; }
-    double v2 = 0, vL = 0;
-    for (int k = t.kb; k < t.ke; k++) {
-        const long kz = sz * k;
-        double ym = 0, yp = 0, xo = 0;
-        if (ld) {
-            zp = a[qc + kz + sz];
-            ym = a[qm + kz];
-            yp = a[qp + kz];
-        }
-        if (exok) xo = a[qc + ex + kz];
-        double xm = shfl(z0, t.lane > 0 ? t.lane - 1 : 0);
-        double xp = shfl(z0, t.lane < TX - 1 ? t.lane + 1 : TX - 1);
-        if (ex < 0) xm = xo;
-        if (ex > 0) xp = xo;
-        double v = 0.0;
-        if (einner) {
-            double c;
-            if (KIND == KIND_EIGHTH) c = 1.0 / 8.0;
-            else if (bcoef) c = cb;
-            else c = radial_coef(xy2, g.zc[k]);
-            double s = difw(xm, z0, xp, wpx, wmx) + difw(ym, z0, yp, wpy, wmy) +
-                       difw(zm, z0, zp, g.zwp[k], g.zwm[k]);
-            v = z0 + c * s;
-        }
-        // x rule on the evaluated row, then the image-row rule
-        v = row_xrule<TYPE>(g, t, i0, v);
-        if (image) v = (t.j == 0 || t.j == g.NY - 1) ? 0.0 : 0.5 * v;
-        if (t.own) b[(long)t.i + sy * t.j + kz] = v;
-        if (k == 2) v2 = v;
-        if (k == g.NZ - 3) vL = v;
-        zm = z0;
-        z0 = zp;
-    }
-    if (t.own) {
-        long q = (long)t.i + sy * t.j;
-        if (t.kb == 2) { b[q] = 0.0; b[q + sz] = 0.5 * v2; }
-        if (t.ke == g.NZ - 2) { b[q + sz * (g.NZ - 1)] = 0.0; b[q + sz * (g.NZ - 2)] = 0.5 * vL; }
-    }
-    (void)inner;
-}
-
-// Multi-field variant of k_sweep: NF fields per thread (coefficient once per
-// cell), loads of the next plane issued one iteration ahead.
-template <int NF, int KIND, int TYPE>
-__global__ __launch_bounds__(TX * SY) void k_sweepn(Geo g, const double *__restrict__ A, double *__restrict__ B,
-                                                    int kc)
-{
-    BlockPos bp = block_pos();
-    RowTile t = rowtile_of(g, kc, bp, bp.bz);
-    if (t.kb >= t.ke) return;
-    const int i0 = t.i - t.lane;
-    const long S = g.S, sz = g.sz, sy = g.sy;
-    const bool image = t.valid && yimage(g, t.j);
-    const int je = image ? (t.j < 2 ? 2 : g.NY - 3) : t.j;
-    const bool einner = t.valid && t.i >= 2 && t.i <= g.NX - 3;
-    const int ic = min(t.i, g.NX - 1);
-    const long qc = (long)ic + sy * rmap(g, je);
-    const long qm = (long)ic + sy * rmap(g, je - 1);
-    const long qp = (long)ic + sy * rmap(g, je + 1);
-    const bool ld = t.i < g.NX && (g.periodic || (je >= 0 && je < g.NY));
-    double cb = 0.0, wpx = 0, wmx = 0, wpy = 0, wmy = 0, xy2 = 0;
-    if (einner) {
-        wpx = g.xwp[t.i]; wmx = g.xwm[t.i]; wpy = g.ywp[je]; wmy = g.ywm[je];
-        xy2 = g.xc[t.i] * g.xc[t.i] + g.yc[je] * g.yc[je];
-        if (KIND == KIND_STATE) cb = g.cbx[t.i];
-    }
-    const int ex = t.lane == 0 ? -1 : (t.lane == TX - 1 ? 1 : 0);
-    const bool exok = ex != 0 && t.i + ex >= 0 && t.i + ex < g.NX;
-    // registers: plane k-1, k, k+1 of the own column; rows j-1, j+1 and the
-    // outer x neighbour of plane k; the prefetched plane-k+1 rows
-    double zm[NF], z0[NF], zp[NF], ym[NF], yp[NF], xo[NF];
-    double nym[NF], nyp[NF], nxo[NF];
-#pragma unroll
-    for (int m = 0; m < NF; m++) {
-        zm[m] = z0[m] = zp[m] = ym[m] = yp[m] = xo[m] = 0.0;
-        nym[m] = nyp[m] = nxo[m] = 0.0;
-    }
-    {
-        const long kz = sz * t.kb;
-        if (ld) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) {
-                const double *a = A + m * S;
-                zm[m] = a[qc + kz - sz]; z0[m] = a[qc + kz]; zp[m] = a[qc + kz + sz];
-                ym[m] = a[qm + kz]; yp[m] = a[qp + kz];
-            }
-        }
-        if (exok) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) xo[m] = A[m * S + qc + ex + kz];
+            const double x1 = xleft1(g, p2, lane_of(v, l2 + 1), lane_of(v, l2 + 2));
+            if (apply && i == 1) out = x1;
+            else if (apply && i == 0) out = xleft0(g, x1, p2, lane_of(v, l2 + 1));
         }
     }
-    double v2[NF], vL[NF];
-    for (int k = t.kb; k < t.ke; k++) {
-        const long kz = sz * k;
-        // prefetch plane k+1 (rows) and plane k+2 (own column)
-        double nzp[NF];
-        const bool more = k + 1 < t.ke;
-#pragma unroll
-        for (int m = 0; m < NF; m++) nzp[m] = 0.0;
-        if (ld && more) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) {
-                const double *a = A + m * S;
-                nzp[m] = a[qc + kz + 2 * sz];
-                nym[m] = a[qm + kz + sz];
-                nyp[m] = a[qp + kz + sz];
-            }
-        }
-        if (exok && more) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) nxo[m] = A[m * S + qc + ex + kz + sz];
-        }
-        double crad = 0.0;
-        if (KIND == KIND_EIGHTH) crad = 1.0 / 8.0;
-        else if (einner) crad = radial_coef(xy2, g.zc[k]);
-        const double wpz = g.zwp[k], wmz = g.zwm[k];
-        double v[NF];
-#pragma unroll
-        for (int m = 0; m < NF; m++) {
-            double xm = shfl(z0[m], t.lane > 0 ? t.lane - 1 : 0);
-            double xp = shfl(z0[m], t.lane < TX - 1 ? t.lane + 1 : TX - 1);
-            if (ex < 0) xm = xo[m];
-            if (ex > 0) xp = xo[m];
-            v[m] = 0.0;
-            if (einner) {
-                double c = (KIND == KIND_STATE && m >= 4 && m <= 6) ? cb : crad;
-                double s = difw(xm, z0[m], xp, wpx, wmx) + difw(ym[m], z0[m], yp[m], wpy, wmy) +
-                           difw(zm[m], z0[m], zp[m], wpz, wmz);
-                v[m] = z0[m] + c * s;
-            }
-            v[m] = row_xrule<TYPE>(g, t, i0, v[m]);
-            if (image) v[m] = (t.j == 0 || t.j == g.NY - 1) ? 0.0 : 0.5 * v[m];
-        }
-        if (t.own) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) B[m * S + (long)t.i + sy * t.j + kz] = v[m];
-        }
-        if (k == 2) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) v2[m] = v[m];
-        }
-        if (k == g.NZ - 3) {
-#pragma unroll
-            for (int m = 0; m < NF; m++) vL[m] = v[m];
-        }
-#pragma unroll
-        for (int m = 0; m < NF; m++) {
-            zm[m] = z0[m]; z0[m] = zp[m]; zp[m] = nzp[m];
-            ym[m] = nym[m]; yp[m] = nyp[m]; xo[m] = nxo[m];
-        }
-    }
-    if (t.own) {
-        long q = (long)t.i + sy * t.j;
-#pragma unroll
-        for (int m = 0; m < NF; m++) {
-            double *b = B + m * S;
-            if (t.kb == 2) { b[q] = 0.0; b[q + sz] = 0.5 * v2[m]; }
-            if (t.ke == g.NZ - 2) { b[q + sz * (g.NZ - 1)] = 0.0; b[q + sz * (g.NZ - 2)] = 0.5 * vL[m]; }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Point-parallel smoothing sweep: one thread per (i, j, k) of the extended
-// range and NF fields, no z march, so every thread issues its loads at
-// once (the 7-point reuse is served by L1/L2).  A block is 64 x PY x 1:
-// each wavefront is one x row, so the x-face rules run on lane shuffles.
-// Points on an image row or a z face evaluate the sweep at row 2 (NY-3) /
-// plane 2 (NZ-3) themselves and apply the rules to that value.
-// ---------------------------------------------------------------------------
-constexpr int PY = 4;
-
-template <int NF, int KIND, int TYPE>
-__global__ __launch_bounds__(TX * PY) void k_sweep_pt(Geo g, const double *__restrict__ A, double *__restrict__ B)
-{
-    const int lane = threadIdx.x % TX, row = threadIdx.x / TX;
-    const int nomx = blockIdx.x * TX;
-    const int i0 = g.NX >= TX ? min(nomx, g.NX - TX) : 0;
-    const int nyt = g.yhi - g.ylo + 1;
-    const int nomy = g.ylo + blockIdx.y * PY;
-    const int j0 = nyt >= PY ? min(nomy, g.yhi + 1 - PY) : g.ylo;
-    const int i = i0 + lane, j = j0 + row, k = blockIdx.z;
-    const bool valid = i < g.NX && j <= g.yhi;
-    const bool own = valid && i >= nomx && j >= nomy;
-    const long S = g.S, sy = g.sy, sz = g.sz;
-    // point of evaluation
-    const bool image = valid && yimage(g, j);
-    const bool zface = k < 2 || k > g.NZ - 3;
-    const int je = image ? (j < 2 ? 2 : g.NY - 3) : j;
-    const int ke = zface ? (k < 2 ? 2 : g.NZ - 3) : k;
-    const bool einner = valid && i >= 2 && i <= g.NX - 3;
-    const int ic = min(i, g.NX - 1);
-    const bool ld = i < g.NX && (g.periodic || je < g.NY);
-    double v[NF];
-#pragma unroll
-    for (int m = 0; m < NF; m++) v[m] = 0.0;
-    if (einner && ld) {
-        const long qc = (long)ic + sy * rmap(g, je) + sz * ke;
-        const long qm = (long)ic + sy * rmap(g, je - 1) + sz * ke;
-        const long qp = (long)ic + sy * rmap(g, je + 1) + sz * ke;
-        double c = 1.0 / 8.0;
-        if (KIND != KIND_EIGHTH) c = radial_coef(g.xc[i] * g.xc[i] + g.yc[je] * g.yc[je], g.zc[ke]);
-        const double cb = KIND == KIND_STATE ? g.cbx[i] : 0.0;
-        const double wpx = g.xwp[i], wmx = g.xwm[i], wpy = g.ywp[je], wmy = g.ywm[je];
-        const double wpz = g.zwp[ke], wmz = g.zwm[ke];
-#pragma unroll
-        for (int m = 0; m < NF; m++) {
-            const double *a = A + m * S;
-            double f0 = a[qc];
-            double s = difw(a[qc - 1], f0, a[qc + 1], wpx, wmx) + difw(a[qm], f0, a[qp], wpy, wmy) +
-                       difw(a[qc - sz], f0, a[qc + sz], wpz, wmz);
-            double cc = (KIND == KIND_STATE && m >= 4 && m <= 6) ? cb : c;
-            v[m] = f0 + cc * s;
-        }
-    }
-    // every lane takes part in the shuffles of its row
-#pragma unroll
-    for (int m = 0; m < NF; m++) {
-        v[m] = row_xrule_lanes<TYPE>(g, i, i0, valid, v[m]);
-        if (image) v[m] = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * v[m];
-        if (zface) v[m] = (k == 0 || k == g.NZ - 1) ? 0.0 : 0.5 * v[m];
-    }
-    if (own) {
-        const long q = (long)i + sy * j + sz * k;
-#pragma unroll
-        for (int m = 0; m < NF; m++) B[q + m * S] = v[m];
-    }
+    return out;
 }
 
 // ---------------------------------------------------------------------------
 // z-unrolled point-parallel smoothing sweep (the production sweep).  Thread
-// (i, j) computes UZ consecutive planes [k0, k0+UZ) of all NF fields (one
-// field at a time); the column values of planes k0-1..k0+UZ are loaded once
-// and shared between the planes.  The plane groups tile [0, NZ) from plane
-// 0; the last group is shifted back to end at NZ-1 (it writes only its own
+// (i, j) computes UZ consecutive planes [k0, k0+UZ) of FPB fields (one field
+// at a time); the column values of planes k0-1..k0+UZ are loaded once and
+// shared between the planes.  The plane groups tile [0, NZ) from plane 0;
+// the last group is shifted back to end at NZ-1 (it writes only its own
 // planes), so the z-face planes 0, 1 (NZ-2, NZ-1) always sit in the group
-// that also computes plane 2 (NZ-3).  Stores are non-temporal: the output
-// is not re-read by this sweep, and keeping it out of L2 and the Infinity
-// Cache leaves room for the neighbour planes the other threads still read.
+// that also computes plane 2 (NZ-3).  Stores are non-temporal: the output is
+// not re-read by this sweep.
+// Block order (remap != 0): x tile fastest, then y tile, plane group and
+// field block, handed to the XCDs in contiguous ranges (block_pos), so the
+// tiles that share halo rows and planes meet in one L2.
+// With RKS >= 0 the sweep is the second avrg pass of RK stage RKS and ends
+// in that stage's register update (t3dsdsc.f:1090-1132) instead of storing
+// xdif.
 // ---------------------------------------------------------------------------
-constexpr int UZ = 4;
 constexpr int BY = 8;
+constexpr int SWEEP_UZ = 4;
 
-template <int NF, int KIND, int TYPE, int FPB>
-__global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restrict__ A, double *__restrict__ B)
+struct RKArgs {
+    const double *dtp;     // device dt
+    const double *XF;      // xfold
+    double *XM, *X;        // xm, x
+};
+
+template <int RKS>
+__device__ inline void rk_point(const RKArgs &r, long q, double d, double dt)
 {
-    // FPB fields per block: blockIdx.z = plane group + ngroups * field block
+    if (RKS == 0) {
+        double xf = r.XF[q];
+        st(r.XM + q, xf + d * dt / 6.);
+        st(r.X + q, xf + d * dt / 2.);
+    } else if (RKS == 1) {
+        st(r.XM + q, r.XM[q] + d * dt / 3.);
+        st(r.X + q, r.XF[q] + d * dt / 2.);
+    } else if (RKS == 2) {
+        st(r.XM + q, r.XM[q] + d * dt / 3.);
+        st(r.X + q, r.XF[q] + d * dt);
+    } else {
+        st(r.X + q, r.XM[q] + d * dt / 6.);
+    }
+}
+
+template <int NF, int KIND, int TYPE, int FPB, int UZ, int RKS>
+__global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restrict__ A, double *__restrict__ B,
+                                                    RKArgs rk, int remap)
+{
+    BlockPos bp;
+    if (remap) bp = block_pos();
+    else { bp.bx = blockIdx.x; bp.by = blockIdx.y; bp.bz = blockIdx.z; }
+    // FPB fields per block: bz = plane group + ngz * field block
     const int ngz = (g.NZ + UZ - 1) / UZ;
-    const int mb = (blockIdx.z / ngz) * FPB;
+    const int mb = (bp.bz / ngz) * FPB;
     const int lane = threadIdx.x % TX, row = threadIdx.x / TX;
-    const int nomx = blockIdx.x * TX;
+    const int nomx = bp.bx * TX;
     const int i0 = g.NX >= TX ? min(nomx, g.NX - TX) : 0;
     const int nyt = g.yhi - g.ylo + 1;
-    const int nomy = g.ylo + blockIdx.y * BY;
+    const int nomy = g.ylo + bp.by * BY;
     const int j0 = nyt >= BY ? min(nomy, g.yhi + 1 - BY) : g.ylo;
-    const int nomz = (blockIdx.z % ngz) * UZ;
+    const int nomz = (bp.bz % ngz) * UZ;
     const int k0 = min(nomz, g.NZ - UZ);
     const int i = i0 + lane, j = j0 + row;
     const bool valid = i < g.NX && j <= g.yhi;
@@ -568,6 +237,7 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
     const double wpx = einner ? g.xwp[i] : 0, wmx = einner ? g.xwm[i] : 0;
     const double wpy = einner ? g.ywp[je] : 0, wmy = einner ? g.ywm[je] : 0;
     const double cb = (KIND == KIND_STATE && einner) ? g.cbx[i] : 0.0;
+    const double dt = RKS >= 0 ? *rk.dtp : 0.0;
 #pragma unroll
     for (int mm = 0; mm < FPB; mm++) {
         const int m = mb + mm;
@@ -602,7 +272,7 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
                 v[u] = f0 + (bcoef ? cb : cr[u]) * s;
             }
             // x rule on the row, then the image-row rule
-            if (xft) v[u] = row_xrule_lanes<TYPE>(g, i, i0, valid, v[u]);
+            if (xft) v[u] = row_xrule_rl<TYPE>(g, i, i0, valid, v[u]);
             if (image) v[u] = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * v[u];
         }
         // z-face planes from plane 2 / NZ-3 of the same group
@@ -610,14 +280,204 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
         for (int u = 0; u < UZ; u++) {
             int k = k0 + u;
             if (k == 0 || k == g.NZ - 1) v[u] = 0.0;
-            else if (k == 1) v[u] = 0.5 * v[u + 1];
-            else if (k == g.NZ - 2) v[u] = 0.5 * v[u - 1];
+            else if (k == 1 && u + 1 < UZ) v[u] = 0.5 * v[u + 1];
+            else if (k == g.NZ - 2 && u >= 1) v[u] = 0.5 * v[u - 1];
         }
         if (own) {
-            double *b = B + m * S + (long)i + sy * j;
+            const long qb = m * S + (long)i + sy * j;
 #pragma unroll
-            for (int u = 0; u < UZ; u++)
-                if (k0 + u >= nomz) st(b + sz * (k0 + u), v[u]);
+            for (int u = 0; u < UZ; u++) {
+                if (k0 + u < nomz) continue;
+                if (RKS < 0) st(B + qb + sz * (k0 + u), v[u]);
+                else rk_point<RKS>(rk, qb + sz * (k0 + u), v[u], dt);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Temporal blocking of the smoothing chains: NP consecutive passes of one
+// smoothing kind (with the boundary fill after each pass) on one field, in
+// one z-marching block.  A block of 64 x ZB threads covers a window of 64
+// columns x ZB rows; "level p" is the field after p passes.  Iteration t
+// brings plane t of level 0 (loaded ZPF planes ahead) and forms level p at
+// plane t-p, p = 1..NP:
+//   z neighbours  the thread's own register queue of level p-1,
+//   x neighbours  the neighbouring lanes,
+//   y neighbours  level p-1's plane in LDS (written one iteration earlier,
+//                 so one barrier per iteration suffices).
+// Each pass costs the window one column and row at every edge that is not a
+// domain face, so a tile owns the inner (64-2NP) x (ZB-2NP) columns x rows.
+// Every level gets the boundary rules of its pass: x faces from columns 2,
+// 3, 4, NX-3 of the row (lane reads), non-periodic image rows from rows 2 /
+// NY-3 (an extra LDS exchange), z faces from planes 2 / NZ-3 (the register
+// queue of planes 0, 1 is rewritten once plane 2 is known).  The output of
+// the last level is stored, or with RKS >= 0 the RK stage update is applied
+// to it (the second avrg pass of the stage).
+// ---------------------------------------------------------------------------
+constexpr int ZB = 16;            // rows (wavefronts) per block
+constexpr int ZPF = 3;            // planes loaded ahead
+constexpr int ZKC = 32;           // interior planes per block
+
+template <int NP, int KIND, int TYPE, int RKS>
+__global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
+                                                     RKArgs rk, int nch)
+{
+    constexpr int W = TX - 2 * NP, H = ZB - 2 * NP;
+    constexpr int NZW = ZKC + 4 * NP;          // z-weight planes of a chunk
+    constexpr bool RAD = KIND != KIND_EIGHTH;   // reads the radial coefficient table
+    __shared__ double s0[3][ZB][TX];           // level 0 by plane mod 3
+    __shared__ double sl[NP][2][ZB][TX];       // level p (1..NP) by plane parity
+    __shared__ double szw[2][NZW];             // z weights of the chunk's planes
+    const int lane = threadIdx.x % TX;
+    const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);   // uniform over the wavefront
+    const int chunk = blockIdx.z % nch, m = blockIdx.z / nch;
+    // columns: lane l holds column i0 + l
+    const int nomx = blockIdx.x * W;
+    const int i0 = max(0, min(nomx - NP, g.NX - TX));
+    const int i = i0 + lane;
+    const bool vx = i < g.NX;
+    const bool ownx = vx && i >= nomx && i < nomx + W;
+    // rows: wavefront r holds logical row j0 + r (periodic rows map through rmap)
+    const int nomy = g.ylo + blockIdx.y * H;
+    const int j0 = g.periodic ? max(g.ylo - NP, min(nomy - NP, g.yhi + NP + 1 - ZB))
+                              : max(0, min(nomy - NP, g.NY - ZB));
+    const int j = j0 + r;
+    const bool vy = g.periodic || j < g.NY;
+    const bool own = ownx && vy && j >= nomy && j < nomy + H && j <= g.yhi;
+    const bool ld = vx && vy;
+    const int jp = vy ? rmap(g, j) : 0;
+    const bool yin = g.periodic || (j >= 2 && j <= g.NY - 3);
+    const bool pin = ld && yin && i >= 2 && i <= g.NX - 3;     // the stencil applies here
+    const bool img = !g.periodic && vy && (j < 2 || j > g.NY - 3);
+    const int rsrc = img ? (j < 2 ? 2 - j0 : g.NY - 3 - j0) : 0;
+    const bool xft = i0 < 2 || i0 + TX > g.NX - 2;             // the window holds an x face
+    const long S = g.S;
+    const unsigned sz = (unsigned)g.sz;
+    // every lane reads a valid address; lanes outside the range read column 0
+    const unsigned qc = ld ? (unsigned)i + (unsigned)g.sy * (unsigned)jp : 0u;
+    const double *a = A + m * S;                // wave-uniform field bases
+    const double *crad = g.crad;
+    const double wpx = pin ? g.xwp[i] : 0.0, wmx = pin ? g.xwm[i] : 0.0;
+    const double wpy = pin ? g.ywp[jp] : 0.0, wmy = pin ? g.ywm[jp] : 0.0;
+    const bool bcomp = KIND == KIND_STATE && m >= 4 && m <= 6;
+    const double cb = (KIND == KIND_STATE && pin) ? g.cbx[i] : 0.0;
+    const int kb = 2 + chunk * ZKC, ke = min(kb + ZKC, g.NZ - 2);
+    const int t0 = kb - NP, t1 = ke - 1 + NP;
+    const int kz0 = t0 - NP;                    // lowest plane a level touches
+    for (int n = threadIdx.x; n < NZW; n += TX * ZB) {
+        const int k = kz0 + n;
+        const bool kin = k >= 2 && k <= g.NZ - 3;
+        szw[0][n] = kin ? g.zwp[k] : 0.0;
+        szw[1][n] = kin ? g.zwm[k] : 0.0;
+    }
+    const double dt = RKS >= 0 ? *rk.dtp : 0.0;
+    // owned points are unique rows, where rmap(j) == j: the output offset is qc
+    double *bo = RKS < 0 ? B + m * S : nullptr;
+    const double *xf = RKS >= 0 ? rk.XF + m * S : nullptr;
+    double *xmo = RKS >= 0 ? rk.XM + m * S : nullptr;
+    double *xo = RKS >= 0 ? rk.X + m * S : nullptr;
+    // final value v of point e: store it, or apply the RK stage update
+    auto put = [&](unsigned e, double v) {
+        if (RKS < 0) st(fldp(bo, e), v);
+        else if (RKS == 0) { const double f = fld(xf, e); st(fldp(xmo, e), f + v * dt / 6.); st(fldp(xo, e), f + v * dt / 2.); }
+        else if (RKS == 1) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt / 2.); }
+        else if (RKS == 2) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt); }
+        else st(fldp(xo, e), fld(xmo, e) + v * dt / 6.);
+    };
+
+    double q[NP + 1][3];   // level p at its three latest planes, oldest first
+    double cq[NP];         // radial coefficient of plane t-p, used by level p
+    double cprev = 0.0;    // radial coefficient of plane t-1
+    double pf[ZPF], pc[ZPF];
+#pragma unroll
+    for (int p = 0; p <= NP; p++) q[p][0] = q[p][1] = q[p][2] = 0.0;
+#pragma unroll
+    for (int p = 0; p < NP; p++) cq[p] = 0.0;
+    if (RAD && t0 >= 1) cprev = fld(crad, qc + sz * (unsigned)(t0 - 1));
+#pragma unroll
+    for (int d = 0; d < ZPF; d++) {
+        const unsigned e = qc + sz * (unsigned)min(t0 + d, t1);
+        pf[d] = fld(a, e);
+        pc[d] = RAD ? fld(crad, e) : 0.0;
+    }
+
+    // the march runs in groups of ZPF planes so that every prefetch slot is a
+    // fixed register (the loads stay in flight); planes past t1 are dummies
+    // that load plane t1 again and store nothing
+    const int tend = t0 + ((t1 - t0 + ZPF) / ZPF) * ZPF;
+    for (int tb = t0; tb < tend; tb += ZPF) {
+#pragma unroll
+        for (int u = 0; u < ZPF; u++) {
+            const int t = tb + u;
+            const double v0 = ld ? pf[u] : 0.0;
+            const double cv = pc[u];
+            {
+                const unsigned e = qc + sz * (unsigned)min(t + ZPF, t1);
+                pf[u] = fld(a, e);
+                if (RAD) pc[u] = fld(crad, e);
+            }
+            q[0][0] = q[0][1]; q[0][1] = q[0][2]; q[0][2] = v0;
+            s0[t % 3][r][lane] = v0;
+            if (RAD) {
+#pragma unroll
+                for (int p = NP - 1; p > 0; p--) cq[p] = cq[p - 1];
+                cq[0] = cprev;
+                cprev = cv;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int p = 1; p <= NP; p++) {
+                const int k = t - p;                        // plane of level p
+                const bool kin = k >= 2 && k <= g.NZ - 3;
+                const int rm = r > 0 ? r - 1 : 0, rp = r < ZB - 1 ? r + 1 : ZB - 1;
+                const double f0 = q[p - 1][1];              // level p-1 at plane k
+                double ym, yp;
+                if (p == 1) {
+                    const int b = (k + 3) % 3;
+                    ym = s0[b][rm][lane];
+                    yp = s0[b][rp][lane];
+                } else {
+                    ym = sl[p - 2][k & 1][rm][lane];
+                    yp = sl[p - 2][k & 1][rp][lane];
+                }
+                const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
+                const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
+                double v = 0.0;
+                if (pin && kin) {
+                    const double c = KIND == KIND_EIGHTH ? 0.125 : (bcomp ? cb : cq[p - 1]);
+                    const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
+                                     difw(q[p - 1][0], f0, q[p - 1][2], szw[0][k - kz0], szw[1][k - kz0]);
+                    v = f0 + c * s;
+                }
+                if (xft) v = row_xrule_rl<TYPE>(g, i, i0, vx && yin, v);
+                if (!g.periodic && kin) {
+                    // image rows: 0 on the outer row, half of row 2 (NY-3) on the next
+                    sl[p - 1][k & 1][r][lane] = v;
+                    __syncthreads();
+                    if (img) {
+                        v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sl[p - 1][k & 1][rsrc][lane];
+                        sl[p - 1][k & 1][r][lane] = v;
+                    }
+                } else if (p < NP) {
+                    sl[p - 1][k & 1][r][lane] = v;
+                }
+                // z faces above the interior: plane NZ-2 is half of NZ-3, NZ-1 is 0
+                if (k == g.NZ - 2) v = 0.5 * q[p][2];
+                else if (k == g.NZ - 1) v = 0.0;
+                q[p][0] = q[p][1]; q[p][1] = q[p][2]; q[p][2] = v;
+                // below: planes 1 and 0 follow from plane 2
+                if (k == 2) { q[p][1] = 0.5 * v; q[p][0] = 0.0; }
+                if (p == NP && own) {
+                    if (k >= kb && k < ke) put(qc + sz * (unsigned)k, v);
+                    // z-face planes: 0, 1 after plane 2; NZ-2, NZ-1 after NZ-3
+                    const bool lo = k == 2 && kb == 2, hi = k == g.NZ - 3 && ke == g.NZ - 2;
+                    if (lo || hi) {
+                        put(qc + sz * (unsigned)(lo ? 1 : g.NZ - 2), 0.5 * v);
+                        put(qc + sz * (unsigned)(lo ? 0 : g.NZ - 1), 0.0);
+                    }
+                }
+            }
         }
     }
 }

@@ -100,7 +100,9 @@ int mhd_get_divb(mhd_handle *h, double *divb);
 /* stage-level access for parity checks of the RHS (right, t3dsdsc.f:1297-1673):
  * mhd_rhs runs one right(t) on the current state (t == 0 skips the final
  * smoothing as t3dsdsc.f:1669 does); mhd_get_xdif reads the xdif array
- * (8 fields) right() leaves behind; xresd is the frozen momentum residual. */
+ * (8 fields) the last mhd_rhs / mhd_init_residual left behind (a step's
+ * last sweep updates the state in place and leaves none: MHD_ERR_ARG);
+ * xresd is the frozen momentum residual. */
 int mhd_rhs(mhd_handle *h, double t);
 int mhd_get_xdif(mhd_handle *h, double *xdif);
 int mhd_get_residual(mhd_handle *h, double *xresd);

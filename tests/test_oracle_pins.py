@@ -15,7 +15,9 @@ checked against facts fixed independently of its code:
 * RK4 register scheme: textbook RK4 built from the oracle's own RHS calls;
 * whole step: exact equilibrium invariance, z-mirror symmetry,
   y-translation invariance;
-* setdt: closed form on a uniform state and its index ranges.
+* setdt: closed form on a uniform state and its index ranges;
+* diagnostics: div B of solenoidal and linear-divergence polynomial fields,
+  energy integrals of a uniform state, current extrema of a uniform current.
 """
 import math
 
@@ -427,3 +429,95 @@ def test_setdt_closed_form_and_ranges():
     assert planted(c.mx + 1, 5, 8) == base       # jx = xlast-1 excluded
     assert planted(8, c.my, 8) == base           # jy = my excluded
     assert planted(8, 5, c.mz + 1) == base       # jz = zlast-1 excluded
+
+
+# ---------------------------------------------------------------- diagnostics
+def _coords(c):
+    Z, Y, X = np.meshgrid(c.zz, c.yy, c.xx, indexing="ij")
+    return X, Y, Z
+
+
+def _state_with_b(c, b5, b6, b7):
+    x0 = np.zeros((8,) + c.shape3)
+    x0[0] = 1.0
+    x0[7] = 1.0
+    x0[4], x0[5], x0[6] = b5, b6, b7
+    return x0
+
+
+@pytest.mark.parametrize("stretched", [False, True])
+def test_divb_of_solenoidal_quadratic_field_is_zero(stretched):
+    """B = (x^2 y, -x y^2, 0): the 5-point first derivatives are exact for
+    quadratics on any mesh, so diverB is zero at every interior point"""
+    c = harris.case_small(20, 12, 20, stretched=stretched, periodic_y=False)
+    X, Y, Z = _coords(c)
+    c.x0 = _state_with_b(c, X * X * Y, -X * Y * Y, 0 * X)
+    dv = mk(c).divb()
+    inner = dv[4:c.mz, 2:c.my - 2, 4:c.mx]        # jz, jy, jx = 5..m, 3..my-2, 5..m
+    assert inner.shape == (c.mz - 4, c.my - 4, c.mx - 4)
+    assert np.abs(inner).max() < 1e-12
+
+
+@pytest.mark.parametrize("stretched", [False, True])
+def test_divb_linear_in_z_everywhere(stretched):
+    """B = (0, 0.3 y z, 0.2 z^2): div B = 0.7 z.  Constant in x and y and
+    linear in z, so every bndrydiv rule (normalised x extrapolation, the
+    5/2, -2, 1/2 y and z extrapolation in index space) reproduces it on a
+    uniform mesh: 0.7 z on the whole physical range, faces included.  On a
+    stretched z mesh the index-space z extrapolation is not exact, so there
+    the check covers the interior planes (x and y faces included)."""
+    c = harris.case_small(20, 12, 20, stretched=stretched, periodic_y=False)
+    X, Y, Z = _coords(c)
+    c.x0 = _state_with_b(c, 0 * X, 0.3 * Y * Z, 0.2 * Z * Z)
+    dv = mk(c).divb()
+    zr = slice(4, c.mz) if stretched else slice(2, c.mz + 2)
+    phys = np.s_[zr, :, 2:c.mx + 2]
+    err = np.abs(dv[phys] - 0.7 * Z[phys]).max()
+    assert err < 1e-11
+    if not stretched:
+        # a quadratic in z is not reproduced at the z faces (5/2, -2, 1/2 is
+        # exact for linear data only): the check above can fail
+        c.x0 = _state_with_b(c, 0 * X, 0 * X, 0.1 * Z ** 3)
+        dv = mk(c).divb()
+        assert np.abs(dv[phys] - 0.3 * Z[phys] ** 2).max() > 1e-3
+
+
+def test_energy_of_uniform_state():
+    """uniform state: each integral is its density times the box volume
+    (trapezoid weights over the physical range, unique periodic rows)"""
+    c = harris.case_small(20, 8, 24, stretched=True)
+    rho, v, b, p = 2.0, (0.1, 0.2, 0.3), (0.5, 0.6, 0.7), 0.9
+    x0 = np.zeros((8,) + c.shape3)
+    x0[0] = rho
+    for m in range(3):
+        x0[1 + m] = rho * v[m]
+        x0[4 + m] = b[m]
+    x0[7] = p
+    c.x0 = x0
+    for a in (c.bxi, c.byi, c.bzi):
+        a[...] = 0.0
+    vol = (c.xx[c.mx + 1] - c.xx[2]) * (c.zz[c.mz + 1] - c.zz[2]) * c.ny * c.dy[2]
+    e = mk(c).energy()
+    expect = [0.5 * rho * sum(t * t for t in v) * vol, 0.5 * sum(t * t for t in b) * vol,
+              p / (c.gamma - 1.0) * vol, 0.5 * sum(t * t for t in b) * vol]
+    assert np.allclose(e, expect, rtol=1e-13, atol=0)
+
+
+def test_diagn_extrema_of_uniform_current():
+    """B - B_i = (0, 0, c x): J = (0, -c, 0) inside.  Smoothing keeps a
+    constant exactly away from the faces, the face rules only raise it
+    towards 0 (plane 1 is half plane 2, plane 0 and the ghost columns are
+    0), so on the midplane cymin = -c and every other extremum is 0"""
+    c = harris.case_small(24, 8, 24, stretched=True, periodic_y=True)
+    X, Y, Z = _coords(c)
+    cc = 0.3
+    st = _state_with_b(c, 0 * X, 0 * X, cc * X)
+    st[:, :, :, :2] = 0.0
+    st[:, :, :, -2:] = 0.0
+    c.x0 = st
+    for a in (c.bxi, c.byi, c.bzi):
+        a[...] = 0.0
+    d = mk(c).diagn()
+    assert d[0] == 0.0 and d[1] == 0.0
+    assert d[2] == 0.0 and abs(d[3] + cc) < 1e-13
+    assert d[4] == 0.0 and d[5] == 0.0
