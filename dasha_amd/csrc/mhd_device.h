@@ -141,7 +141,7 @@ __device__ inline Tile tile_of(const Geo &g, int kc, const BlockPos &bp)
     Tile t;
     int chunk = bp.bz;
     t.tx = threadIdx.x % TX;
-    t.ty = threadIdx.x / TX;
+    t.ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);   // one row per wavefront
     int nomx = bp.bx * TX;
     int i0 = g.NX >= TX ? min(nomx, g.NX - TX) : 0;
     int nyt = g.yhi - g.ylo + 1;

@@ -14,28 +14,18 @@ namespace mhd {
 // non-temporal store: outputs are read by the next kernel, not by this one
 __device__ inline void st(double *p, double v) { __builtin_nontemporal_store(v, p); }
 
-// z rules of AUX/STATE fills on the first / last chunk: plane 1 (NZ-2) is
-// half of plane 2 (NZ-3), plane 0 (NZ-1) is zero (t3dsdsc.f:2614-2644,
-// 1821-1861)
+// z rules of AUX/STATE fills, applied as soon as plane 2 (NZ-3) is known:
+// plane 1 (NZ-2) is half of it, plane 0 (NZ-1) is zero (t3dsdsc.f:2614-2644,
+// 1821-1861).  q = offset of the column in field 0.
 template <int NF>
-__device__ inline void zrules(const Geo &g, const Tile &t, double *__restrict__ B, const double *v2,
-                              const double *vL)
+__device__ inline void zface_store(const Geo &g, double *__restrict__ B, long q, int k, const double *v)
 {
-    if (!t.own) return;
-    long q = t.i + g.sy * t.j;
-    if (t.kb == 2) {
+    if (k != 2 && k != g.NZ - 3) return;
+    const long qh = q + g.sz * (k == 2 ? 1 : g.NZ - 2), q0 = q + g.sz * (k == 2 ? 0 : g.NZ - 1);
 #pragma unroll
-        for (int m = 0; m < NF; m++) {
-            st(B + q + m * g.S, 0.0);
-            st(B + q + g.sz + m * g.S, 0.5 * v2[m]);
-        }
-    }
-    if (t.ke == g.NZ - 2) {
-#pragma unroll
-        for (int m = 0; m < NF; m++) {
-            st(B + q + g.sz * (g.NZ - 1) + m * g.S, 0.0);
-            st(B + q + g.sz * (g.NZ - 2) + m * g.S, 0.5 * vL[m]);
-        }
+    for (int m = 0; m < NF; m++) {
+        st(B + qh + m * g.S, 0.5 * v[m]);
+        st(B + q0 + m * g.S, 0.0);
     }
 }
 
@@ -69,7 +59,6 @@ __global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X
         ax[0] = g.x1a[t.i]; ax[1] = g.x1b[t.i]; ax[2] = g.x1c[t.i]; ax[3] = g.x1d[t.i];
         ay[0] = g.y1a[t.j]; ay[1] = g.y1b[t.j]; ay[2] = g.y1c[t.j]; ay[3] = g.y1d[t.j];
     }
-    double v2[3] = {0, 0, 0}, vL[3] = {0, 0, 0};
     for (int k = t.kb; k < t.ke; k++) {
         const long kz = sz * k;
         if (ld) {
@@ -99,14 +88,12 @@ __global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X
         if (t.own) {
             long q = q0 + kz;
             st(cur + q, v[0]); st(cur + q + S, v[1]); st(cur + q + 2 * S, v[2]);
+            zface_store<3>(g, cur, q0, k, v);
         }
-        if (k == 2) { v2[0] = v[0]; v2[1] = v[1]; v2[2] = v[2]; }
-        if (k == g.NZ - 3) { vL[0] = v[0]; vL[1] = v[1]; vL[2] = v[2]; }
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < 4; s++) { uq[s] = uq[s + 1]; vq[s] = vq[s + 1]; }
     }
-    zrules<3>(g, t, cur, v2, vL);
 }
 
 // ---------------------------------------------------------------------------
@@ -576,7 +563,6 @@ __global__ __launch_bounds__(NT) void k_rhs_a(Geo g, const double *__restrict__ 
         ax[0] = g.x1a[t.i]; ax[1] = g.x1b[t.i]; ax[2] = g.x1c[t.i]; ax[3] = g.x1d[t.i];
         ay[0] = g.y1a[t.j]; ay[1] = g.y1b[t.j]; ay[2] = g.y1c[t.j]; ay[3] = g.y1d[t.j];
     }
-    double h2[3] = {0, 0, 0}, hL[3] = {0, 0, 0};
     for (int k = t.kb; k < t.ke; k++) {
         const long kz = sz * k;
         const long qk = q0 + kz;
@@ -635,9 +621,10 @@ __global__ __launch_bounds__(NT) void k_rhs_a(Geo g, const double *__restrict__ 
         }
         if (hall) {
             plane_rules<3, BC_AUX>(g, t, bs, hv);
-            if (t.own) { st(XH + qk, hv[0]); st(XH + qk + S, hv[1]); st(XH + qk + 2 * S, hv[2]); }
-            if (k == 2) { h2[0] = hv[0]; h2[1] = hv[1]; h2[2] = hv[2]; }
-            if (k == g.NZ - 3) { hL[0] = hv[0]; hL[1] = hv[1]; hL[2] = hv[2]; }
+            if (t.own) {
+                st(XH + qk, hv[0]); st(XH + qk + S, hv[1]); st(XH + qk + 2 * S, hv[2]);
+                zface_store<3>(g, XH, q0, k, hv);
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -649,7 +636,6 @@ __global__ __launch_bounds__(NT) void k_rhs_a(Geo g, const double *__restrict__ 
             yx2[s] = yx2[s + 1]; yx3[s] = yx3[s + 1]; yt22[s] = yt22[s + 1]; yt23[s] = yt23[s + 1]; yt33[s] = yt33[s + 1];
         }
     }
-    if (hall) zrules<3>(g, t, XH, h2, hL);
 }
 
 // ---------------------------------------------------------------------------
@@ -695,7 +681,6 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
             by[0] = g.y2a[t.j]; by[1] = g.y2b[t.j]; by[2] = g.y2c[t.j]; by[3] = g.y2d[t.j];
         }
     }
-    double v2[8], vL[8];
     for (int k = t.kb; k < t.ke; k++) {
         const long kz = sz * k;
         const long qk = q0 + kz;
@@ -762,14 +747,7 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
         if (t.own) {
 #pragma unroll
             for (int m = 0; m < 8; m++) st(XD + qk + m * S, v[m]);
-        }
-        if (k == 2) {
-#pragma unroll
-            for (int m = 0; m < 8; m++) v2[m] = v[m];
-        }
-        if (k == g.NZ - 3) {
-#pragma unroll
-            for (int m = 0; m < 8; m++) vL[m] = v[m];
+            zface_store<8>(g, XD, q0, k, v);
         }
         __syncthreads();
 #pragma unroll
@@ -778,7 +756,6 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
             if (VD) { qvx[s] = qvx[s + 1]; qvy[s] = qvy[s + 1]; qrh[s] = qrh[s + 1]; }
         }
     }
-    zrules<8>(g, t, XD, v2, vL);
 }
 
 // ---------------------------------------------------------------------------
