@@ -31,19 +31,25 @@ B_ALG_NOHALL = 5104         # d_i = 0
 
 def per_cell_bytes(case):
     """algorithmic HBM bytes per physical cell of one launch of each kernel
-    family: every input field read once, every output field written once"""
+    family: every input field of every smoothing pass read once and every
+    output written once (the per-sweep table of the reference's step), so a
+    launch that fuses two passes counts both"""
     hall = case.hall
+    rk2 = (256 + 320 + 320 + 192) // 4        # avrg pass 2 + RK update, mean of the 4 stages
     return {
-        "curl": 8 * (6 + 3),                            # B 3 + B_i 3 in, J 3 out
-        "smooth3": 8 * (3 + 3),                         # avrgc / avrgE pass
-        "smooth3_eighth": 8 * (3 + 3),                  # avrgh pass
-        "smooth8": 8 * (8 + 8),                         # avrg pass 1
-        # avrg pass 2 fused with the RK register update, mean of the 4 stages:
-        # (xdif+xfold -> xm+x), 2 x (xdif+xfold+xm -> xm+x), (xdif+xm -> x)
-        "rk_update": 8 * (32 + 40 + 40 + 24) // 4,
+        "curl": 8 * (6 + 3),                    # B 3 + B_i 3 in, J 3 out
+        "avrg_radial_x2": 2 * 8 * (3 + 3),      # two avrgc / avrgE passes
+        "avrg_radial_x1": 8 * (3 + 3),
+        "avrgh_x2": 2 * 8 * (3 + 3),
+        "avrgh_x1": 8 * (3 + 3),
+        "avrg_state_x2": 2 * 8 * (8 + 8),
+        "avrg_state_x1": 8 * (8 + 8),
+        "avrg_state_x2_rk": 8 * (8 + 8) + rk2,  # avrg pass 1 + (pass 2 fused with the RK update)
+        "avrg_state_x1_rk": rk2,
+        "rk_update": rk2,                       # stand-alone RK update (same fields)
         "rhs_flux_E_hall": 8 * (15 + 10) if hall else 8 * (15 + 7),   # U8,p_i,J3,xresd3 -> xdif1-4,E3(,xdh3)
         "rhs_curlE_pressure": 8 * ((18 if hall else 15) + 8),         # E3(,xdh3),U5,J3,xdif1-4 -> xdif8
-        "smooth8_post": 8 * ((8 + 5 + 8) + (8 + 8)) // 2,             # mean of the two avrgtt passes
+        "avrgtt": 8 * ((8 + 5 + 8) + (8 + 8)) // 2,                   # mean of the two avrgtt passes
         "post_add_eq": 8 * (8 + 5 + 8),
         "setdt": 8 * 8,
     }

@@ -418,9 +418,15 @@ static void prof_mark(mhd_handle *h, int tag)
     h->ev_tag.push_back(tag);
 }
 
-enum { PT_CURL, PT_FILT3, PT_FILT3E, PT_FILT8, PT_FILTTT, PT_BC, PT_RHSA, PT_RHSB, PT_RK, PT_POST, PT_DT, PT_N };
-static const char *PT_NAMES[PT_N] = {"curl", "smooth3", "smooth3_eighth", "smooth8", "smooth8_post", "boundary_post",
-                                     "rhs_flux_E_hall", "rhs_curlE_pressure", "rk_update", "post_add_eq", "setdt"};
+// kernel families (one name per kernel shape; _x2 = two passes in one launch)
+enum {
+    PT_CURL, PT_RAD2, PT_RAD1, PT_EIG2, PT_EIG1, PT_ST2, PT_ST1, PT_ST2RK, PT_ST1RK, PT_RK, PT_FILTTT, PT_BC,
+    PT_RHSA, PT_RHSB, PT_POST, PT_DT, PT_N
+};
+static const char *PT_NAMES[PT_N] = {"curl", "avrg_radial_x2", "avrg_radial_x1", "avrgh_x2", "avrgh_x1",
+                                     "avrg_state_x2", "avrg_state_x1", "avrg_state_x2_rk", "avrg_state_x1_rk",
+                                     "rk_update", "avrgtt", "bndrytt", "rhs_flux_E_hall", "rhs_curlE_pressure",
+                                     "post_add_eq", "setdt"};
 
 static int prof_collect(mhd_handle *h)
 {
@@ -499,10 +505,11 @@ static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
 // kk smoothing sweeps (+ fills) on a 3-field array; returns the result buffer
 static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk)
 {
-    const int tag = kind == KIND_RADIAL ? PT_FILT3 : PT_FILT3E;
     while (kk > 0) {
+        const bool two = h->zs && kk >= 2;
+        const int tag = kind == KIND_RADIAL ? (two ? PT_RAD2 : PT_RAD1) : (two ? PT_EIG2 : PT_EIG1);
         PROF_BEGIN(tag);
-        if (h->zs && kk >= 2) {
+        if (two) {
             if (kind == KIND_RADIAL) launch_zsmooth<3, 2, KIND_RADIAL, BC_AUX, -1>(h, a, b);
             else launch_zsmooth<3, 2, KIND_EIGHTH, BC_AUX, -1>(h, a, b);
             kk -= 2;
@@ -555,7 +562,7 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
     if (smooth_xdif && h->zs) {
         // avrg twice (t3dsdsc.f:1668-1671) in one temporal-blocked kernel;
         // with rks >= 0 it also performs the RK stage update
-        PROF_BEGIN(rks >= 0 ? PT_RK : PT_FILT8);
+        PROF_BEGIN(rks >= 0 ? PT_ST2RK : PT_ST2);
         switch (rks) {
         case 0: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 0>(h, a, nullptr); break;
         case 1: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 1>(h, a, nullptr); break;
@@ -563,16 +570,16 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
         case 3: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 3>(h, a, nullptr); break;
         default: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, -1>(h, a, b); std::swap(a, b); break;
         }
-        PROF_END(rks >= 0 ? PT_RK : PT_FILT8);
+        PROF_END(rks >= 0 ? PT_ST2RK : PT_ST2);
         if (rks >= 0) a = nullptr;
     } else if (smooth_xdif) {
         // avrg twice (t3dsdsc.f:1668-1671); with rks >= 0 the second pass
         // also performs the RK stage update
-        PROF_BEGIN(PT_FILT8);
+        PROF_BEGIN(PT_ST1);
         launch_sweep<8, KIND_STATE, BC_STATE, -1>(h, a, b);
-        PROF_END(PT_FILT8);
+        PROF_END(PT_ST1);
         std::swap(a, b);
-        PROF_BEGIN(rks >= 0 ? PT_RK : PT_FILT8);
+        PROF_BEGIN(rks >= 0 ? PT_ST1RK : PT_ST1);
         switch (rks) {
         case 0: launch_sweep<8, KIND_STATE, BC_STATE, 0>(h, a, nullptr); break;
         case 1: launch_sweep<8, KIND_STATE, BC_STATE, 1>(h, a, nullptr); break;
@@ -580,7 +587,7 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
         case 3: launch_sweep<8, KIND_STATE, BC_STATE, 3>(h, a, nullptr); break;
         default: launch_sweep<8, KIND_STATE, BC_STATE, -1>(h, a, b); std::swap(a, b); break;
         }
-        PROF_END(rks >= 0 ? PT_RK : PT_FILT8);
+        PROF_END(rks >= 0 ? PT_ST1RK : PT_ST1);
         if (rks >= 0) a = nullptr;
     }
     h->xd = a;

@@ -1,0 +1,142 @@
+"""Summaries of rocprofv3 output for the step's kernels.
+
+``python -m dasha_amd.profiling --stats DIR [--fetch DIR --write DIR]
+--workload NAME --command "..." --out profiles/NAME.md`` turns the CSV files
+of a ``rocprofv3 --kernel-trace --stats`` run and of two ``--pmc
+FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs into a short markdown summary and,
+with the counters, ``profiles/traffic_NAME.json``: HBM bytes per launch of
+each kernel family, scaled by what the same counters read for the library's
+plain copy kernel ``k_copy`` (bytes known: the probe copies ``--copy-bytes``
+per launch, read and write).
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import re
+
+# kernel symbol (demangled prefix) -> family name used by bench.py
+_FAMILIES = [
+    (r"k_curl\b", "curl"),
+    (r"k_zsmooth<2, 0, 0, -1>", "avrg_radial_x2"),
+    (r"k_sweepz<3, 0, 0, 3, 4, -1>", "avrg_radial_x1"),
+    (r"k_zsmooth<2, 1, 0, -1>", "avrgh_x2"),
+    (r"k_sweepz<3, 1, 0, 3, 4, -1>", "avrgh_x1"),
+    (r"k_zsmooth<2, 2, 1, -1>", "avrg_state_x2"),
+    (r"k_sweepz<8, 2, 1, 8, 4, -1>", "avrg_state_x1"),
+    (r"k_zsmooth<2, 2, 1, [0-3]>", "avrg_state_x2_rk"),
+    (r"k_sweepz<8, 2, 1, 8, 4, [0-3]>", "avrg_state_x1_rk"),
+    (r"k_rk\b", "rk_update"),
+    (r"k_rhs_a\b", "rhs_flux_E_hall"),
+    (r"k_rhs_b<", "rhs_curlE_pressure"),
+    (r"k_smooth_tt<", "avrgtt"),
+    (r"k_bc_tt\b", "bndrytt"),
+    (r"k_add_eq\b", "post_add_eq"),
+    (r"k_dt_(partial|final)\b", "setdt"),
+    (r"k_copy\b", "copy_probe"),
+]
+
+
+def family(kernel_name: str) -> str | None:
+    for pat, fam in _FAMILIES:
+        if re.search(pat, kernel_name):
+            return fam
+    return None
+
+
+def _one(d: str, pattern: str) -> str:
+    hits = glob.glob(os.path.join(d, "**", pattern), recursive=True)
+    if not hits:
+        raise FileNotFoundError(f"no {pattern} under {d}")
+    return hits[0]
+
+
+def kernel_stats(d: str):
+    rows = list(csv.DictReader(open(_one(d, "*kernel_stats.csv"))))
+    out = []
+    for r in rows:
+        out.append(dict(name=r["Name"], family=family(r["Name"]), calls=int(r["Calls"]),
+                        avg_us=float(r["AverageNs"]) / 1e3, total_ms=float(r["TotalDurationNs"]) / 1e6,
+                        pct=float(r["Percentage"])))
+    return out
+
+
+def counter_per_kernel(d: str, counter: str):
+    """mean value of `counter` per dispatch, by kernel name (KiB)"""
+    acc = collections.defaultdict(list)
+    for r in csv.DictReader(open(_one(d, "*counter_collection.csv"))):
+        if r["Counter_Name"] == counter:
+            acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def traffic(fetch_dir: str, write_dir: str, copy_bytes: int):
+    """bytes per launch by family: FETCH_SIZE and WRITE_SIZE (KiB) scaled so
+    that the copy kernel reads and writes copy_bytes"""
+    fetch = counter_per_kernel(fetch_dir, "FETCH_SIZE")
+    write = counter_per_kernel(write_dir, "WRITE_SIZE")
+    ck = [k for k in fetch if family(k) == "copy_probe"]
+    if not ck:
+        raise RuntimeError("the copy kernel is missing from the counter runs")
+    f_scale = copy_bytes / (fetch[ck[0]] * 1024.0)
+    w_scale = copy_bytes / (write[ck[0]] * 1024.0)
+    fams = collections.defaultdict(lambda: [0.0, 0.0, 0])
+    for k in fetch:
+        fam = family(k)
+        if fam is None or fam == "copy_probe" or k not in write:
+            continue
+        a = fams[fam]
+        a[0] += fetch[k] * 1024.0 * f_scale
+        a[1] += write[k] * 1024.0 * w_scale
+        a[2] += 1
+    res = {fam: {"read_bytes_per_launch": v[0] / v[2], "write_bytes_per_launch": v[1] / v[2],
+                 "bytes_per_launch": (v[0] + v[1]) / v[2]} for fam, v in fams.items()}
+    return res, f_scale, w_scale
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-bytes", type=int, default=2 << 30)
+    ap.add_argument("--workload", required=True)
+    ap.add_argument("--command", required=True, help="the rocprofv3 command lines, verbatim")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--cells", type=int, default=0, help="physical cells (for bytes per cell)")
+    args = ap.parse_args()
+    ks = kernel_stats(args.stats)
+    lines = [f"# {args.workload}: kernel times (rocprofv3)", "", "Commands:", "", "```"]
+    lines += args.command.split("\\n")
+    lines += ["```", "", "| kernel family | kernel | calls | avg us | total ms | % |", "|---|---|---|---|---|---|"]
+    for r in sorted(ks, key=lambda r: -r["total_ms"]):
+        nm = re.sub(r"\(.*", "", r["name"].replace("(anonymous namespace)::", ""))
+        lines.append(f"| {r['family'] or '-'} | `{nm}` | {r['calls']} | {r['avg_us']:.1f} | "
+                     f"{r['total_ms']:.3f} | {r['pct']:.1f} |")
+    if args.fetch and args.write:
+        tr, fs, ws = traffic(args.fetch, args.write, args.copy_bytes)
+        lines += ["", "HBM traffic per launch (FETCH_SIZE x %.3f, WRITE_SIZE x %.3f: the factors that make "
+                  "the copy kernel read and write %d bytes)" % (fs, ws, args.copy_bytes), "",
+                  "| family | read MB | write MB | total MB |" + (" B/cell |" if args.cells else ""),
+                  "|---|---|---|---|" + ("---|" if args.cells else "")]
+        for fam, v in sorted(tr.items()):
+            row = (f"| {fam} | {v['read_bytes_per_launch'] / 1e6:.1f} | {v['write_bytes_per_launch'] / 1e6:.1f} | "
+                   f"{v['bytes_per_launch'] / 1e6:.1f} |")
+            if args.cells:
+                row += f" {v['bytes_per_launch'] / args.cells:.0f} |"
+            lines.append(row)
+        jpath = os.path.join(os.path.dirname(args.out), f"traffic_{args.workload}.json")
+        with open(jpath, "w") as f:
+            json.dump({"workload": args.workload, "fetch_scale": fs, "write_scale": ws,
+                       "kernels": tr}, f, indent=1, sort_keys=True)
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(args.out)
+
+
+if __name__ == "__main__":
+    main()
