@@ -482,11 +482,11 @@ static dim3 z_grid(const mhd_handle *h, int nfblocks, int uz)
 // block and 4 planes per thread measured fastest on MI355X at 256x128x256
 // (against one field per block, 8 planes, XCD-contiguous block order).
 template <int NF, int KIND, int TYPE, int RKS>
-static void launch_sweep(mhd_handle *h, const double *a, double *b)
+static void launch_sweep(mhd_handle *h, const double *a, double *b, const double *addf = nullptr)
 {
     RKArgs rk{h->dtv, h->xfold, h->xm, h->x};
     hipLaunchKernelGGL((k_sweepz<NF, KIND, TYPE, NF, SWEEP_UZ, RKS>), z_grid(h, 1, SWEEP_UZ), dim3(TX * BY), 0,
-                       h->stream, h->g, a, b, rk, 0);
+                       h->stream, h->g, a, b, rk, 0, addf);
 }
 
 // NP passes a -> b in one temporal-blocked kernel (one field per block);
@@ -502,11 +502,13 @@ static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
     hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, nch);
 }
 
-// kk smoothing sweeps (+ fills) on a 3-field array; returns the result buffer
-static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk)
+// kk smoothing sweeps (+ fills) on a 3-field array; returns the result
+// buffer.  With addf, the result is the smoothed field plus addf (the last
+// sweep adds it; that sweep is a single pass)
+static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk, const double *addf = nullptr)
 {
     while (kk > 0) {
-        const bool two = h->zs && kk >= 2;
+        const bool two = h->zs && kk >= 2 && !(addf && kk == 2);
         const int tag = kind == KIND_RADIAL ? (two ? PT_RAD2 : PT_RAD1) : (two ? PT_EIG2 : PT_EIG1);
         PROF_BEGIN(tag);
         if (two) {
@@ -514,8 +516,9 @@ static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk)
             else launch_zsmooth<3, 2, KIND_EIGHTH, BC_AUX, -1>(h, a, b);
             kk -= 2;
         } else {
-            if (kind == KIND_RADIAL) launch_sweep<3, KIND_RADIAL, BC_AUX, -1>(h, a, b);
-            else launch_sweep<3, KIND_EIGHTH, BC_AUX, -1>(h, a, b);
+            const double *add = kk == 1 ? addf : nullptr;
+            if (kind == KIND_RADIAL) launch_sweep<3, KIND_RADIAL, BC_AUX, -1>(h, a, b, add);
+            else launch_sweep<3, KIND_EIGHTH, BC_AUX, -1>(h, a, b, add);
             kk -= 1;
         }
         PROF_END(tag);
@@ -547,8 +550,10 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
                        eta, c.hall, c.di, h->xdA, h->eA, h->hA, h->kc);
     PROF_END(PT_RHSA);
     double *E = smooth_aux(h, h->eA, h->eB, KIND_RADIAL, 4);
-    double *XH = nullptr;
-    if (c.hall) XH = smooth_aux(h, h->hA, h->hB, KIND_EIGHTH, 5);
+    // with the Hall term the last avrgh sweep leaves E + xdh, the field whose
+    // curl enters the induction equation (t3dsdsc.f:1426-1433)
+    if (c.hall) E = smooth_aux(h, h->hA, h->hB, KIND_EIGHTH, 5, E);
+    const double *XH = nullptr;
     bool vd = fmu != 0.0 || frho != 0.0 || fpr != 0.0;
     PROF_BEGIN(PT_RHSB);
     if (vd)

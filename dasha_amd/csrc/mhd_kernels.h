@@ -180,7 +180,7 @@ __device__ inline void rk_point(const RKArgs &r, long q, double d, double dt)
 
 template <int NF, int KIND, int TYPE, int FPB, int UZ, int RKS>
 __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restrict__ A, double *__restrict__ B,
-                                                    RKArgs rk, int remap)
+                                                    RKArgs rk, int remap, const double *__restrict__ addf)
 {
     BlockPos bp;
     if (remap) bp = block_pos();
@@ -275,8 +275,10 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
 #pragma unroll
             for (int u = 0; u < UZ; u++) {
                 if (k0 + u < nomz) continue;
-                if (RKS < 0) st(B + qb + sz * (k0 + u), v[u]);
-                else rk_point<RKS>(rk, qb + sz * (k0 + u), v[u], dt);
+                const long qq = qb + sz * (k0 + u);
+                // addf: the sweep's output plus a field (E + xdh for the curl of right)
+                if (RKS < 0) st(B + qq, addf ? addf[qq] + v[u] : v[u]);
+                else rk_point<RKS>(rk, qq, v[u], dt);
             }
         }
     }
@@ -526,7 +528,7 @@ __device__ inline void e_point(const double *__restrict__ X, const double *__res
     st(E + q + 2 * S, (-x2 * x6 + x3 * x5) / x1 + eta * cur[q + 2 * S]);
 }
 
-__global__ __launch_bounds__(NT) void k_rhs_a(Geo g, const double *__restrict__ X, const double *__restrict__ pri,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_rhs_a(Geo g, const double *__restrict__ X, const double *__restrict__ pri,
                                               const double *__restrict__ cur, const double *__restrict__ xresd,
                                               double eta, int hall, double cht, double *__restrict__ XD,
                                               double *__restrict__ E, double *__restrict__ XH, int kc)
