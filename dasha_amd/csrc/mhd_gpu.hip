@@ -421,12 +421,12 @@ static void prof_mark(mhd_handle *h, int tag)
 // kernel families (one name per kernel shape; _x2 = two passes in one launch)
 enum {
     PT_CURL, PT_RAD2, PT_RAD1, PT_EIG2, PT_EIG1, PT_ST2, PT_ST1, PT_ST2RK, PT_ST1RK, PT_RK, PT_FILTTT, PT_BC,
-    PT_RHSA, PT_RHSB, PT_POST, PT_DT, PT_N
+    PT_RHSA, PT_RHSB, PT_POST, PT_DT, PT_TT2, PT_N
 };
 static const char *PT_NAMES[PT_N] = {"curl", "avrg_radial_x2", "avrg_radial_x1", "avrgh_x2", "avrgh_x1",
                                      "avrg_state_x2", "avrg_state_x1", "avrg_state_x2_rk", "avrg_state_x1_rk",
                                      "rk_update", "avrgtt", "bndrytt", "rhs_flux_E_hall", "rhs_curlE_pressure",
-                                     "post_add_eq", "setdt"};
+                                     "post_add_eq", "setdt", "avrgtt_x2_eq"};
 
 static int prof_collect(mhd_handle *h)
 {
@@ -484,7 +484,7 @@ static dim3 z_grid(const mhd_handle *h, int nfblocks, int uz)
 template <int NF, int KIND, int TYPE, int RKS>
 static void launch_sweep(mhd_handle *h, const double *a, double *b, const double *addf = nullptr)
 {
-    RKArgs rk{h->dtv, h->xfold, h->xm, h->x};
+    RKArgs rk{h->dtv, h->xfold, h->xm, h->x, h->eq};
     hipLaunchKernelGGL((k_sweepz<NF, KIND, TYPE, NF, SWEEP_UZ, RKS>), z_grid(h, 1, SWEEP_UZ), dim3(TX * BY), 0,
                        h->stream, h->g, a, b, rk, 0, addf);
 }
@@ -494,7 +494,7 @@ static void launch_sweep(mhd_handle *h, const double *a, double *b, const double
 template <int NF, int NP, int KIND, int TYPE, int RKS>
 static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
 {
-    RKArgs rk{h->dtv, h->xfold, h->xm, h->x};
+    RKArgs rk{h->dtv, h->xfold, h->xm, h->x, h->eq};
     const int W = TX - 2 * NP, H = ZB - 2 * NP;
     const int nyt = h->g.yhi - h->g.ylo + 1;
     const int nch = blocks(h->NZ - 4, ZKC);
@@ -573,6 +573,7 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
         case 1: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 1>(h, a, nullptr); break;
         case 2: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 2>(h, a, nullptr); break;
         case 3: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 3>(h, a, nullptr); break;
+        case 4: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, 4>(h, a, nullptr); break;
         default: launch_zsmooth<8, 2, KIND_STATE, BC_STATE, -1>(h, a, b); std::swap(a, b); break;
         }
         PROF_END(rks >= 0 ? PT_ST2RK : PT_ST2);
@@ -619,14 +620,23 @@ static void launch_bc_tt(mhd_handle *h, double *B, const double *A, const double
 static void step_body(mhd_handle *h)
 {
     std::swap(h->x, h->xfold);   // xfold = x (1079-1084) by renaming
+    // temporal-blocked path: the last stage leaves xm = x - eq and one kernel
+    // runs both avrgtt passes with their bndrytt fills and adds eq back
+    const bool tb = h->zs && h->fuse_rk;
     for (int s = 0; s < 4; s++) {
         const double *X = s == 0 ? h->xfold : h->x;
         if (h->fuse_rk) {
-            rhs(h, X, true, false, s);
+            rhs(h, X, true, false, (tb && s == 3) ? 4 : s);
         } else {
             rhs(h, X, true, false, -1);
             rk(h, s);
         }
+    }
+    if (tb) {
+        PROF_BEGIN(PT_TT2);
+        launch_zsmooth<8, 2, KIND_STATE, BC_TT, -1>(h, h->xm, h->x);
+        PROF_END(PT_TT2);
+        return;
     }
     // perturbation smoothing (1135-1165): xm = x - eq; avrgtt x2; x = xm + eq
     Eq sub = eq_state(h, true);

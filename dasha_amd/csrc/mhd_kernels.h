@@ -158,6 +158,7 @@ struct RKArgs {
     const double *dtp;     // device dt
     const double *XF;      // xfold
     double *XM, *X;        // xm, x
+    const double *EQ;      // equilibrium rho, bx, by, bz, p (field stride S)
 };
 
 template <int RKS>
@@ -315,6 +316,8 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
     constexpr int W = TX - 2 * NP, H = ZB - 2 * NP;
     constexpr int NZW = ZKC + 4 * NP;          // z-weight planes of a chunk
     constexpr bool RAD = KIND != KIND_EIGHTH;   // reads the radial coefficient table
+    constexpr bool TT = TYPE == BC_TT;          // post-step passes on x - eq (bndrytt fills)
+    constexpr int XT = TT ? BC_AUX : TYPE;      // x-face rule of the pass
     __shared__ double s0[3][ZB][TX];           // level 0 by plane mod 3
     __shared__ double sl[NP][2][ZB][TX];       // level p (1..NP) by plane parity
     __shared__ double szw[2][NZW];             // z weights of the chunk's planes
@@ -366,13 +369,26 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
     const double *xf = RKS >= 0 ? rk.XF + m * S : nullptr;
     double *xmo = RKS >= 0 ? rk.XM + m * S : nullptr;
     double *xo = RKS >= 0 ? rk.X + m * S : nullptr;
-    // final value v of point e: store it, or apply the RK stage update
+    // equilibrium of field m (rho, B, p; none for the momentum)
+    const double *eqm = ((TT || RKS == 4) && (m == 0 || m >= 4)) ? rk.EQ + (m == 0 ? 0 : m - 3) * S : nullptr;
+    // final value v of point e: store it (plus eq after the post-step
+    // passes, t3dsdsc.f:1157-1164), or apply the RK stage update; RKS 4 is the
+    // last stage followed by xm = x - eq (t3dsdsc.f:1131, 1138-1145)
     auto put = [&](unsigned e, double v) {
-        if (RKS < 0) st(fldp(bo, e), v);
+        if (RKS < 0) st(fldp(bo, e), (TT && eqm) ? v + fld(eqm, e) : v);
         else if (RKS == 0) { const double f = fld(xf, e); st(fldp(xmo, e), f + v * dt / 6.); st(fldp(xo, e), f + v * dt / 2.); }
         else if (RKS == 1) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt / 2.); }
         else if (RKS == 2) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt); }
-        else st(fldp(xo, e), fld(xmo, e) + v * dt / 6.);
+        else if (RKS == 3) st(fldp(xo, e), fld(xmo, e) + v * dt / 6.);
+        else { const double xn = fld(xmo, e) + v * dt / 6.; st(fldp(xmo, e), eqm ? xn - fld(eqm, e) : xn); }
+    };
+    // x-left columns of a row from the level held by lanes 2, 3, 4 (every
+    // lane of the wavefront calls; the window holds the left face)
+    auto tt_left = [&](double lv, double v) {
+        const int l2 = 2 - i0;
+        const double p2 = lane_of(lv, l2), p3 = lane_of(lv, l2 + 1);
+        const double x1 = xleft1(g, p2, p3, lane_of(lv, l2 + 2));
+        return i == 1 ? x1 : i == 0 ? xleft0(g, x1, p2, p3) : v;
     };
 
     double q[NP + 1][3];   // level p at its three latest planes, oldest first
@@ -439,7 +455,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
                                      difw(q[p - 1][0], f0, q[p - 1][2], szw[0][k - kz0], szw[1][k - kz0]);
                     v = f0 + c * s;
                 }
-                if (xft) v = row_xrule_rl<TYPE>(g, i, i0, vx && yin, v);
+                if (xft) v = row_xrule_rl<XT>(g, i, i0, vx && yin, v);
                 if (!g.periodic && kin) {
                     // image rows: 0 on the outer row, half of row 2 (NY-3) on the next
                     sl[p - 1][k & 1][r][lane] = v;
@@ -454,16 +470,32 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
                 // z faces above the interior: plane NZ-2 is half of NZ-3, NZ-1 is 0
                 if (k == g.NZ - 2) v = 0.5 * q[p][2];
                 else if (k == g.NZ - 1) v = 0.0;
+                // bndrytt skips x < x_first in its y and z rules: on image rows
+                // and z-face planes the x-left columns extrapolate this pass's
+                // input of that row and plane (t3dsdsc.f:2108-2168)
+                if (TT && i0 < 2 && (img || !kin)) v = tt_left(f0, v);
                 q[p][0] = q[p][1]; q[p][1] = q[p][2]; q[p][2] = v;
-                // below: planes 1 and 0 follow from plane 2
-                if (k == 2) { q[p][1] = 0.5 * v; q[p][0] = 0.0; }
-                if (p == NP && own) {
-                    if (k >= kb && k < ke) put(qc + sz * (unsigned)k, v);
-                    // z-face planes: 0, 1 after plane 2; NZ-2, NZ-1 after NZ-3
+                // below: planes 1 and 0 follow from plane 2 (bndrytt: the x-left
+                // columns of plane 1 from the input's plane 1; plane 0 stays 0
+                // as the input is 0 there from the second pass on)
+                if (k == 2 && p < NP) {
+                    q[p][1] = 0.5 * v;
+                    q[p][0] = 0.0;
+                    if (TT && i0 < 2) q[p][1] = tt_left(q[p - 1][0], q[p][1]);
+                }
+                if (p == NP) {
+                    // the last level's z-face planes: 1 (NZ-2) is half of plane
+                    // 2 (NZ-3), 0 (NZ-1) is 0; under bndrytt the x-left columns
+                    // of planes 1 and NZ-2 extrapolate the previous level there
                     const bool lo = k == 2 && kb == 2, hi = k == g.NZ - 3 && ke == g.NZ - 2;
-                    if (lo || hi) {
-                        put(qc + sz * (unsigned)(lo ? 1 : g.NZ - 2), 0.5 * v);
-                        put(qc + sz * (unsigned)(lo ? 0 : g.NZ - 1), 0.0);
+                    double vf = 0.5 * v;
+                    if (TT && i0 < 2 && (lo || hi)) vf = tt_left(lo ? q[p - 1][0] : q[p - 1][2], vf);
+                    if (own) {
+                        if (k >= kb && k < ke) put(qc + sz * (unsigned)k, v);
+                        if (lo || hi) {
+                            put(qc + sz * (unsigned)(lo ? 1 : g.NZ - 2), vf);
+                            put(qc + sz * (unsigned)(lo ? 0 : g.NZ - 1), 0.0);
+                        }
                     }
                 }
             }
