@@ -525,12 +525,15 @@ __device__ inline Rec rec_at(const double *__restrict__ X, const double *__restr
     double x1 = X[q];
     r.x2 = X[q + S]; r.x3 = X[q + 2 * S]; r.x4 = X[q + 3 * S]; r.p = X[q + 7 * S];
     double dp = r.p - pri[q];
-    r.t22 = r.x2 * r.x2 / x1 + dp;
-    r.t23 = r.x2 * r.x3 / x1;
-    r.t24 = r.x2 * r.x4 / x1;
-    r.t33 = r.x3 * r.x3 / x1 + dp;
-    r.t34 = r.x4 * r.x3 / x1;
-    r.t44 = r.x4 * r.x4 / x1 + dp;
+    // the six quotients by rho from one rounded reciprocal (divc: equal to
+    // the IEEE quotients)
+    const double y = 1.0 / x1;
+    r.t22 = divc(r.x2 * r.x2, x1, y) + dp;
+    r.t23 = divc(r.x2 * r.x3, x1, y);
+    r.t24 = divc(r.x2 * r.x4, x1, y);
+    r.t33 = divc(r.x3 * r.x3, x1, y) + dp;
+    r.t34 = divc(r.x4 * r.x3, x1, y);
+    r.t44 = divc(r.x4 * r.x4, x1, y) + dp;
     return r;
 }
 
@@ -555,9 +558,10 @@ __device__ inline void e_point(const double *__restrict__ X, const double *__res
 {
     double x1 = X[q], x2 = X[q + S], x3 = X[q + 2 * S], x4 = X[q + 3 * S];
     double x5 = X[q + 4 * S], x6 = X[q + 5 * S], x7 = X[q + 6 * S];
-    st(E + q, (-x3 * x7 + x4 * x6) / x1 + eta * cur[q]);
-    st(E + q + S, (-x4 * x5 + x2 * x7) / x1 + eta * cur[q + S]);
-    st(E + q + 2 * S, (-x2 * x6 + x3 * x5) / x1 + eta * cur[q + 2 * S]);
+    const double y = 1.0 / x1;
+    st(E + q, divc(-x3 * x7 + x4 * x6, x1, y) + eta * cur[q]);
+    st(E + q + S, divc(-x4 * x5 + x2 * x7, x1, y) + eta * cur[q + S]);
+    st(E + q + 2 * S, divc(-x2 * x6 + x3 * x5, x1, y) + eta * cur[q + 2 * S]);
 }
 
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_rhs_a(Geo g, const double *__restrict__ X, const double *__restrict__ pri,
@@ -645,9 +649,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             xd3 = xd3 - xresd[qk + 2 * S] + c1 * x6 - c2 * x5;
             st(XD + qk, xd0); st(XD + qk + S, xd1); st(XD + qk + 2 * S, xd2); st(XD + qk + 3 * S, xd3);
             if (hall) {
-                hv[0] = cht * (c2 * x7 - c3 * x6 - DXL(LP)) / x1;
-                hv[1] = cht * (c3 * x5 - c1 * x7 - DYL(LP)) / x1;
-                hv[2] = cht * (c1 * x6 - c2 * x5 - DZQ(qp)) / x1;
+                const double y = 1.0 / x1;
+                hv[0] = divc(cht * (c2 * x7 - c3 * x6 - DXL(LP)), x1, y);
+                hv[1] = divc(cht * (c3 * x5 - c1 * x7 - DYL(LP)), x1, y);
+                hv[2] = divc(cht * (c1 * x6 - c2 * x5 - DZQ(qp)), x1, y);
             }
 #undef DZQ
 #undef DXL
@@ -699,8 +704,14 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
     double qe1[5], qe2[5], qp[5], qvz[5], qvx[5], qvy[5], qrh[5];
     auto load_q = [&](int s, long q) {
         double r = X[q];
-        qe1[s] = ept(q, 0); qe2[s] = ept(q, 1); qp[s] = X[q + 7 * S]; qvz[s] = X[q + 3 * S] / r;
-        if (VD) { qvx[s] = X[q + S] / r; qvy[s] = X[q + 2 * S] / r; qrh[s] = r; }
+        qe1[s] = ept(q, 0); qe2[s] = ept(q, 1); qp[s] = X[q + 7 * S];
+        if (VD) {
+            const double y = 1.0 / r;
+            qvz[s] = divc(X[q + 3 * S], r, y);
+            qvx[s] = divc(X[q + S], r, y); qvy[s] = divc(X[q + 2 * S], r, y); qrh[s] = r;
+        } else {
+            qvz[s] = X[q + 3 * S] / r;
+        }
     };
     if (ld) {
 #pragma unroll
