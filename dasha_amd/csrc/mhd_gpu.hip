@@ -336,6 +336,7 @@ struct mhd_handle {
     int kc = 16;                           // z planes per tile chunk
     int fuse_rk = 1;                       // RK update fused into the last avrg pass
     int zs = 1;                            // pairs of smoothing passes in one temporal-blocked kernel
+    int rb2 = 1;                           // windowed curl/pressure kernels for right, part 2
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -556,12 +557,26 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
     const double *XH = nullptr;
     bool vd = fmu != 0.0 || frho != 0.0 || fpr != 0.0;
     PROF_BEGIN(PT_RHSB);
-    if (vd)
+    if (vd) {
         hipLaunchKernelGGL((k_rhs_b<true>), tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->cur, E, XH, eta,
                            c.gamma, fmu, frho, fpr, h->xdA, h->kc);
-    else
+    } else if (h->rb2) {
+        // -curl(E + xdh), the pressure equation, and the fill of xdif(1..4)
+        const int nyt = h->g.yhi - h->g.ylo + 1;
+        const int rb = h->rb2 == 2 ? 8 : 16;
+        dim3 grid((unsigned)blocks(h->NX, TX - 4), (unsigned)blocks(nyt, rb - 4), (unsigned)blocks(h->NZ - 4, RB_KC));
+        if (rb == 8) {
+            hipLaunchKernelGGL((k_rhs_b2<0, 8>), grid, dim3(TX * 8), 0, h->stream, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
+            hipLaunchKernelGGL((k_rhs_b2<1, 8>), grid, dim3(TX * 8), 0, h->stream, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
+        } else {
+            hipLaunchKernelGGL((k_rhs_b2<0, 16>), grid, dim3(TX * 16), 0, h->stream, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
+            hipLaunchKernelGGL((k_rhs_b2<1, 16>), grid, dim3(TX * 16), 0, h->stream, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
+        }
+        hipLaunchKernelGGL(k_fill_state, dim3(blocks(face_count(h->g), 256)), dim3(256), 0, h->stream, h->g, h->xdA, 4);
+    } else {
         hipLaunchKernelGGL((k_rhs_b<false>), tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->cur, E, XH, eta,
                            c.gamma, fmu, frho, fpr, h->xdA, h->kc);
+    }
     PROF_END(PT_RHSB);
     double *a = h->xdA, *b = h->xdB;
     if (smooth_xdif && h->zs) {
@@ -837,6 +852,10 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     if (const char *e = getenv("MHD_KC")) h->kc = std::max(1, atoi(e));
     if (const char *e = getenv("MHD_FUSE_RK")) h->fuse_rk = atoi(e);
     if (const char *e = getenv("MHD_ZS")) h->zs = atoi(e);
+    // windowed right-part-2 kernels pay off on large volumes (measured at
+    // 512x256x512 against 256x128x256)
+    h->rb2 = h->S >= (1L << 25) ? 1 : 0;
+    if (const char *e = getenv("MHD_RB2")) h->rb2 = atoi(e);
     h->pt.resize(PT_N);
     for (int t = 0; t < PT_N; t++) h->pt[t] = Prof{PT_NAMES[t], 0.0, 0};
     *out = h;

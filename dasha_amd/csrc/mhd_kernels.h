@@ -804,6 +804,177 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// right, part 2 without viscosity and diffusion, in the windowed z-march form
+// of the smoothing passes:
+//   PART 0  xdif(5..7) = -curl(E + xdh)          (t3dsdsc.f:1435-1452)
+//   PART 1  xdif(8), the pressure equation       (t3dsdsc.f:1550-1566)
+// each with the bndry fill of its fields (t3dsdsc.f:1665; xdif(1..4) are
+// filled by k_fill_state).  A block of 64 x ZB threads covers 64 columns x ZB
+// rows and owns the inner 60 x ZB-4 (the 5-point stencils reach 2 points).
+// Iteration t loads plane t, forms the x- and y-derivative parts of plane t
+// from the plane in LDS, and completes plane t-2 with the z derivatives from
+// the register queues.  Partial sums keep the reference's order of
+// operations, so the results are those of the one-pass formulas.
+// ---------------------------------------------------------------------------
+constexpr int RB_KC = 32;        // interior planes per block
+
+template <int PART, int RB>
+__global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restrict__ X, const double *__restrict__ cur,
+                                                   const double *__restrict__ E, double eta, double gam,
+                                                   double *__restrict__ XD)
+{
+    constexpr int W = TX - 4, H = RB - 4;
+    constexpr int NO = PART == 0 ? 3 : 1;        // output fields
+    __shared__ double sp[2][3][RB][TX];         // the plane's fields, by plane parity
+    __shared__ double sx[NO][2][TX];            // rows 2 / NY-3 of the outputs
+    const int lane = threadIdx.x % TX;
+    const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
+    const int nomx = blockIdx.x * W;
+    const int i0 = max(0, min(nomx - 2, g.NX - TX));
+    const int i = i0 + lane;
+    const bool vx = i < g.NX;
+    const bool ownx = vx && i >= nomx && i < nomx + W;
+    const int nomy = g.ylo + blockIdx.y * H;
+    const int j0 = g.periodic ? max(g.ylo - 2, min(nomy - 2, g.yhi + 3 - RB)) : max(0, min(nomy - 2, g.NY - RB));
+    const int j = j0 + r;
+    const bool vy = g.periodic || j < g.NY;
+    const bool own = ownx && vy && j >= nomy && j < nomy + H && j <= g.yhi;
+    const bool ld = vx && vy;
+    const int jp = vy ? rmap(g, j) : 0;
+    const bool yin = g.periodic || (j >= 2 && j <= g.NY - 3);
+    const bool pin = ld && yin && i >= 2 && i <= g.NX - 3;
+    const bool img = !g.periodic && vy && (j < 2 || j > g.NY - 3);
+    const bool xft = i0 < 2 || i0 + TX > g.NX - 2;
+    const int rm1 = max(r - 1, 0), rm2 = max(r - 2, 0), rp1 = min(r + 1, RB - 1), rp2 = min(r + 2, RB - 1);
+    const int lm1 = max(lane - 1, 0), lm2 = max(lane - 2, 0), lp1 = min(lane + 1, TX - 1), lp2 = min(lane + 2, TX - 1);
+    const long S = g.S;
+    const unsigned sz = (unsigned)g.sz;
+    const unsigned qc = ld ? (unsigned)i + (unsigned)g.sy * (unsigned)jp : 0u;
+    const double ax0 = pin ? g.x1a[i] : 0.0, ax1 = pin ? g.x1b[i] : 0.0, ax2 = pin ? g.x1c[i] : 0.0,
+                 ax3 = pin ? g.x1d[i] : 0.0;
+    const double ay0 = g.y1a[jp], ay1 = g.y1b[jp], ay2 = g.y1c[jp], ay3 = g.y1d[jp];   // row-uniform
+    const int kb = 2 + blockIdx.z * RB_KC, ke = min(kb + RB_KC, g.NZ - 2);
+    const int t0 = kb - 2, t1 = ke + 1;
+#define DXS(f) fd5(sp[b][f][r][lm2], sp[b][f][r][lm1], sp[b][f][r][lane], sp[b][f][r][lp1], sp[b][f][r][lp2], ax0, ax1, ax2, ax3)
+#define DYS(f) fd5(sp[b][f][rm2][lane], sp[b][f][rm1][lane], sp[b][f][r][lane], sp[b][f][rp1][lane], sp[b][f][rp2][lane], ay0, ay1, ay2, ay3)
+#define DZQ(a) fd5(a[0], a[1], a[2], a[3], a[4], az0, az1, az2, az3)
+    // z queues (planes t-4..t) and partial sums of planes t-1, t
+    double qa[5], qb[5];
+    double pa[2], pb[2], pc[2];
+#pragma unroll
+    for (int s = 0; s < 5; s++) qa[s] = qb[s] = 0.0;
+    pa[0] = pa[1] = pb[0] = pb[1] = pc[0] = pc[1] = 0.0;
+    // raw values of the next plane
+    double n0, n1, n2, n3 = 0.0, n4 = 0.0;
+    auto fetch = [&](int t) {
+        const unsigned e = qc + sz * (unsigned)t;
+        if (PART == 0) {
+            n0 = fld(E, e); n1 = fld(E + S, e); n2 = fld(E + 2 * S, e);
+        } else {
+            n0 = fld(X, e); n1 = fld(X + S, e); n2 = fld(X + 2 * S, e); n3 = fld(X + 3 * S, e); n4 = fld(X + 7 * S, e);
+        }
+    };
+    fetch(t0);
+    for (int t = t0; t <= t1; t++) {
+        // plane t: LDS fields and queue entries of the own point
+        double f0, f1, f2, za, zb;
+        if (PART == 0) {
+            f0 = n0; f1 = n1; f2 = n2;           // e1, e2, e3
+            za = n0; zb = n1;                     // z derivatives of e1 and e2
+        } else {
+            const double y = 1.0 / n0;
+            f0 = n4;                              // p
+            f1 = divc(n1, n0, y);                 // vx
+            f2 = divc(n2, n0, y);                 // vy
+            za = n4; zb = divc(n3, n0, y);        // p, vz
+        }
+        if (!ld) { f0 = f1 = f2 = za = zb = 0.0; }
+        if (t < t1) fetch(t + 1);
+        const int b = t & 1;
+        sp[b][0][r][lane] = f0; sp[b][1][r][lane] = f1; sp[b][2][r][lane] = f2;
+#pragma unroll
+        for (int s = 0; s < 4; s++) { qa[s] = qa[s + 1]; qb[s] = qb[s + 1]; }
+        qa[4] = za; qb[4] = zb;
+        // J^2 of the output plane t-2 (pressure equation)
+        double j2 = 0.0;
+        if (PART == 1 && t - 2 >= kb) {
+            const unsigned e = qc + sz * (unsigned)(t - 2);
+            const double c1 = fld(cur, e), c2 = fld(cur + S, e), c3 = fld(cur + 2 * S, e);
+            j2 = c1 * c1 + c2 * c2 + c3 * c3;
+        }
+        __syncthreads();
+        // x/y parts of plane t
+        double xa = 0.0, xb = 0.0, xc = 0.0;
+        if (pin && t >= 2 && t <= g.NZ - 3) {
+            if (PART == 0) {
+                xa = -DYS(2);                     // xd5 = -d/dy e3 + ...
+                xb = DXS(2);                      // xd6 = ... + d/dx e3
+                xc = -DXS(1) + DYS(0);            // xd7 = -d/dx e2 + d/dy e1
+            } else {
+                xa = -DXS(0) * f1 - DYS(0) * f2;  // -dp/dx vx - dp/dy vy
+                xb = -DXS(1) - DYS(2);            // -dvx/dx - dvy/dy
+            }
+        }
+        // plane k = t-2: complete, fill, store
+        const int k = t - 2;
+        if (k >= kb) {
+            double v[NO];
+            if (pin) {
+                const double az0 = g.z1a[k], az1 = g.z1b[k], az2 = g.z1c[k], az3 = g.z1d[k];
+                if (PART == 0) {
+                    v[0] = pa[0] + DZQ(qb);               // xd5
+                    v[NO > 1 ? 1 : 0] = -DZQ(qa) + pb[0]; // xd6
+                    v[NO > 2 ? 2 : 0] = pc[0];            // xd7
+                } else {
+                    const double hs0 = qb[2], p0 = qa[2];
+                    v[0] = pa[0] - DZQ(qa) * hs0 + (pb[0] - DZQ(qb)) * gam * p0 + (gam - 1.) * eta * j2;
+                }
+            } else {
+#pragma unroll
+                for (int o = 0; o < NO; o++) v[o] = 0.0;
+            }
+            if (xft) {
+#pragma unroll
+                for (int o = 0; o < NO; o++) v[o] = row_xrule_rl<BC_STATE>(g, i, i0, vx && yin, v[o]);
+            }
+            if (!g.periodic) {
+                if (j == 2 || j == g.NY - 3) {
+#pragma unroll
+                    for (int o = 0; o < NO; o++) sx[o][j == 2 ? 0 : 1][lane] = v[o];
+                }
+                __syncthreads();
+                if (img) {
+#pragma unroll
+                    for (int o = 0; o < NO; o++)
+                        v[o] = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sx[o][j < 2 ? 0 : 1][lane];
+                }
+            }
+            if (own) {
+                const unsigned e = qc + sz * (unsigned)k;
+                const int m0 = PART == 0 ? 4 : 7;
+#pragma unroll
+                for (int o = 0; o < NO; o++) st(fldp(XD + (m0 + o) * S, e), v[o]);
+                if (k == 2 || k == g.NZ - 3) {
+                    const unsigned eh = qc + sz * (unsigned)(k == 2 ? 1 : g.NZ - 2);
+                    const unsigned e0 = qc + sz * (unsigned)(k == 2 ? 0 : g.NZ - 1);
+#pragma unroll
+                    for (int o = 0; o < NO; o++) {
+                        st(fldp(XD + (m0 + o) * S, eh), 0.5 * v[o]);
+                        st(fldp(XD + (m0 + o) * S, e0), 0.0);
+                    }
+                }
+            }
+        }
+        pa[0] = pa[1]; pb[0] = pb[1]; pc[0] = pc[1];
+        pa[1] = xa; pb[1] = xb; pc[1] = xc;
+    }
+#undef DXS
+#undef DYS
+#undef DZQ
+}
+
+
+// ---------------------------------------------------------------------------
 // post-step smoothing of the perturbation (t3dsdsc.f:1135-1165, avrgtt
 // 5754-5862): one sweep on 8 fields of A (or of x - eq on the fly), interior
 // only; its bndrytt fill runs in k_bc_tt.
@@ -945,6 +1116,33 @@ __global__ void k_bc_tt(Geo g, double *B, const double *A, const double *X, Eq e
         TTEval ev{g, B + m * g.S, A ? A + m * g.S : nullptr, X ? X + m * g.S : nullptr, X ? eq.f[m] : nullptr};
         B[q + m * g.S] = ev.Zv(i, j, k);
     }
+}
+
+// bndry fill of xdif(1..4) (t3dsdsc.f:1677-1985 with the x-left rule
+// mirrored): the face points from the interior values the flux kernel left,
+// in the order of the fills inside the sweeps (x rule on interior rows, the
+// image-row rule from rows 2 / NY-3, then the z rule from planes 2 / NZ-3)
+__device__ inline double state_face(const Geo &g, const double *G, int i, int j, int k)
+{
+    const int kk = k < 2 ? 2 : k > g.NZ - 3 ? g.NZ - 3 : k;
+    const int jj = g.periodic ? j : (j < 2 ? 2 : j > g.NY - 3 ? g.NY - 3 : j);
+    double v;
+    const long r = g.sy * rmap(g, jj) + g.sz * kk;
+    if (i >= 2 && i <= g.NX - 3) v = G[r + i];
+    else if (i == 0 || i == g.NX - 1) v = 0.0;
+    else v = 0.5 * G[r + (i == 1 ? 2 : g.NX - 3)];
+    if (jj != j) v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * v;
+    if (kk != k) v = (k == 0 || k == g.NZ - 1) ? 0.0 : 0.5 * v;
+    return v;
+}
+
+__global__ void k_fill_state(Geo g, double *__restrict__ XD, int nf)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!face_point(g, t, i, j, k)) return;
+    const long q = IDX(g, i, j, k);
+    for (int m = 0; m < nf; m++) XD[q + m * g.S] = state_face(g, XD + m * g.S, i, j, k);
 }
 
 } // namespace mhd
