@@ -50,6 +50,7 @@ def per_cell_bytes(case):
         "rhs_flux_E_hall": 8 * (15 + 10) if hall else 8 * (15 + 7),   # U8,p_i,J3,xresd3 -> xdif1-4,E3(,xdh3)
         "rhs_curlE_pressure": 8 * ((18 if hall else 15) + 8),         # E3(,xdh3),U5,J3,xdif1-4 -> xdif8
         "avrgtt": 8 * ((8 + 5 + 8) + (8 + 8)) // 2,                   # mean of the two avrgtt passes
+        "avrgtt_x2_eq": 8 * 2 * (8 + 5 + 8),    # both post-step passes on x - eq and x = xm + eq
         "post_add_eq": 8 * (8 + 5 + 8),
         "setdt": 8 * 8,
     }

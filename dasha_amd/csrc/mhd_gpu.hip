@@ -337,6 +337,7 @@ struct mhd_handle {
     int fuse_rk = 1;                       // RK update fused into the last avrg pass
     int zs = 1;                            // pairs of smoothing passes in one temporal-blocked kernel
     int rb2 = 1;                           // windowed curl/pressure kernels for right, part 2
+    int zkc = 32;                          // interior planes per block of the temporal-blocked kernels
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -498,9 +499,12 @@ static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
     RKArgs rk{h->dtv, h->xfold, h->xm, h->x, h->eq};
     const int W = TX - 2 * NP, H = ZB - 2 * NP;
     const int nyt = h->g.yhi - h->g.ylo + 1;
-    const int nch = blocks(h->NZ - 4, ZKC);
+    const int nch = blocks(h->NZ - 4, h->zkc);
     dim3 grid((unsigned)blocks(h->NX, W), (unsigned)blocks(nyt, H), (unsigned)(nch * NF));
-    hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, nch);
+    if (h->zkc == 64)
+        hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, 64>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, nch);
+    else
+        hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, 32>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, nch);
 }
 
 // kk smoothing sweeps (+ fills) on a 3-field array; returns the result
@@ -855,6 +859,8 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     // windowed right-part-2 kernels pay off on large volumes (measured at
     // 512x256x512 against 256x128x256)
     h->rb2 = h->S >= (1L << 25) ? 1 : 0;
+    h->zkc = h->S >= (1L << 25) ? 64 : 32;
+    if (const char *e = getenv("MHD_ZKC")) h->zkc = atoi(e) == 64 ? 64 : 32;
     if (const char *e = getenv("MHD_RB2")) h->rb2 = atoi(e);
     h->pt.resize(PT_N);
     for (int t = 0; t < PT_N; t++) h->pt[t] = Prof{PT_NAMES[t], 0.0, 0};

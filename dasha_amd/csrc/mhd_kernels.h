@@ -307,14 +307,13 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
 // ---------------------------------------------------------------------------
 constexpr int ZB = 16;            // rows (wavefronts) per block
 constexpr int ZPF = 3;            // planes loaded ahead
-constexpr int ZKC = 32;           // interior planes per block
 
-template <int NP, int KIND, int TYPE, int RKS>
+template <int NP, int KIND, int TYPE, int RKS, int KC>
 __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
                                                      RKArgs rk, int nch)
 {
     constexpr int W = TX - 2 * NP, H = ZB - 2 * NP;
-    constexpr int NZW = ZKC + 4 * NP;          // z-weight planes of a chunk
+    constexpr int NZW = KC + 4 * NP;           // z-weight planes of a chunk
     constexpr bool RAD = KIND != KIND_EIGHTH;   // reads the radial coefficient table
     constexpr bool TT = TYPE == BC_TT;          // post-step passes on x - eq (bndrytt fills)
     constexpr int XT = TT ? BC_AUX : TYPE;      // x-face rule of the pass
@@ -354,7 +353,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
     const double wpy = pin ? g.ywp[jp] : 0.0, wmy = pin ? g.ywm[jp] : 0.0;
     const bool bcomp = KIND == KIND_STATE && m >= 4 && m <= 6;
     const double cb = (KIND == KIND_STATE && pin) ? g.cbx[i] : 0.0;
-    const int kb = 2 + chunk * ZKC, ke = min(kb + ZKC, g.NZ - 2);
+    const int kb = 2 + chunk * KC, ke = min(kb + KC, g.NZ - 2);
     const int t0 = kb - NP, t1 = ke - 1 + NP;
     const int kz0 = t0 - NP;                    // lowest plane a level touches
     for (int n = threadIdx.x; n < NZW; n += TX * ZB) {

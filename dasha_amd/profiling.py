@@ -22,17 +22,18 @@ import re
 # kernel symbol (demangled prefix) -> family name used by bench.py
 _FAMILIES = [
     (r"k_curl\b", "curl"),
-    (r"k_zsmooth<2, 0, 0, -1>", "avrg_radial_x2"),
+    (r"k_zsmooth<2, 0, 0, -1[,>]", "avrg_radial_x2"),
     (r"k_sweepz<3, 0, 0, 3, 4, -1>", "avrg_radial_x1"),
-    (r"k_zsmooth<2, 1, 0, -1>", "avrgh_x2"),
+    (r"k_zsmooth<2, 1, 0, -1[,>]", "avrgh_x2"),
     (r"k_sweepz<3, 1, 0, 3, 4, -1>", "avrgh_x1"),
-    (r"k_zsmooth<2, 2, 1, -1>", "avrg_state_x2"),
+    (r"k_zsmooth<2, 2, 1, -1[,>]", "avrg_state_x2"),
     (r"k_sweepz<8, 2, 1, 8, 4, -1>", "avrg_state_x1"),
-    (r"k_zsmooth<2, 2, 1, [0-3]>", "avrg_state_x2_rk"),
+    (r"k_zsmooth<2, 2, 1, [0-4][,>]", "avrg_state_x2_rk"),
     (r"k_sweepz<8, 2, 1, 8, 4, [0-3]>", "avrg_state_x1_rk"),
+    (r"k_zsmooth<2, 2, 2, -1[,>]", "avrgtt_x2_eq"),
     (r"k_rk\b", "rk_update"),
     (r"k_rhs_a\b", "rhs_flux_E_hall"),
-    (r"k_rhs_b<", "rhs_curlE_pressure"),
+    (r"k_rhs_b<|k_rhs_b2<|k_fill_state\b", "rhs_curlE_pressure"),
     (r"k_smooth_tt<", "avrgtt"),
     (r"k_bc_tt\b", "bndrytt"),
     (r"k_add_eq\b", "post_add_eq"),
