@@ -91,6 +91,26 @@ def cpu_baseline(case_name: str, steps: int, threads: int):
                       f"by the C/OpenMP oracle, {el:.2f} s"}
 
 
+def time_config(name: str, steps: int, warmup: int):
+    """the same timed loop on another configuration (no profiling)"""
+    from dasha_amd import harris
+    from dasha_amd.mhd import MHDStep
+    case = harris.CASES[name]()
+    g = MHDStep(case)
+    g.init_residual()
+    dt0 = g.compute_dt()
+    tm, _, _ = g.advance(warmup, 0.0, dt0)
+    tm, _, ms = g.advance(steps, tm, dt0)
+    g.close()
+    value = case.cells * steps / (ms / 1e3)
+    b_alg = B_ALG_HALL if case.hall else B_ALG_NOHALL
+    print(f"[bench] {case.name}: {ms / steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": f"{case.name} Harris sheet, RK4 Hall MHD step (stepon+setdt)",
+            "grid": [case.mx, case.ny, case.mz], "steps": steps, "warmup": warmup,
+            "value": value, "unit": "cell-updates/s", "ms_per_step": ms / steps,
+            "step_roofline_frac": value * b_alg / HBM_PEAK}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +121,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event run")
     ap.add_argument("--copy-probe", action="store_true", help="also measure a plain device copy")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the 512x256x512 timing added to the 256x128x256 line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
@@ -181,6 +203,9 @@ def main():
     if args.copy_probe:
         from dasha_amd.mhd import copy_rate
         out["copy_probe_GBps"] = copy_rate() / 1e9
+    g.close()
+    if args.config == "C2" and not args.no_c3:
+        out["C3"] = time_config("C3", 10, 2)
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_steps, threads)

@@ -327,6 +327,7 @@ struct mhd_handle {
     double *cur = nullptr;                 // buffer holding the last current
     double *red = nullptr;                 // reduction scratch
     double *crad = nullptr;                // radial smoothing coefficient table
+    double *stage = nullptr;               // pinned staging buffer of host transfers
     double *dtv = nullptr;                 // device dt slots
     int nred = 0;
     std::vector<double> wx, wy, wz;        // energy weights (host copy)
@@ -685,29 +686,66 @@ static void dt_kernels(mhd_handle *h, int advance_time)
 }
 
 // ---- host <-> device layout conversion ----
+// Host arrays are the reference's padded layout (mx+4, my, mz+4); the device
+// keeps the physical range only.  Transfers go through a pinned staging buffer
+// in slabs of whole planes (one contiguous DMA per slab) and the rows are
+// scattered / gathered on the host.
+static constexpr size_t STAGE_BYTES = 64u << 20;
+
+static int ensure_stage(mhd_handle *h)
+{
+    if (h->stage) return 0;
+    HCHK(hipHostMalloc((void **)&h->stage, STAGE_BYTES, hipHostMallocDefault));
+    return 0;
+}
+
 static int upload_fields(mhd_handle *h, double *dst, const double *src, int nf)
 {
-    long px = h->cfg.mx + 4, plane = px * h->NY;
+    const long px = h->cfg.mx + 4, plane = px * h->NY, total = plane * (h->cfg.mz + 4);
+    const long dplane = (long)h->NX * h->NY;
+    const int nzc = (int)std::max<long>(1, (long)(STAGE_BYTES / sizeof(double)) / dplane);
+    if (ensure_stage(h)) return MHD_ERR_HIP;
     for (int m = 0; m < nf; m++) {
-        const double *s = src + (long)m * plane * (h->cfg.mz + 4) + 2 * plane + 2;
-        HCHK(hipMemcpy2DAsync(dst + m * h->S, h->NX * sizeof(double), s, px * sizeof(double), h->NX * sizeof(double),
-                              (size_t)h->NY * h->NZ, hipMemcpyHostToDevice, h->stream));
+        const double *s = src + (long)m * total;
+        for (int z0 = 0; z0 < h->NZ; z0 += nzc) {
+            const int z1 = std::min(h->NZ, z0 + nzc);
+            for (int k = z0; k < z1; k++)
+                for (int j = 0; j < h->NY; j++)
+                    memcpy(h->stage + ((long)(k - z0) * h->NY + j) * h->NX, s + (k + 2) * plane + j * px + 2,
+                           sizeof(double) * h->NX);
+            HCHK(hipMemcpyAsync(dst + m * h->S + (long)z0 * dplane, h->stage, sizeof(double) * (z1 - z0) * dplane,
+                                hipMemcpyHostToDevice, h->stream));
+            HCHK(hipStreamSynchronize(h->stream));
+        }
     }
-    HCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
 static int download_fields(mhd_handle *h, double *dst, const double *src, int nf)
 {
-    long px = h->cfg.mx + 4, plane = px * h->NY;
-    long total = plane * (h->cfg.mz + 4);
+    const long px = h->cfg.mx + 4, plane = px * h->NY;
+    const long total = plane * (h->cfg.mz + 4);
+    const long dplane = (long)h->NX * h->NY;
+    const int nzc = (int)std::max<long>(1, (long)(STAGE_BYTES / sizeof(double)) / dplane);
+    if (ensure_stage(h)) return MHD_ERR_HIP;
     for (int m = 0; m < nf; m++) {
         double *d = dst + (long)m * total;
-        memset(d, 0, sizeof(double) * total);
-        HCHK(hipMemcpy2DAsync(d + 2 * plane + 2, px * sizeof(double), src + m * h->S, h->NX * sizeof(double),
-                              h->NX * sizeof(double), (size_t)h->NY * h->NZ, hipMemcpyDeviceToHost, h->stream));
+        // the ghost planes and columns the device does not hold read 0
+        memset(d, 0, sizeof(double) * 2 * plane);
+        memset(d + (h->cfg.mz + 2) * plane, 0, sizeof(double) * 2 * plane);
+        for (int z0 = 0; z0 < h->NZ; z0 += nzc) {
+            const int z1 = std::min(h->NZ, z0 + nzc);
+            HCHK(hipMemcpyAsync(h->stage, src + m * h->S + (long)z0 * dplane, sizeof(double) * (z1 - z0) * dplane,
+                                hipMemcpyDeviceToHost, h->stream));
+            HCHK(hipStreamSynchronize(h->stream));
+            for (int k = z0; k < z1; k++)
+                for (int j = 0; j < h->NY; j++) {
+                    double *row = d + (k + 2) * plane + j * px;
+                    row[0] = row[1] = row[px - 2] = row[px - 1] = 0.0;
+                    memcpy(row + 2, h->stage + ((long)(k - z0) * h->NY + j) * h->NX, sizeof(double) * h->NX);
+                }
+        }
     }
-    HCHK(hipStreamSynchronize(h->stream));
     if (h->cfg.periodic_y) {
         // image rows (never computed on the device): 0,1,NY-2,NY-1 <- NY-4,NY-3,2,3
         int NY = h->NY;
@@ -1133,6 +1171,7 @@ void mhd_destroy(mhd_handle *h)
     for (auto p : ps)
         if (p) hipFree(p);
     for (auto e : h->pool) (void)hipEventDestroy(e);
+    if (h->stage) (void)hipHostFree(h->stage);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

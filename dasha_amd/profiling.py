@@ -42,6 +42,10 @@ _FAMILIES = [
 ]
 
 
+# families whose kernels are launched together for one call of the family
+_TOGETHER = {"rhs_curlE_pressure"}
+
+
 def family(kernel_name: str) -> str | None:
     for pat, fam in _FAMILIES:
         if re.search(pat, kernel_name):
@@ -94,8 +98,13 @@ def traffic(fetch_dir: str, write_dir: str, copy_bytes: int):
         a[0] += fetch[k] * 1024.0 * f_scale
         a[1] += write[k] * 1024.0 * w_scale
         a[2] += 1
-    res = {fam: {"read_bytes_per_launch": v[0] / v[2], "write_bytes_per_launch": v[1] / v[2],
-                 "bytes_per_launch": (v[0] + v[1]) / v[2]} for fam, v in fams.items()}
+    # a family is either alternative kernels (one per launch: the mean) or
+    # kernels that run together inside one timed bracket (the sum)
+    res = {}
+    for fam, v in fams.items():
+        n = 1 if fam in _TOGETHER else v[2]
+        res[fam] = {"read_bytes_per_launch": v[0] / n, "write_bytes_per_launch": v[1] / n,
+                    "bytes_per_launch": (v[0] + v[1]) / n}
     return res, f_scale, w_scale
 
 
