@@ -128,6 +128,7 @@ __device__ inline BlockPos block_pos()
     return b;
 }
 
+
 struct Tile {
     int tx, ty, i, j;
     bool valid;   // (i, j) is a computed column of the range

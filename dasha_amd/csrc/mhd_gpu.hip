@@ -312,6 +312,12 @@ struct Prof {
     int calls;
 };
 
+// block table of a temporal-blocked launch (device words, one per block)
+struct ZTab {
+    unsigned *d = nullptr;
+    unsigned n = 0;
+};
+
 struct mhd_handle {
     mhd_config cfg;
     int NX, NY, NZ;
@@ -339,6 +345,8 @@ struct mhd_handle {
     int zs = 1;                            // pairs of smoothing passes in one temporal-blocked kernel
     int rb2 = 1;                           // windowed curl/pressure kernels for right, part 2
     int zkc = 32;                          // interior planes per block of the temporal-blocked kernels
+    int zord = 1;                          // block order of the temporal-blocked kernels (zs_tables)
+    ZTab zt[2][2];                         // their block tables: [3 or 8 fields][without / with face rules]
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -494,18 +502,97 @@ static void launch_sweep(mhd_handle *h, const double *a, double *b, const double
 
 // NP passes a -> b in one temporal-blocked kernel (one field per block);
 // with RKS >= 0 the last pass ends in RK stage RKS (b unused)
+// Block tables of the two-pass temporal-blocked kernels over nf fields, one
+// for the windows that hold no x face and no image row (the kernel without
+// face rules) and one for the rest.  The hardware hands dispatch position L
+// to XCD L % 8.  Order 0 enumerates x tile, y tile, chunk, field (x fastest)
+// by position.  Order 1 gives each XCD a contiguous range of an enumeration
+// with the field fastest, then x tile, y tile, chunk: one tile's fields read
+// the same radial coefficients together, and neighbouring tiles share halo
+// columns and rows, in one L2.
+static int zs_tables(mhd_handle *h, int nf, ZTab *out)
+{
+    const int np = 2, W = TX - 2 * np, H = ZB - 2 * np;
+    const Geo &g = h->g;
+    const unsigned gx = (unsigned)blocks(h->NX, W);
+    const unsigned gy = (unsigned)blocks(g.yhi - g.ylo + 1, H);
+    const unsigned nch = (unsigned)blocks(h->NZ - 4, h->zkc);
+    if (gx >= (1u << ZT_BY) || gy >= (1u << (ZT_CH - ZT_BY)) || nch >= (1u << (ZT_F - ZT_CH)) ||
+        nf > (1 << (32 - ZT_F))) {
+        set_err(h, "grid too large for the smoothing kernels' block table");
+        return MHD_ERR_ARG;
+    }
+    // the window of tile (bx, by) holds an x face or an image row (the same
+    // window placement as the kernel's)
+    auto faces = [&](unsigned bx, unsigned by) {
+        const int i0 = std::max(0, std::min((int)bx * W - np, h->NX - TX));
+        bool f = i0 < 2 || i0 + TX > h->NX - 2;
+        if (!g.periodic) {
+            const int j0 = std::max(0, std::min(g.ylo + (int)by * H - np, h->NY - ZB));
+            f = f || j0 < 2 || j0 + ZB > h->NY - 2;
+        }
+        return f;
+    };
+    std::vector<unsigned> list[2];
+    auto add = [&](unsigned bx, unsigned by, unsigned ch, unsigned m) {
+        list[faces(bx, by) ? 1 : 0].push_back(bx | by << ZT_BY | ch << ZT_CH | m << ZT_F);
+    };
+    if (h->zord == 0) {
+        for (unsigned m = 0; m < (unsigned)nf; m++)
+            for (unsigned ch = 0; ch < nch; ch++)
+                for (unsigned by = 0; by < gy; by++)
+                    for (unsigned bx = 0; bx < gx; bx++) add(bx, by, ch, m);
+    } else {
+        for (unsigned ch = 0; ch < nch; ch++)
+            for (unsigned by = 0; by < gy; by++)
+                for (unsigned bx = 0; bx < gx; bx++)
+                    for (unsigned m = 0; m < (unsigned)nf; m++) add(bx, by, ch, m);
+    }
+    for (int c = 0; c < 2; c++) {
+        const unsigned T = (unsigned)list[c].size();
+        out[c].n = T;
+        if (T == 0) continue;
+        std::vector<unsigned> w(T);
+        const unsigned per = T / NXCD, rem = T % NXCD;
+        for (unsigned L = 0; L < T; L++) {
+            const unsigned xcd = L % NXCD, n = L / NXCD;
+            w[L] = h->zord == 0 ? list[c][L] : list[c][xcd * per + (xcd < rem ? xcd : rem) + n];
+        }
+        HCHK(hipMalloc(&out[c].d, T * sizeof(unsigned)));
+        HCHK(hipMemcpy(out[c].d, w.data(), T * sizeof(unsigned), hipMemcpyHostToDevice));
+    }
+    return MHD_OK;
+}
+
+template <int NP, int KIND, int TYPE, int RKS, int MODE>
+static void launch_zs(mhd_handle *h, dim3 grid, const unsigned *tab, int nch, const double *a, double *b,
+                      const RKArgs &rk)
+{
+    if (h->zkc == 64)
+        hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, 64, MODE>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, tab, nch);
+    else
+        hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, 32, MODE>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, tab, nch);
+}
+
+// NP smoothing passes of NF fields a -> b.  The RK stage sweeps (two more
+// operand streams per point) run as one launch over all windows with the
+// full march; the other sweeps in two launches, the windows without face
+// rules, then the windows with them (measured faster for each).
 template <int NF, int NP, int KIND, int TYPE, int RKS>
 static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
 {
-    RKArgs rk{h->dtv, h->xfold, h->xm, h->x, h->eq};
-    const int W = TX - 2 * NP, H = ZB - 2 * NP;
-    const int nyt = h->g.yhi - h->g.ylo + 1;
-    const int nch = blocks(h->NZ - 4, h->zkc);
-    dim3 grid((unsigned)blocks(h->NX, W), (unsigned)blocks(nyt, H), (unsigned)(nch * NF));
-    if (h->zkc == 64)
-        hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, 64>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, nch);
-    else
-        hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, 32>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, nch);
+    static_assert(NF == 3 || NF == 8, "block tables exist for 3 and 8 fields");
+    const RKArgs rk{h->dtv, h->xfold, h->xm, h->x, h->eq};
+    if constexpr (RKS >= 0) {
+        const int W = TX - 2 * NP, H = ZB - 2 * NP;
+        const int nch = blocks(h->NZ - 4, h->zkc);
+        const dim3 grid((unsigned)blocks(h->NX, W), (unsigned)blocks(h->g.yhi - h->g.ylo + 1, H), (unsigned)(nch * NF));
+        launch_zs<NP, KIND, TYPE, RKS, ZM_ALL>(h, grid, nullptr, nch, a, b, rk);
+    } else {
+        const ZTab *zt = h->zt[NF == 8 ? 1 : 0];
+        if (zt[0].n) launch_zs<NP, KIND, TYPE, RKS, ZM_PLAIN>(h, dim3(zt[0].n), zt[0].d, 0, a, b, rk);
+        if (zt[1].n) launch_zs<NP, KIND, TYPE, RKS, ZM_FACE>(h, dim3(zt[1].n), zt[1].d, 0, a, b, rk);
+    }
 }
 
 // kk smoothing sweeps (+ fills) on a 3-field array; returns the result
@@ -900,6 +987,11 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     h->zkc = h->S >= (1L << 25) ? 64 : 32;
     if (const char *e = getenv("MHD_ZKC")) h->zkc = atoi(e) == 64 ? 64 : 32;
     if (const char *e = getenv("MHD_RB2")) h->rb2 = atoi(e);
+    if (const char *e = getenv("MHD_ZORD")) h->zord = std::min(2, std::max(0, atoi(e)));
+    for (int c = 0; c < 2; c++) {
+        int r = zs_tables(h, c ? 8 : 3, h->zt[c]);
+        if (r) { *out = h; return r; }
+    }
     h->pt.resize(PT_N);
     for (int t = 0; t < PT_N; t++) h->pt[t] = Prof{PT_NAMES[t], 0.0, 0};
     *out = h;
@@ -1170,6 +1262,9 @@ void mhd_destroy(mhd_handle *h)
                     h->eA, h->eB, h->hA, h->hB, h->red, h->dtv, h->crad};
     for (auto p : ps)
         if (p) hipFree(p);
+    for (auto &zc : h->zt)
+        for (auto &t : zc)
+            if (t.d) hipFree(t.d);
     for (auto e : h->pool) (void)hipEventDestroy(e);
     if (h->stage) (void)hipHostFree(h->stage);
     if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -35,6 +35,10 @@ CASES = {
     "stretched_nonperiodic_nohall": lambda: harris.case_small(24, 12, 20, periodic_y=False, di=0.0,
                                                                seed=5),
     "guide_field_visc": lambda: _visc_case(),
+    # wide enough for smoothing windows without x faces or image rows (the
+    # kernels without face rules) and for plane groups clear of the z faces
+    "wide_periodic_hall": lambda: harris.case_small(136, 8, 44, seed=13),
+    "wide_nonperiodic_hall": lambda: harris.case_small(130, 30, 40, periodic_y=False, seed=17),
 }
 
 
