@@ -310,18 +310,19 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
 // ---------------------------------------------------------------------------
 constexpr int ZB = 16;            // rows (wavefronts) per block
 // block table word: x tile | y tile << ZT_BY | chunk << ZT_CH | field << ZT_F
-constexpr int ZT_BY = 10, ZT_CH = 20, ZT_F = 29;
+// | (window holds an x face or image row) << ZT_FACE
+constexpr int ZT_BY = 10, ZT_CH = 20, ZT_F = 28, ZT_FACE = 31;
 // launch modes: every window from a gx x gy x (nch * nf) grid with the full
-// march; the windows without x faces or image rows (block table); the
-// windows with them (block table).  The last two use the fast plane groups.
-enum { ZM_ALL = 0, ZM_PLAIN = 1, ZM_FACE = 2 };
+// march; from a block table, the windows without x faces or image rows, the
+// windows with them, or both (the word's face bit picks the march).  The
+// table modes use the fast plane groups.
+enum { ZM_ALL = 0, ZM_PLAIN = 1, ZM_FACE = 2, ZM_MIXED = 3 };
 constexpr int ZPF = 3;            // planes loaded ahead
 
 template <int NP, int KIND, int TYPE, int RKS, int KC, int MODE>
 __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
                                                      RKArgs rk, const unsigned *__restrict__ ztab, int nch)
 {
-    constexpr bool FACES = MODE != ZM_PLAIN;   // the windows may hold x faces or image rows
     static_assert(NP >= 1 && NP <= 2, "the radial coefficient ring holds planes t-2..t");
     constexpr int W = TX - 2 * NP, H = ZB - 2 * NP;
     constexpr int NZW = KC + 4 * NP;           // z-weight planes of a chunk
@@ -336,6 +337,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
     // the block's tile, chunk and field: from the grid position (ZM_ALL), or
     // from the host's block table by dispatch position
     int bx, by, chunk, m;
+    bool face_block = MODE != ZM_PLAIN;
     if (MODE == ZM_ALL) {
         bx = blockIdx.x;
         by = blockIdx.y;
@@ -346,279 +348,290 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
         bx = zw & ((1u << ZT_BY) - 1);
         by = (zw >> ZT_BY) & ((1u << (ZT_CH - ZT_BY)) - 1);
         chunk = (zw >> ZT_CH) & ((1u << (ZT_F - ZT_CH)) - 1);
-        m = zw >> ZT_F;
+        m = (zw >> ZT_F) & ((1u << (ZT_FACE - ZT_F)) - 1);
+        if (MODE == ZM_MIXED) face_block = (zw >> ZT_FACE) != 0;
     }
     // columns: lane l holds column i0 + l
     const int nomx = bx * W;
     const int i0 = max(0, min(nomx - NP, g.NX - TX));
     const int i = i0 + lane;
-    // FACES false: the host gives this kernel only windows that hold no x face
-    // and no image row, where every lane is an interior point
-    const bool vx = !FACES || i < g.NX;
-    const bool ownx = vx && i >= nomx && i < nomx + W;
-    // rows: wavefront r holds logical row j0 + r (periodic rows map through rmap)
-    const int nomy = g.ylo + by * H;
-    const int j0 = g.periodic ? max(g.ylo - NP, min(nomy - NP, g.yhi + NP + 1 - ZB))
-                              : max(0, min(nomy - NP, g.NY - ZB));
-    const int j = j0 + r;
-    const bool vy = !FACES || g.periodic || j < g.NY;
-    const bool own = ownx && vy && j >= nomy && j < nomy + H && j <= g.yhi;
-    const bool ld = vx && vy;
-    const int jp = vy ? rmap(g, j) : 0;
-    const bool yin = !FACES || g.periodic || (j >= 2 && j <= g.NY - 3);
-    const bool pin = !FACES || (ld && yin && i >= 2 && i <= g.NX - 3);   // the stencil applies here
-    const bool yex = FACES && !g.periodic;         // image rows exchanged through LDS
-    const bool img = yex && vy && (j < 2 || j > g.NY - 3);
-    const int rsrc = img ? (j < 2 ? 2 - j0 : g.NY - 3 - j0) : 0;
-    const bool xft = FACES && (i0 < 2 || i0 + TX > g.NX - 2);       // the window holds an x face
-    const long S = g.S;
-    const unsigned sz = (unsigned)g.sz;
-    // every lane reads a valid address; lanes outside the range read column 0
-    const unsigned qc = ld ? (unsigned)i + (unsigned)g.sy * (unsigned)jp : 0u;
-    const double *a = A + m * S;                // wave-uniform field bases
-    const double *crad = g.crad;
-    const double wpx = pin ? g.xwp[i] : 0.0, wmx = pin ? g.xwm[i] : 0.0;
-    const double wpy = pin ? g.ywp[jp] : 0.0, wmy = pin ? g.ywm[jp] : 0.0;
-    const bool bcomp = KIND == KIND_STATE && m >= 4 && m <= 6;
-    const double cb = (KIND == KIND_STATE && pin) ? g.cbx[i] : 0.0;
-    const int kb = 2 + chunk * KC, ke = min(kb + KC, g.NZ - 2);
-    const int t0 = kb - NP, t1 = ke - 1 + NP;
-    const int kz0 = t0 - NP;                    // lowest plane a level touches
-    for (int n = threadIdx.x; n < NZW; n += TX * ZB) {
-        const int k = kz0 + n;
-        const bool kin = k >= 2 && k <= g.NZ - 3;
-        szw[0][n] = kin ? g.zwp[k] : 0.0;
-        szw[1][n] = kin ? g.zwm[k] : 0.0;
-    }
-    const double dt = RKS >= 0 ? *rk.dtp : 0.0;
-    // owned points are unique rows, where rmap(j) == j: the output offset is qc
-    double *bo = RKS < 0 ? B + m * S : nullptr;
-    const double *xf = RKS >= 0 ? rk.XF + m * S : nullptr;
-    double *xmo = RKS >= 0 ? rk.XM + m * S : nullptr;
-    double *xo = RKS >= 0 ? rk.X + m * S : nullptr;
-    // equilibrium of field m (rho, B, p; none for the momentum)
-    const double *eqm = ((TT || RKS == 4) && (m == 0 || m >= 4)) ? rk.EQ + (m == 0 ? 0 : m - 3) * S : nullptr;
-    // final value v of point e: store it (plus eq after the post-step
-    // passes, t3dsdsc.f:1157-1164), or apply the RK stage update; RKS 4 is the
-    // last stage followed by xm = x - eq (t3dsdsc.f:1131, 1138-1145)
-    // the same in two parts: put_pre reads the operands (eq, xfold, xm) that
-    // put_with combines with v
-    auto put_pre = [&](unsigned e, double &o1, double &o2) {
-        o1 = o2 = 0.0;
-        if (RKS < 0) { if (TT && eqm) o1 = fld(eqm, e); }
-        else if (RKS == 0) o1 = fld(xf, e);
-        else if (RKS <= 2) { o1 = fld(xmo, e); o2 = fld(xf, e); }
-        else if (RKS == 3) o1 = fld(xmo, e);
-        else { o1 = fld(xmo, e); if (eqm) o2 = fld(eqm, e); }
-    };
-    auto put_with = [&](unsigned e, double v, double o1, double o2) {
-        if (RKS < 0) st(fldp(bo, e), (TT && eqm) ? v + o1 : v);
-        else if (RKS == 0) { st(fldp(xmo, e), o1 + v * dt / 6.); st(fldp(xo, e), o1 + v * dt / 2.); }
-        else if (RKS == 1) { st(fldp(xmo, e), o1 + v * dt / 3.); st(fldp(xo, e), o2 + v * dt / 2.); }
-        else if (RKS == 2) { st(fldp(xmo, e), o1 + v * dt / 3.); st(fldp(xo, e), o2 + v * dt); }
-        else if (RKS == 3) st(fldp(xo, e), o1 + v * dt / 6.);
-        else { const double xn = o1 + v * dt / 6.; st(fldp(xmo, e), eqm ? xn - o2 : xn); }
-    };
-    auto put = [&](unsigned e, double v) {
-        if (RKS < 0) st(fldp(bo, e), (TT && eqm) ? v + fld(eqm, e) : v);
-        else if (RKS == 0) { const double f = fld(xf, e); st(fldp(xmo, e), f + v * dt / 6.); st(fldp(xo, e), f + v * dt / 2.); }
-        else if (RKS == 1) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt / 2.); }
-        else if (RKS == 2) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt); }
-        else if (RKS == 3) st(fldp(xo, e), fld(xmo, e) + v * dt / 6.);
-        else { const double xn = fld(xmo, e) + v * dt / 6.; st(fldp(xmo, e), eqm ? xn - fld(eqm, e) : xn); }
-    };
-    // x-left columns of a row from the level held by lanes 2, 3, 4 (every
-    // lane of the wavefront calls; the window holds the left face)
-    auto tt_left = [&](double lv, double v) {
-        const int l2 = 2 - i0;
-        const double p2 = lane_of(lv, l2), p3 = lane_of(lv, l2 + 1);
-        const double x1 = xleft1(g, p2, p3, lane_of(lv, l2 + 2));
-        return i == 1 ? x1 : i == 0 ? xleft0(g, x1, p2, p3) : v;
-    };
-
-    double q[NP + 1][3];   // level p at its three latest planes, oldest first
-    double cr[3];          // radial coefficient of planes t-2, t-1, t
-    double pf[ZPF], pc[ZPF];
-#pragma unroll
-    for (int p = 0; p <= NP; p++) q[p][0] = q[p][1] = q[p][2] = 0.0;
-    cr[0] = cr[1] = cr[2] = 0.0;
-    if (RAD && t0 >= 1) cr[2] = fld(crad, qc + sz * (unsigned)(t0 - 1));
-#pragma unroll
-    for (int d = 0; d < ZPF; d++) {
-        const unsigned e = qc + sz * (unsigned)min(t0 + d, t1);
-        pf[d] = fld(a, e);
-        pc[d] = RAD ? fld(crad, e) : 0.0;
-    }
-
-    // one plane of the march with every rule: x faces, image rows, z faces
-    auto step = [&](const int u, const int t) {
-        const double v0 = ld ? pf[u] : 0.0;
-        const double cv = pc[u];
-        {
-            const unsigned e = qc + sz * (unsigned)min(t + ZPF, t1);
-            pf[u] = fld(a, e);
-            if (RAD) pc[u] = fld(crad, e);
-        }
-        q[0][0] = q[0][1]; q[0][1] = q[0][2]; q[0][2] = v0;
-        s0[t % 3][r][lane] = v0;
-        if (RAD) { cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cv; }
-        __syncthreads();
-#pragma unroll
-        for (int p = 1; p <= NP; p++) {
-            const int k = t - p;                        // plane of level p
+    // the march, compiled for windows with face rules (FACES) and without
+    auto march = [&](auto faces_tag) {
+        constexpr bool FACES = decltype(faces_tag)::value;
+        // FACES false: the window holds no x face and no image row (the
+        // host's block table says so), every lane is an interior point
+        const bool vx = !FACES || i < g.NX;
+        const bool ownx = vx && i >= nomx && i < nomx + W;
+        // rows: wavefront r holds logical row j0 + r (periodic rows map through rmap)
+        const int nomy = g.ylo + by * H;
+        const int j0 = g.periodic ? max(g.ylo - NP, min(nomy - NP, g.yhi + NP + 1 - ZB))
+                                  : max(0, min(nomy - NP, g.NY - ZB));
+        const int j = j0 + r;
+        const bool vy = !FACES || g.periodic || j < g.NY;
+        const bool own = ownx && vy && j >= nomy && j < nomy + H && j <= g.yhi;
+        const bool ld = vx && vy;
+        const int jp = vy ? rmap(g, j) : 0;
+        const bool yin = !FACES || g.periodic || (j >= 2 && j <= g.NY - 3);
+        const bool pin = !FACES || (ld && yin && i >= 2 && i <= g.NX - 3);   // the stencil applies here
+        const bool yex = FACES && !g.periodic;         // image rows exchanged through LDS
+        const bool img = yex && vy && (j < 2 || j > g.NY - 3);
+        const int rsrc = img ? (j < 2 ? 2 - j0 : g.NY - 3 - j0) : 0;
+        const bool xft = FACES && (i0 < 2 || i0 + TX > g.NX - 2);       // the window holds an x face
+        const long S = g.S;
+        const unsigned sz = (unsigned)g.sz;
+        // every lane reads a valid address; lanes outside the range read column 0
+        const unsigned qc = ld ? (unsigned)i + (unsigned)g.sy * (unsigned)jp : 0u;
+        const double *a = A + m * S;                // wave-uniform field bases
+        const double *crad = g.crad;
+        const double wpx = pin ? g.xwp[i] : 0.0, wmx = pin ? g.xwm[i] : 0.0;
+        const double wpy = pin ? g.ywp[jp] : 0.0, wmy = pin ? g.ywm[jp] : 0.0;
+        const bool bcomp = KIND == KIND_STATE && m >= 4 && m <= 6;
+        const double cb = (KIND == KIND_STATE && pin) ? g.cbx[i] : 0.0;
+        const int kb = 2 + chunk * KC, ke = min(kb + KC, g.NZ - 2);
+        const int t0 = kb - NP, t1 = ke - 1 + NP;
+        const int kz0 = t0 - NP;                    // lowest plane a level touches
+        for (int n = threadIdx.x; n < NZW; n += TX * ZB) {
+            const int k = kz0 + n;
             const bool kin = k >= 2 && k <= g.NZ - 3;
-            const int rm = r > 0 ? r - 1 : 0, rp = r < ZB - 1 ? r + 1 : ZB - 1;
-            const double f0 = q[p - 1][1];              // level p-1 at plane k
-            double ym, yp;
-            if (p == 1) {
-                const int b = (k + 3) % 3;
-                ym = s0[b][rm][lane];
-                yp = s0[b][rp][lane];
-            } else {
-                ym = sl[p - 2][k & 1][rm][lane];
-                yp = sl[p - 2][k & 1][rp][lane];
-            }
-            const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
-            const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
-            double v = 0.0;
-            if (pin && kin) {
-                const double c = KIND == KIND_EIGHTH ? 0.125 : (bcomp ? cb : cr[2 - p]);
-                const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
-                                 difw(q[p - 1][0], f0, q[p - 1][2], szw[0][k - kz0], szw[1][k - kz0]);
-                v = f0 + c * s;
-            }
-            if (xft) v = row_xrule_rl<XT>(g, i, i0, vx && yin, v);
-            if (yex && kin) {
-                // image rows: 0 on the outer row, half of row 2 (NY-3) on the next
-                sl[p - 1][k & 1][r][lane] = v;
-                __syncthreads();
-                if (img) {
-                    v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sl[p - 1][k & 1][rsrc][lane];
-                    sl[p - 1][k & 1][r][lane] = v;
-                }
-            } else if (p < NP) {
-                sl[p - 1][k & 1][r][lane] = v;
-            }
-            // z faces above the interior: plane NZ-2 is half of NZ-3, NZ-1 is 0
-            if (k == g.NZ - 2) v = 0.5 * q[p][2];
-            else if (k == g.NZ - 1) v = 0.0;
-            // bndrytt skips x < x_first in its y and z rules: on image rows
-            // and z-face planes the x-left columns extrapolate this pass's
-            // input of that row and plane (t3dsdsc.f:2108-2168)
-            if (TT && FACES && i0 < 2 && (img || !kin)) v = tt_left(f0, v);
-            q[p][0] = q[p][1]; q[p][1] = q[p][2]; q[p][2] = v;
-            // below: planes 1 and 0 follow from plane 2 (bndrytt: the x-left
-            // columns of plane 1 from the input's plane 1; plane 0 stays 0
-            // as the input is 0 there from the second pass on)
-            if (k == 2 && p < NP) {
-                q[p][1] = 0.5 * v;
-                q[p][0] = 0.0;
-                if (TT && FACES && i0 < 2) q[p][1] = tt_left(q[p - 1][0], q[p][1]);
-            }
-            if (p == NP) {
-                // the last level's z-face planes: 1 (NZ-2) is half of plane
-                // 2 (NZ-3), 0 (NZ-1) is 0; under bndrytt the x-left columns
-                // of planes 1 and NZ-2 extrapolate the previous level there
-                const bool lo = k == 2 && kb == 2, hi = k == g.NZ - 3 && ke == g.NZ - 2;
-                double vf = 0.5 * v;
-                if (TT && FACES && i0 < 2 && (lo || hi)) vf = tt_left(lo ? q[p - 1][0] : q[p - 1][2], vf);
-                if (own) {
-                    if (k >= kb && k < ke) put(qc + sz * (unsigned)k, v);
-                    if (lo || hi) {
-                        put(qc + sz * (unsigned)(lo ? 1 : g.NZ - 2), vf);
-                        put(qc + sz * (unsigned)(lo ? 0 : g.NZ - 1), 0.0);
-                    }
-                }
-            }
+            szw[0][n] = kin ? g.zwp[k] : 0.0;
+            szw[1][n] = kin ? g.zwm[k] : 0.0;
         }
-    };
+        const double dt = RKS >= 0 ? *rk.dtp : 0.0;
+        // owned points are unique rows, where rmap(j) == j: the output offset is qc
+        double *bo = RKS < 0 ? B + m * S : nullptr;
+        const double *xf = RKS >= 0 ? rk.XF + m * S : nullptr;
+        double *xmo = RKS >= 0 ? rk.XM + m * S : nullptr;
+        double *xo = RKS >= 0 ? rk.X + m * S : nullptr;
+        // equilibrium of field m (rho, B, p; none for the momentum)
+        const double *eqm = ((TT || RKS == 4) && (m == 0 || m >= 4)) ? rk.EQ + (m == 0 ? 0 : m - 3) * S : nullptr;
+        // final value v of point e: store it (plus eq after the post-step
+        // passes, t3dsdsc.f:1157-1164), or apply the RK stage update; RKS 4 is the
+        // last stage followed by xm = x - eq (t3dsdsc.f:1131, 1138-1145)
+        // the same in two parts: put_pre reads the operands (eq, xfold, xm) that
+        // put_with combines with v
+        auto put_pre = [&](unsigned e, double &o1, double &o2) {
+            o1 = o2 = 0.0;
+            if (RKS < 0) { if (TT && eqm) o1 = fld(eqm, e); }
+            else if (RKS == 0) o1 = fld(xf, e);
+            else if (RKS <= 2) { o1 = fld(xmo, e); o2 = fld(xf, e); }
+            else if (RKS == 3) o1 = fld(xmo, e);
+            else { o1 = fld(xmo, e); if (eqm) o2 = fld(eqm, e); }
+        };
+        auto put_with = [&](unsigned e, double v, double o1, double o2) {
+            if (RKS < 0) st(fldp(bo, e), (TT && eqm) ? v + o1 : v);
+            else if (RKS == 0) { st(fldp(xmo, e), o1 + v * dt / 6.); st(fldp(xo, e), o1 + v * dt / 2.); }
+            else if (RKS == 1) { st(fldp(xmo, e), o1 + v * dt / 3.); st(fldp(xo, e), o2 + v * dt / 2.); }
+            else if (RKS == 2) { st(fldp(xmo, e), o1 + v * dt / 3.); st(fldp(xo, e), o2 + v * dt); }
+            else if (RKS == 3) st(fldp(xo, e), o1 + v * dt / 6.);
+            else { const double xn = o1 + v * dt / 6.; st(fldp(xmo, e), eqm ? xn - o2 : xn); }
+        };
+        auto put = [&](unsigned e, double v) {
+            if (RKS < 0) st(fldp(bo, e), (TT && eqm) ? v + fld(eqm, e) : v);
+            else if (RKS == 0) { const double f = fld(xf, e); st(fldp(xmo, e), f + v * dt / 6.); st(fldp(xo, e), f + v * dt / 2.); }
+            else if (RKS == 1) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt / 2.); }
+            else if (RKS == 2) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt); }
+            else if (RKS == 3) st(fldp(xo, e), fld(xmo, e) + v * dt / 6.);
+            else { const double xn = fld(xmo, e) + v * dt / 6.; st(fldp(xmo, e), eqm ? xn - fld(eqm, e) : xn); }
+        };
+        // x-left columns of a row from the level held by lanes 2, 3, 4 (every
+        // lane of the wavefront calls; the window holds the left face)
+        auto tt_left = [&](double lv, double v) {
+            const int l2 = 2 - i0;
+            const double p2 = lane_of(lv, l2), p3 = lane_of(lv, l2 + 1);
+            const double x1 = xleft1(g, p2, p3, lane_of(lv, l2 + 2));
+            return i == 1 ? x1 : i == 0 ? xleft0(g, x1, p2, p3) : v;
+        };
 
-    // plane u of a group of ZPF planes none of whose levels touches a z-face
-    // rule (3 <= t-p <= NZ-4): the stencil with the x-face and image-row
-    // rules of the window (none without FACES).  The queues are rings here:
-    // after the group's u-th push the oldest plane of a level sits in slot
-    // (u+1) % 3, the newest in u, so a whole group leaves them in the order
-    // the full step keeps.
-    auto fast_step = [&](const int u, const int t) {
-        // the output plane's operands, read before the passes (every lane
-        // reads an address inside the volume: plane t-NP >= 3) and before
-        // the prefetch, so that waiting for them leaves the prefetch in flight
-        const unsigned eo = qc + sz * (unsigned)(t - NP);
-        double o1, o2;
-        put_pre(eo, o1, o2);
-        const double v0 = ld ? pf[u] : 0.0;
-        const double cv = pc[u];
-        {
-            const unsigned e = qc + sz * (unsigned)min(t + ZPF, t1);
-            pf[u] = fld(a, e);
-            if (RAD) pc[u] = fld(crad, e);
+        double q[NP + 1][3];   // level p at its three latest planes, oldest first
+        double cr[3];          // radial coefficient of planes t-2, t-1, t
+        double pf[ZPF], pc[ZPF];
+    #pragma unroll
+        for (int p = 0; p <= NP; p++) q[p][0] = q[p][1] = q[p][2] = 0.0;
+        cr[0] = cr[1] = cr[2] = 0.0;
+        if (RAD && t0 >= 1) cr[2] = fld(crad, qc + sz * (unsigned)(t0 - 1));
+    #pragma unroll
+        for (int d = 0; d < ZPF; d++) {
+            const unsigned e = qc + sz * (unsigned)min(t0 + d, t1);
+            pf[d] = fld(a, e);
+            pc[d] = RAD ? fld(crad, e) : 0.0;
         }
-        q[0][u] = v0;
-        s0[t % 3][r][lane] = v0;
-        if (RAD) cr[u] = cv;
-        __syncthreads();
-#pragma unroll
-        for (int p = 1; p <= NP; p++) {
-            const int k = t - p;
-            const int rm = r > 0 ? r - 1 : 0, rp = r < ZB - 1 ? r + 1 : ZB - 1;
-            const double zm = q[p - 1][(u + 1) % 3], f0 = q[p - 1][(u + 2) % 3], zp = q[p - 1][u];
-            double ym, yp;
-            if (p == 1) {
-                const int b = (k + 3) % 3;
-                ym = s0[b][rm][lane];
-                yp = s0[b][rp][lane];
-            } else {
-                ym = sl[p - 2][k & 1][rm][lane];
-                yp = sl[p - 2][k & 1][rp][lane];
+
+        // one plane of the march with every rule: x faces, image rows, z faces
+        auto step = [&](const int u, const int t) {
+            const double v0 = ld ? pf[u] : 0.0;
+            const double cv = pc[u];
+            {
+                const unsigned e = qc + sz * (unsigned)min(t + ZPF, t1);
+                pf[u] = fld(a, e);
+                if (RAD) pc[u] = fld(crad, e);
             }
-            const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
-            const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
-            const double c = KIND == KIND_EIGHTH ? 0.125 : (bcomp ? cb : cr[(u + 3 - p) % 3]);
-            const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
-                             difw(zm, f0, zp, szw[0][k - kz0], szw[1][k - kz0]);
-            double v = pin ? f0 + c * s : 0.0;
-            if (FACES) {
+            q[0][0] = q[0][1]; q[0][1] = q[0][2]; q[0][2] = v0;
+            s0[t % 3][r][lane] = v0;
+            if (RAD) { cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cv; }
+            __syncthreads();
+    #pragma unroll
+            for (int p = 1; p <= NP; p++) {
+                const int k = t - p;                        // plane of level p
+                const bool kin = k >= 2 && k <= g.NZ - 3;
+                const int rm = r > 0 ? r - 1 : 0, rp = r < ZB - 1 ? r + 1 : ZB - 1;
+                const double f0 = q[p - 1][1];              // level p-1 at plane k
+                double ym, yp;
+                if (p == 1) {
+                    const int b = (k + 3) % 3;
+                    ym = s0[b][rm][lane];
+                    yp = s0[b][rp][lane];
+                } else {
+                    ym = sl[p - 2][k & 1][rm][lane];
+                    yp = sl[p - 2][k & 1][rp][lane];
+                }
+                const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
+                const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
+                double v = 0.0;
+                if (pin && kin) {
+                    const double c = KIND == KIND_EIGHTH ? 0.125 : (bcomp ? cb : cr[2 - p]);
+                    const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
+                                     difw(q[p - 1][0], f0, q[p - 1][2], szw[0][k - kz0], szw[1][k - kz0]);
+                    v = f0 + c * s;
+                }
                 if (xft) v = row_xrule_rl<XT>(g, i, i0, vx && yin, v);
-                if (yex) {
+                if (yex && kin) {
+                    // image rows: 0 on the outer row, half of row 2 (NY-3) on the next
                     sl[p - 1][k & 1][r][lane] = v;
                     __syncthreads();
                     if (img) {
                         v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sl[p - 1][k & 1][rsrc][lane];
                         sl[p - 1][k & 1][r][lane] = v;
                     }
-                    if (TT && i0 < 2 && img) v = tt_left(f0, v);
                 } else if (p < NP) {
                     sl[p - 1][k & 1][r][lane] = v;
                 }
-            } else if (p < NP) {
-                sl[p - 1][k & 1][r][lane] = v;
+                // z faces above the interior: plane NZ-2 is half of NZ-3, NZ-1 is 0
+                if (k == g.NZ - 2) v = 0.5 * q[p][2];
+                else if (k == g.NZ - 1) v = 0.0;
+                // bndrytt skips x < x_first in its y and z rules: on image rows
+                // and z-face planes the x-left columns extrapolate this pass's
+                // input of that row and plane (t3dsdsc.f:2108-2168)
+                if (TT && FACES && i0 < 2 && (img || !kin)) v = tt_left(f0, v);
+                q[p][0] = q[p][1]; q[p][1] = q[p][2]; q[p][2] = v;
+                // below: planes 1 and 0 follow from plane 2 (bndrytt: the x-left
+                // columns of plane 1 from the input's plane 1; plane 0 stays 0
+                // as the input is 0 there from the second pass on)
+                if (k == 2 && p < NP) {
+                    q[p][1] = 0.5 * v;
+                    q[p][0] = 0.0;
+                    if (TT && FACES && i0 < 2) q[p][1] = tt_left(q[p - 1][0], q[p][1]);
+                }
+                if (p == NP) {
+                    // the last level's z-face planes: 1 (NZ-2) is half of plane
+                    // 2 (NZ-3), 0 (NZ-1) is 0; under bndrytt the x-left columns
+                    // of planes 1 and NZ-2 extrapolate the previous level there
+                    const bool lo = k == 2 && kb == 2, hi = k == g.NZ - 3 && ke == g.NZ - 2;
+                    double vf = 0.5 * v;
+                    if (TT && FACES && i0 < 2 && (lo || hi)) vf = tt_left(lo ? q[p - 1][0] : q[p - 1][2], vf);
+                    if (own) {
+                        if (k >= kb && k < ke) put(qc + sz * (unsigned)k, v);
+                        if (lo || hi) {
+                            put(qc + sz * (unsigned)(lo ? 1 : g.NZ - 2), vf);
+                            put(qc + sz * (unsigned)(lo ? 0 : g.NZ - 1), 0.0);
+                        }
+                    }
+                }
             }
-            q[p][u] = v;
-            if (p == NP && own && k >= kb && k < ke) put_with(eo, v, o1, o2);
+        };
+
+        // plane u of a group of ZPF planes none of whose levels touches a z-face
+        // rule (3 <= t-p <= NZ-4): the stencil with the x-face and image-row
+        // rules of the window (none without FACES).  The queues are rings here:
+        // after the group's u-th push the oldest plane of a level sits in slot
+        // (u+1) % 3, the newest in u, so a whole group leaves them in the order
+        // the full step keeps.
+        auto fast_step = [&](const int u, const int t) {
+            // the output plane's operands, read before the passes (every lane
+            // reads an address inside the volume: plane t-NP >= 3) and before
+            // the prefetch, so that waiting for them leaves the prefetch in flight
+            const unsigned eo = qc + sz * (unsigned)(t - NP);
+            double o1, o2;
+            put_pre(eo, o1, o2);
+            const double v0 = ld ? pf[u] : 0.0;
+            const double cv = pc[u];
+            {
+                const unsigned e = qc + sz * (unsigned)min(t + ZPF, t1);
+                pf[u] = fld(a, e);
+                if (RAD) pc[u] = fld(crad, e);
+            }
+            q[0][u] = v0;
+            s0[t % 3][r][lane] = v0;
+            if (RAD) cr[u] = cv;
+            __syncthreads();
+    #pragma unroll
+            for (int p = 1; p <= NP; p++) {
+                const int k = t - p;
+                const int rm = r > 0 ? r - 1 : 0, rp = r < ZB - 1 ? r + 1 : ZB - 1;
+                const double zm = q[p - 1][(u + 1) % 3], f0 = q[p - 1][(u + 2) % 3], zp = q[p - 1][u];
+                double ym, yp;
+                if (p == 1) {
+                    const int b = (k + 3) % 3;
+                    ym = s0[b][rm][lane];
+                    yp = s0[b][rp][lane];
+                } else {
+                    ym = sl[p - 2][k & 1][rm][lane];
+                    yp = sl[p - 2][k & 1][rp][lane];
+                }
+                const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
+                const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
+                const double c = KIND == KIND_EIGHTH ? 0.125 : (bcomp ? cb : cr[(u + 3 - p) % 3]);
+                const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
+                                 difw(zm, f0, zp, szw[0][k - kz0], szw[1][k - kz0]);
+                double v = pin ? f0 + c * s : 0.0;
+                if (FACES) {
+                    if (xft) v = row_xrule_rl<XT>(g, i, i0, vx && yin, v);
+                    if (yex) {
+                        sl[p - 1][k & 1][r][lane] = v;
+                        __syncthreads();
+                        if (img) {
+                            v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sl[p - 1][k & 1][rsrc][lane];
+                            sl[p - 1][k & 1][r][lane] = v;
+                        }
+                        if (TT && i0 < 2 && img) v = tt_left(f0, v);
+                    } else if (p < NP) {
+                        sl[p - 1][k & 1][r][lane] = v;
+                    }
+                } else if (p < NP) {
+                    sl[p - 1][k & 1][r][lane] = v;
+                }
+                q[p][u] = v;
+                if (p == NP && own && k >= kb && k < ke) put_with(eo, v, o1, o2);
+            }
+        };
+
+        // the march runs in groups of ZPF planes so that every prefetch slot is a
+        // fixed register (the loads stay in flight); planes past t1 are dummies
+        // that load plane t1 again and store nothing
+        static_assert(ZPF == 3, "the fast step's rings rotate once per group of 3 planes");
+        const int tend = t0 + ((t1 - t0 + ZPF) / ZPF) * ZPF;
+        // the plane groups that may use the fast step form one run
+        auto fast = [&](int b) { return b >= NP + 3 && b + ZPF - 1 <= g.NZ - 3; };
+        int tb = t0;
+        if (MODE != ZM_ALL) {
+        for (; tb < tend && !fast(tb); tb += ZPF) {
+    #pragma unroll
+            for (int u = 0; u < ZPF; u++) step(u, tb + u);
+        }
+        for (; tb < tend && fast(tb); tb += ZPF) {
+    #pragma unroll
+            for (int u = 0; u < ZPF; u++) fast_step(u, tb + u);
+        }
+        }
+        for (; tb < tend; tb += ZPF) {
+    #pragma unroll
+            for (int u = 0; u < ZPF; u++) step(u, tb + u);
         }
     };
-
-    // the march runs in groups of ZPF planes so that every prefetch slot is a
-    // fixed register (the loads stay in flight); planes past t1 are dummies
-    // that load plane t1 again and store nothing
-    static_assert(ZPF == 3, "the fast step's rings rotate once per group of 3 planes");
-    const int tend = t0 + ((t1 - t0 + ZPF) / ZPF) * ZPF;
-    // the plane groups that may use the fast step form one run
-    auto fast = [&](int b) { return b >= NP + 3 && b + ZPF - 1 <= g.NZ - 3; };
-    int tb = t0;
-    if (MODE != ZM_ALL) {
-    for (; tb < tend && !fast(tb); tb += ZPF) {
-#pragma unroll
-        for (int u = 0; u < ZPF; u++) step(u, tb + u);
-    }
-    for (; tb < tend && fast(tb); tb += ZPF) {
-#pragma unroll
-        for (int u = 0; u < ZPF; u++) fast_step(u, tb + u);
-    }
-    }
-    for (; tb < tend; tb += ZPF) {
-#pragma unroll
-        for (int u = 0; u < ZPF; u++) step(u, tb + u);
+    if constexpr (MODE == ZM_MIXED) {
+        if (face_block) march(std::true_type{});
+        else march(std::false_type{});
+    } else {
+        march(std::integral_constant<bool, MODE != ZM_PLAIN>{});
     }
 }
 
