@@ -346,8 +346,7 @@ struct mhd_handle {
     int rb2 = 1;                           // windowed curl/pressure kernels for right, part 2
     int zkc = 32;                          // interior planes per block of the temporal-blocked kernels
     int zord = 1;                          // block order of the temporal-blocked kernels (zs_tables)
-    ZTab zt[2][3];                         // their block tables: [3 or 8 fields][without / with face rules / all]
-    int zsplit = 0;                        // launch the windows without and with face rules separately
+    ZTab zt[2][2];                         // their block tables: [3 or 8 fields][without / with face rules]
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -505,7 +504,7 @@ static void launch_sweep(mhd_handle *h, const double *a, double *b, const double
 // with RKS >= 0 the last pass ends in RK stage RKS (b unused)
 // Block tables of the two-pass temporal-blocked kernels over nf fields: the
 // windows that hold no x face and no image row (the march without face
-// rules), the rest, and all windows with a face bit in each word.  The hardware hands dispatch position L
+// rules) and the rest.  The hardware hands dispatch position L
 // to XCD L % 8.  Order 0 enumerates x tile, y tile, chunk, field (x fastest)
 // by position.  Order 1 gives each XCD a contiguous range of an enumeration
 // with the field fastest, then x tile, y tile, chunk: one tile's fields read
@@ -519,7 +518,7 @@ static int zs_tables(mhd_handle *h, int nf, ZTab *out)
     const unsigned gy = (unsigned)blocks(g.yhi - g.ylo + 1, H);
     const unsigned nch = (unsigned)blocks(h->NZ - 4, h->zkc);
     if (gx >= (1u << ZT_BY) || gy >= (1u << (ZT_CH - ZT_BY)) || nch >= (1u << (ZT_F - ZT_CH)) ||
-        nf > (1 << (ZT_FACE - ZT_F))) {
+        nf > (1 << (32 - ZT_F))) {
         set_err(h, "grid too large for the smoothing kernels' block table");
         return MHD_ERR_ARG;
     }
@@ -534,12 +533,9 @@ static int zs_tables(mhd_handle *h, int nf, ZTab *out)
         }
         return f;
     };
-    std::vector<unsigned> list[3];
+    std::vector<unsigned> list[2];
     auto add = [&](unsigned bx, unsigned by, unsigned ch, unsigned m) {
-        const unsigned f = faces(bx, by) ? 1u : 0u;
-        const unsigned w = bx | by << ZT_BY | ch << ZT_CH | m << ZT_F;
-        list[f].push_back(w);
-        list[2].push_back(w | f << ZT_FACE);
+        list[faces(bx, by) ? 1 : 0].push_back(bx | by << ZT_BY | ch << ZT_CH | m << ZT_F);
     };
     if (h->zord == 0) {
         for (unsigned m = 0; m < (unsigned)nf; m++)
@@ -552,7 +548,7 @@ static int zs_tables(mhd_handle *h, int nf, ZTab *out)
                 for (unsigned bx = 0; bx < gx; bx++)
                     for (unsigned m = 0; m < (unsigned)nf; m++) add(bx, by, ch, m);
     }
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < 2; c++) {
         const unsigned T = (unsigned)list[c].size();
         out[c].n = T;
         if (T == 0) continue;
@@ -594,12 +590,8 @@ static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
         launch_zs<NP, KIND, TYPE, RKS, ZM_ALL>(h, grid, nullptr, nch, a, b, rk);
     } else {
         const ZTab *zt = h->zt[NF == 8 ? 1 : 0];
-        if (h->zsplit || TYPE == BC_TT) {
-            if (zt[0].n) launch_zs<NP, KIND, TYPE, RKS, ZM_PLAIN>(h, dim3(zt[0].n), zt[0].d, 0, a, b, rk);
-            if (zt[1].n) launch_zs<NP, KIND, TYPE, RKS, ZM_FACE>(h, dim3(zt[1].n), zt[1].d, 0, a, b, rk);
-        } else {
-            launch_zs<NP, KIND, TYPE, RKS, ZM_MIXED>(h, dim3(zt[2].n), zt[2].d, 0, a, b, rk);
-        }
+        if (zt[0].n) launch_zs<NP, KIND, TYPE, RKS, ZM_PLAIN>(h, dim3(zt[0].n), zt[0].d, 0, a, b, rk);
+        if (zt[1].n) launch_zs<NP, KIND, TYPE, RKS, ZM_FACE>(h, dim3(zt[1].n), zt[1].d, 0, a, b, rk);
     }
 }
 
@@ -996,7 +988,6 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     if (const char *e = getenv("MHD_ZKC")) h->zkc = atoi(e) == 64 ? 64 : 32;
     if (const char *e = getenv("MHD_RB2")) h->rb2 = atoi(e);
     if (const char *e = getenv("MHD_ZORD")) h->zord = std::min(1, std::max(0, atoi(e)));
-    if (const char *e = getenv("MHD_ZSPLIT")) h->zsplit = atoi(e);
     for (int c = 0; c < 2; c++) {
         int r = zs_tables(h, c ? 8 : 3, h->zt[c]);
         if (r) { *out = h; return r; }

@@ -310,17 +310,17 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
 // ---------------------------------------------------------------------------
 constexpr int ZB = 16;            // rows (wavefronts) per block
 // block table word: x tile | y tile << ZT_BY | chunk << ZT_CH | field << ZT_F
-// | (window holds an x face or image row) << ZT_FACE
-constexpr int ZT_BY = 10, ZT_CH = 20, ZT_F = 28, ZT_FACE = 31;
+constexpr int ZT_BY = 10, ZT_CH = 20, ZT_F = 29;
 // launch modes: every window from a gx x gy x (nch * nf) grid with the full
-// march; from a block table, the windows without x faces or image rows, the
-// windows with them, or both (the word's face bit picks the march).  The
-// table modes use the fast plane groups.
-enum { ZM_ALL = 0, ZM_PLAIN = 1, ZM_FACE = 2, ZM_MIXED = 3 };
+// march; from a block table, the windows without x faces or image rows, or
+// the windows with them.  The table modes use the fast plane groups.  The
+// face windows' march needs more than 64 registers: that launch runs at 4
+// waves per SIMD (one block per CU) rather than spill.
+enum { ZM_ALL = 0, ZM_PLAIN = 1, ZM_FACE = 2 };
 constexpr int ZPF = 3;            // planes loaded ahead
 
 template <int NP, int KIND, int TYPE, int RKS, int KC, int MODE>
-__global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
+__global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE == ZM_FACE ? 4 : 8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
                                                      RKArgs rk, const unsigned *__restrict__ ztab, int nch)
 {
     static_assert(NP >= 1 && NP <= 2, "the radial coefficient ring holds planes t-2..t");
@@ -337,7 +337,6 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
     // the block's tile, chunk and field: from the grid position (ZM_ALL), or
     // from the host's block table by dispatch position
     int bx, by, chunk, m;
-    bool face_block = MODE != ZM_PLAIN;
     if (MODE == ZM_ALL) {
         bx = blockIdx.x;
         by = blockIdx.y;
@@ -348,8 +347,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
         bx = zw & ((1u << ZT_BY) - 1);
         by = (zw >> ZT_BY) & ((1u << (ZT_CH - ZT_BY)) - 1);
         chunk = (zw >> ZT_CH) & ((1u << (ZT_F - ZT_CH)) - 1);
-        m = (zw >> ZT_F) & ((1u << (ZT_FACE - ZT_F)) - 1);
-        if (MODE == ZM_MIXED) face_block = (zw >> ZT_FACE) != 0;
+        m = zw >> ZT_F;
     }
     // columns: lane l holds column i0 + l
     const int nomx = bx * W;
@@ -627,12 +625,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(8, 8)))
             for (int u = 0; u < ZPF; u++) step(u, tb + u);
         }
     };
-    if constexpr (MODE == ZM_MIXED) {
-        if (face_block) march(std::true_type{});
-        else march(std::false_type{});
-    } else {
-        march(std::integral_constant<bool, MODE != ZM_PLAIN>{});
-    }
+    march(std::integral_constant<bool, MODE != ZM_PLAIN>{});
 }
 
 // ---------------------------------------------------------------------------
