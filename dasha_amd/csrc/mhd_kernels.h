@@ -22,7 +22,7 @@ __device__ inline void zface_store(const Geo &g, double *__restrict__ B, long q,
 {
     if (k != 2 && k != g.NZ - 3) return;
     const long qh = q + g.sz * (k == 2 ? 1 : g.NZ - 2), q0 = q + g.sz * (k == 2 ? 0 : g.NZ - 1);
-#pragma unroll
+    #pragma unroll
     for (int m = 0; m < NF; m++) {
         st(B + qh + m * g.S, 0.5 * v[m]);
         st(B + q0 + m * g.S, 0.0);
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X
     const double *b5 = X + 4 * S, *b6 = X + 5 * S, *b7 = X + 6 * S;
     double uq[5], vq[5];
     if (ld) {
-#pragma unroll
+        #pragma unroll
         for (int s = 0; s < 4; s++) {
             long q = q0 + sz * (t.kb - 2 + s);
             uq[s] = b5[q] - bxi[q];
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X
             zface_store<3>(g, cur, q0, k, v);
         }
         __syncthreads();
-#pragma unroll
+        #pragma unroll
         for (int s = 0; s < 4; s++) { uq[s] = uq[s + 1]; vq[s] = vq[s + 1]; }
     }
 }
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
     double cr[UZ], wpz[UZ], wmz[UZ];
     const bool needr = KIND == KIND_RADIAL || (KIND == KIND_STATE && !(FPB == 1 && mb >= 4 && mb <= 6));
     const double xy2 = einner ? g.xc[i] * g.xc[i] + g.yc[je] * g.yc[je] : 0.0;
-#pragma unroll
+    #pragma unroll
     for (int u = 0; u < UZ; u++) {
         int k = k0 + u;
         bool kin = k >= 2 && k <= g.NZ - 3;
@@ -226,18 +226,18 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
     const double wpy = einner ? g.ywp[je] : 0, wmy = einner ? g.ywm[je] : 0;
     const double cb = (KIND == KIND_STATE && einner) ? g.cbx[i] : 0.0;
     const double dt = RKS >= 0 ? *rk.dtp : 0.0;
-#pragma unroll
+    #pragma unroll
     for (int mm = 0; mm < FPB; mm++) {
         const int m = mb + mm;
         if (m >= NF) break;
         const double *a = A + m * S;
         double col[UZ + 2], xm[UZ], xp[UZ], ym[UZ], yp[UZ];
-#pragma unroll
+        #pragma unroll
         for (int u = 0; u < UZ + 2; u++) {
             int k = k0 - 1 + u;
             col[u] = (ld && k >= 0 && k < g.NZ) ? a[qc + sz * k] : 0.0;
         }
-#pragma unroll
+        #pragma unroll
         for (int u = 0; u < UZ; u++) {
             int k = k0 + u;
             bool kin = ld && einner && k >= 2 && k <= g.NZ - 3;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
         }
         const bool bcoef = KIND == KIND_STATE && m >= 4 && m <= 6;
         double v[UZ];
-#pragma unroll
+        #pragma unroll
         for (int u = 0; u < UZ; u++) {
             int k = k0 + u;
             v[u] = 0.0;
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
             if (image) v[u] = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * v[u];
         }
         // z-face planes from plane 2 / NZ-3 of the same group
-#pragma unroll
+        #pragma unroll
         for (int u = 0; u < UZ; u++) {
             int k = k0 + u;
             if (k == 0 || k == g.NZ - 1) v[u] = 0.0;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
         }
         if (own) {
             const long qb = m * S + (long)i + sy * j;
-#pragma unroll
+            #pragma unroll
             for (int u = 0; u < UZ; u++) {
                 if (k0 + u < nomz) continue;
                 const long qq = qb + sz * (k0 + u);
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
     static_assert(NP >= 1 && NP <= 2, "the radial coefficient ring holds planes t-2..t");
     constexpr int W = TX - 2 * NP, H = ZB - 2 * NP;
     constexpr int NZW = KC + 4 * NP;           // z-weight planes of a chunk
-    constexpr bool RAD = KIND != KIND_EIGHTH;   // reads the radial coefficient table
+    constexpr bool RAD = KIND != KIND_EIGHTH;   // reads a smoothing coefficient stream
     constexpr bool TT = TYPE == BC_TT;          // post-step passes on x - eq (bndrytt fills)
     constexpr int XT = TT ? BC_AUX : TYPE;      // x-face rule of the pass
     __shared__ double s0[3][ZB][TX];           // level 0 by plane mod 3
@@ -384,7 +384,12 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         const double wpx = pin ? g.xwp[i] : 0.0, wmx = pin ? g.xwm[i] : 0.0;
         const double wpy = pin ? g.ywp[jp] : 0.0, wmy = pin ? g.ywm[jp] : 0.0;
         const bool bcomp = KIND == KIND_STATE && m >= 4 && m <= 6;
-        const double cb = (KIND == KIND_STATE && pin) ? g.cbx[i] : 0.0;
+        // smoothing coefficient stream of the field: the radial table, or
+        // for the magnetic field of the state the x-only cbx (plane stride 0,
+        // so those loads hit the cache)
+        const double *cbase = bcomp ? g.cbx : crad;
+        const unsigned cq0 = bcomp ? (unsigned)i : qc, csz = bcomp ? 0u : sz;
+
         const int kb = 2 + chunk * KC, ke = min(kb + KC, g.NZ - 2);
         const int t0 = kb - NP, t1 = ke - 1 + NP;
         const int kz0 = t0 - NP;                    // lowest plane a level touches
@@ -443,15 +448,15 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         double q[NP + 1][3];   // level p at its three latest planes, oldest first
         double cr[3];          // radial coefficient of planes t-2, t-1, t
         double pf[ZPF], pc[ZPF];
-    #pragma unroll
+        #pragma unroll
         for (int p = 0; p <= NP; p++) q[p][0] = q[p][1] = q[p][2] = 0.0;
         cr[0] = cr[1] = cr[2] = 0.0;
-        if (RAD && t0 >= 1) cr[2] = fld(crad, qc + sz * (unsigned)(t0 - 1));
-    #pragma unroll
+        if (RAD && t0 >= 1) cr[2] = fld(cbase, cq0 + csz * (unsigned)(t0 - 1));
+        #pragma unroll
         for (int d = 0; d < ZPF; d++) {
             const unsigned e = qc + sz * (unsigned)min(t0 + d, t1);
             pf[d] = fld(a, e);
-            pc[d] = RAD ? fld(crad, e) : 0.0;
+            pc[d] = RAD ? fld(cbase, cq0 + csz * (unsigned)min(t0 + d, t1)) : 0.0;
         }
 
         // one plane of the march with every rule: x faces, image rows, z faces
@@ -459,15 +464,15 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
             const double v0 = ld ? pf[u] : 0.0;
             const double cv = pc[u];
             {
-                const unsigned e = qc + sz * (unsigned)min(t + ZPF, t1);
-                pf[u] = fld(a, e);
-                if (RAD) pc[u] = fld(crad, e);
+                const unsigned tz = (unsigned)min(t + ZPF, t1);
+                pf[u] = fld(a, qc + sz * tz);
+                if (RAD) pc[u] = fld(cbase, cq0 + csz * tz);
             }
             q[0][0] = q[0][1]; q[0][1] = q[0][2]; q[0][2] = v0;
             s0[t % 3][r][lane] = v0;
             if (RAD) { cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cv; }
             __syncthreads();
-    #pragma unroll
+            #pragma unroll
             for (int p = 1; p <= NP; p++) {
                 const int k = t - p;                        // plane of level p
                 const bool kin = k >= 2 && k <= g.NZ - 3;
@@ -486,7 +491,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
                 const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
                 double v = 0.0;
                 if (pin && kin) {
-                    const double c = KIND == KIND_EIGHTH ? 0.125 : (bcomp ? cb : cr[2 - p]);
+                    const double c = KIND == KIND_EIGHTH ? 0.125 : cr[2 - p];
                     const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
                                      difw(q[p - 1][0], f0, q[p - 1][2], szw[0][k - kz0], szw[1][k - kz0]);
                     v = f0 + c * s;
@@ -553,15 +558,15 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
             const double v0 = ld ? pf[u] : 0.0;
             const double cv = pc[u];
             {
-                const unsigned e = qc + sz * (unsigned)min(t + ZPF, t1);
-                pf[u] = fld(a, e);
-                if (RAD) pc[u] = fld(crad, e);
+                const unsigned tz = (unsigned)min(t + ZPF, t1);
+                pf[u] = fld(a, qc + sz * tz);
+                if (RAD) pc[u] = fld(cbase, cq0 + csz * tz);
             }
             q[0][u] = v0;
             s0[t % 3][r][lane] = v0;
             if (RAD) cr[u] = cv;
             __syncthreads();
-    #pragma unroll
+            #pragma unroll
             for (int p = 1; p <= NP; p++) {
                 const int k = t - p;
                 const int rm = r > 0 ? r - 1 : 0, rp = r < ZB - 1 ? r + 1 : ZB - 1;
@@ -577,7 +582,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
                 }
                 const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
                 const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
-                const double c = KIND == KIND_EIGHTH ? 0.125 : (bcomp ? cb : cr[(u + 3 - p) % 3]);
+                const double c = KIND == KIND_EIGHTH ? 0.125 : cr[(u + 3 - p) % 3];
                 const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
                                  difw(zm, f0, zp, szw[0][k - kz0], szw[1][k - kz0]);
                 double v = pin ? f0 + c * s : 0.0;
@@ -612,16 +617,16 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         int tb = t0;
         if (MODE != ZM_ALL) {
         for (; tb < tend && !fast(tb); tb += ZPF) {
-    #pragma unroll
+            #pragma unroll
             for (int u = 0; u < ZPF; u++) step(u, tb + u);
         }
         for (; tb < tend && fast(tb); tb += ZPF) {
-    #pragma unroll
+            #pragma unroll
             for (int u = 0; u < ZPF; u++) fast_step(u, tb + u);
         }
         }
         for (; tb < tend; tb += ZPF) {
-    #pragma unroll
+            #pragma unroll
             for (int u = 0; u < ZPF; u++) step(u, tb + u);
         }
     };
@@ -712,7 +717,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     double qx4[5], qt24[5], qt34[5], qt44[5], qp[5];
     double yx2[3], yx3[3], yt22[3], yt23[3], yt33[3];
     if (ld) {
-#pragma unroll
+        #pragma unroll
         for (int s = 0; s < 4; s++) {
             Rec r = rec_at(X, pri, S, q0 + sz * (t.kb - 2 + s));
             qx4[s] = r.x4; qt24[s] = r.t24; qt34[s] = r.t34; qt44[s] = r.t44; qp[s] = r.p;
@@ -743,7 +748,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             int hx = t.tx < 2 ? -2 : 2;
             if (loadable(g, t.i + hx, t.j)) {
                 Rec r = rec_at(X, pri, S, t.i + hx + g.sy * rmap(g, t.j) + kz);
-#pragma unroll
+                #pragma unroll
                 for (int m = 0; m < NLA; m++) sp[m][t.ty + 2][t.tx + 2 + hx] = rec_field(r, m);
             }
         }
@@ -751,7 +756,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             int hy = t.ty < 2 ? -2 : 2;
             if (loadable(g, t.i, t.j + hy)) {
                 Rec r = rec_at(X, pri, S, t.i + g.sy * rmap(g, t.j + hy) + kz);
-#pragma unroll
+                #pragma unroll
                 for (int m = 0; m < NLA; m++) sp[m][t.ty + 2 + hy][t.tx + 2] = rec_field(r, m);
             }
         }
@@ -791,11 +796,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             }
         }
         __syncthreads();
-#pragma unroll
+        #pragma unroll
         for (int s = 0; s < 4; s++) {
             qx4[s] = qx4[s + 1]; qt24[s] = qt24[s + 1]; qt34[s] = qt34[s + 1]; qt44[s] = qt44[s + 1]; qp[s] = qp[s + 1];
         }
-#pragma unroll
+        #pragma unroll
         for (int s = 0; s < 2; s++) {
             yx2[s] = yx2[s + 1]; yx3[s] = yx3[s + 1]; yt22[s] = yt22[s + 1]; yt23[s] = yt23[s + 1]; yt33[s] = yt33[s + 1];
         }
@@ -839,7 +844,7 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
         }
     };
     if (ld) {
-#pragma unroll
+        #pragma unroll
         for (int s = 0; s < 4; s++) load_q(s, q0 + sz * (t.kb - 2 + s));
     }
     double ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
@@ -869,7 +874,7 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
         });
         __syncthreads();
         double v[8];
-#pragma unroll
+        #pragma unroll
         for (int m = 0; m < 8; m++) v[m] = 0.0;
         if (t.inner) {
             const int X0 = t.tx + 2, Y0 = t.ty + 2;
@@ -915,12 +920,12 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
         }
         plane_rules<8, BC_STATE>(g, t, bs, v);
         if (t.own) {
-#pragma unroll
+            #pragma unroll
             for (int m = 0; m < 8; m++) st(XD + qk + m * S, v[m]);
             zface_store<8>(g, XD, q0, k, v);
         }
         __syncthreads();
-#pragma unroll
+        #pragma unroll
         for (int s = 0; s < 4; s++) {
             qe1[s] = qe1[s + 1]; qe2[s] = qe2[s + 1]; qp[s] = qp[s + 1]; qvz[s] = qvz[s + 1];
             if (VD) { qvx[s] = qvx[s + 1]; qvy[s] = qvy[s + 1]; qrh[s] = qrh[s + 1]; }
@@ -986,7 +991,7 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
     // z queues (planes t-4..t) and partial sums of planes t-1, t
     double qa[5], qb[5];
     double pa[2], pb[2], pc[2];
-#pragma unroll
+    #pragma unroll
     for (int s = 0; s < 5; s++) qa[s] = qb[s] = 0.0;
     pa[0] = pa[1] = pb[0] = pb[1] = pc[0] = pc[1] = 0.0;
     // raw values of the next plane
@@ -1017,7 +1022,7 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
         if (t < t1) fetch(t + 1);
         const int b = t & 1;
         sp[b][0][r][lane] = f0; sp[b][1][r][lane] = f1; sp[b][2][r][lane] = f2;
-#pragma unroll
+        #pragma unroll
         for (int s = 0; s < 4; s++) { qa[s] = qa[s + 1]; qb[s] = qb[s + 1]; }
         qa[4] = za; qb[4] = zb;
         // J^2 of the output plane t-2 (pressure equation)
@@ -1055,21 +1060,21 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
                     v[0] = pa[0] - DZQ(qa) * hs0 + (pb[0] - DZQ(qb)) * gam * p0 + (gam - 1.) * eta * j2;
                 }
             } else {
-#pragma unroll
+                #pragma unroll
                 for (int o = 0; o < NO; o++) v[o] = 0.0;
             }
             if (xft) {
-#pragma unroll
+                #pragma unroll
                 for (int o = 0; o < NO; o++) v[o] = row_xrule_rl<BC_STATE>(g, i, i0, vx && yin, v[o]);
             }
             if (!g.periodic) {
                 if (j == 2 || j == g.NY - 3) {
-#pragma unroll
+                    #pragma unroll
                     for (int o = 0; o < NO; o++) sx[o][j == 2 ? 0 : 1][lane] = v[o];
                 }
                 __syncthreads();
                 if (img) {
-#pragma unroll
+                    #pragma unroll
                     for (int o = 0; o < NO; o++)
                         v[o] = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sx[o][j < 2 ? 0 : 1][lane];
                 }
@@ -1077,12 +1082,12 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
             if (own) {
                 const unsigned e = qc + sz * (unsigned)k;
                 const int m0 = PART == 0 ? 4 : 7;
-#pragma unroll
+                #pragma unroll
                 for (int o = 0; o < NO; o++) st(fldp(XD + (m0 + o) * S, e), v[o]);
                 if (k == 2 || k == g.NZ - 3) {
                     const unsigned eh = qc + sz * (unsigned)(k == 2 ? 1 : g.NZ - 2);
                     const unsigned e0 = qc + sz * (unsigned)(k == 2 ? 0 : g.NZ - 1);
-#pragma unroll
+                    #pragma unroll
                     for (int o = 0; o < NO; o++) {
                         st(fldp(XD + (m0 + o) * S, eh), 0.5 * v[o]);
                         st(fldp(XD + (m0 + o) * S, e0), 0.0);
@@ -1122,7 +1127,7 @@ __global__ __launch_bounds__(NT) void k_smooth_tt(Geo g, const double *__restric
     };
     double zm[NF], z0[NF], zp[NF];
     if (ld) {
-#pragma unroll
+        #pragma unroll
         for (int m = 0; m < NF; m++) { zm[m] = val(m, q0 + sz * (t.kb - 1)); z0[m] = val(m, q0 + sz * t.kb); }
     }
     const double wpx = t.inner ? g.xwp[t.i] : 0, wmx = t.inner ? g.xwm[t.i] : 0;
@@ -1132,7 +1137,7 @@ __global__ __launch_bounds__(NT) void k_smooth_tt(Geo g, const double *__restric
     for (int k = t.kb; k < t.ke; k++) {
         const long kz = sz * k;
         if (ld) {
-#pragma unroll
+            #pragma unroll
             for (int m = 0; m < NF; m++) zp[m] = val(m, q0 + kz + sz);
         }
         load_plane<1, NF>(g, t, sp, kz, val);
@@ -1140,7 +1145,7 @@ __global__ __launch_bounds__(NT) void k_smooth_tt(Geo g, const double *__restric
         if (t.inner && t.own) {
             const double wpz = g.zwp[k], wmz = g.zwm[k];
             const double crad = radial_coef(xy2, g.zc[k]);
-#pragma unroll
+            #pragma unroll
             for (int m = 0; m < NF; m++) {
                 double f0 = z0[m];
                 double s = difw(sp[m][t.ty + 1][t.tx], f0, sp[m][t.ty + 1][t.tx + 2], wpx, wmx) +
@@ -1151,7 +1156,7 @@ __global__ __launch_bounds__(NT) void k_smooth_tt(Geo g, const double *__restric
             }
         }
         __syncthreads();
-#pragma unroll
+        #pragma unroll
         for (int m = 0; m < NF; m++) { zm[m] = z0[m]; z0[m] = zp[m]; }
     }
 }
@@ -1236,7 +1241,7 @@ __global__ void k_bc_tt(Geo g, double *B, const double *A, const double *X, Eq e
     int i, j, k;
     if (!face_point(g, t, i, j, k)) return;
     long q = IDX(g, i, j, k);
-#pragma unroll
+    #pragma unroll
     for (int m = 0; m < 8; m++) {
         TTEval ev{g, B + m * g.S, A ? A + m * g.S : nullptr, X ? X + m * g.S : nullptr, X ? eq.f[m] : nullptr};
         B[q + m * g.S] = ev.Zv(i, j, k);
