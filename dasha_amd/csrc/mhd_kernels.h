@@ -615,7 +615,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         // the plane groups that may use the fast step form one run
         auto fast = [&](int b) { return b >= NP + 3 && b + ZPF - 1 <= g.NZ - 3; };
         int tb = t0;
-        if (MODE != ZM_ALL) {
+        if (MODE == ZM_PLAIN || MODE == ZM_FACE) {
         for (; tb < tend && !fast(tb); tb += ZPF) {
             #pragma unroll
             for (int u = 0; u < ZPF; u++) step(u, tb + u);
