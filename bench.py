@@ -12,6 +12,8 @@ Prints ONE JSON line:
   roofline       dominant kernel family: algorithmic bytes / HIP-event time
   step_roofline  whole step: value * 6256 B (5104 B without Hall) / 8 TB/s
   cpu_baseline   the CPU oracle (C restatement, OpenMP) on a bounded sample
+  C3             the same timed loop on 512x256x512
+  ensemble       48 uneven Harris runs advanced together vs one after another
 """
 from __future__ import annotations
 
@@ -23,6 +25,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the ensemble runs each use a stream of their own: with 16 hardware queues
+# (HIP's default is 4) more of them execute at once (read at HIP start-up)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
 B_ALG_HALL = 6256           # algorithmic bytes per cell per RK4 step (Hall on)
@@ -111,6 +116,41 @@ def time_config(name: str, steps: int, warmup: int):
             "step_roofline_frac": value * b_alg / HBM_PEAK}
 
 
+def time_ensemble(steps: int, nruns: int = 48):
+    """the batched ensemble (48 Harris runs of uneven size and parameters):
+    aggregate cell-updates/s of the runs advanced together against the
+    same runs advanced one after another"""
+    from dasha_amd import harris
+    from dasha_amd.mhd import MHDEnsemble, MHDStep
+    t0 = time.perf_counter()
+    cases = harris.case_c4(nruns=nruns)
+    runs = [MHDStep(c) for c in cases]
+    for g in runs:
+        g.init_residual()
+    dt0 = [g.compute_dt() for g in runs]
+    cells = sum(c.cells for c in cases)
+    print(f"[bench] ensemble: {nruns} runs, {cells} cells, setup {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    # one warm-up step each, then the runs one after another
+    times = [g.advance(1, 0.0, d)[0] for g, d in zip(runs, dt0)]
+    seq_ms = 0.0
+    for r, (g, d) in enumerate(zip(runs, dt0)):
+        times[r], _, ms = g.advance(steps, times[r], d)
+        seq_ms += ms
+    ens = MHDEnsemble(runs)
+    tt, _ = ens.advance(2, times, dt0)           # captures the step graphs
+    tt, ens_ms = ens.advance(steps, tt, dt0)
+    ens.close()
+    for g in runs:
+        g.close()
+    print(f"[bench] ensemble: sequential {seq_ms:.1f} ms, together {ens_ms:.1f} ms", file=sys.stderr, flush=True)
+    return {"workload": f"{nruns} Harris runs, 64x32x64 to 192x96x192, uneven eta, d_i, amplitude, "
+                        f"thickness (RK4 Hall MHD step each)",
+            "runs": nruns, "cells": cells, "steps": steps, "unit": "cell-updates/s",
+            "value": cells * steps / (ens_ms / 1e3), "sequential_value": cells * steps / (seq_ms / 1e3),
+            "speedup": seq_ms / ens_ms, "ms": ens_ms, "sequential_ms": seq_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,11 +163,18 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="also measure a plain device copy")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the 512x256x512 timing added to the 256x128x256 line")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble timing")
+    ap.add_argument("--ensemble-only", action="store_true", help="time the batched ensemble alone")
+    ap.add_argument("--ensemble-runs", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
               f"(the step runs on one GPU's domain).", file=sys.stderr)
         sys.exit(2)
+
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": time_ensemble(10, args.ensemble_runs)}), flush=True)
+        return
 
     from dasha_amd import harris
     from dasha_amd.mhd import MHDStep
@@ -206,6 +253,8 @@ def main():
     g.close()
     if args.config == "C2" and not args.no_c3:
         out["C3"] = time_config("C3", 10, 2)
+    if args.config == "C2" and not args.no_ensemble:
+        out["ensemble"] = time_ensemble(10, args.ensemble_runs)
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_steps, threads)

@@ -1261,6 +1261,190 @@ int mhd_copy_rate(long bytes, int iters, double *bytes_per_s)
 
 const char *mhd_last_error(const mhd_handle *h) { return h ? h->err.c_str() : "null handle"; }
 
+} // extern "C"
+
+// ---- ensemble: independent runs advanced concurrently ----
+// Every run keeps its own stream.  A step pair (setdt; stepon; setdt;
+// stepon) of a run is captured once as a HIP graph: stepon renames x and
+// xfold on the host, so a pair leaves the pointers where it found them and
+// the graph can be replayed; a graph is kept for each of the two pointer
+// states a run can be in.  The pairs of all runs are launched round-robin,
+// so the device works on as many runs at once as it has hardware queues.
+struct RunGraphs {
+    double *x = nullptr;                   // the run's x buffer when captured
+    hipGraphExec_t first = nullptr;        // pair opening an advance (no time += dt first)
+    hipGraphExec_t next = nullptr;
+};
+
+struct mhd_ensemble {
+    std::vector<mhd_handle *> runs;
+    std::vector<RunGraphs> graphs;         // two per run
+    std::vector<int> status;
+    std::vector<hipEvent_t> done;          // end of each run's work
+    hipStream_t lead = nullptr;            // timing stream
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string err;
+};
+
+static int capture_pair(mhd_handle *h, int opening, hipGraphExec_t *out)
+{
+    hipGraph_t gr = nullptr;
+    HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    dt_kernels(h, opening ? 0 : 1);
+    step_body(h);
+    dt_kernels(h, 1);
+    step_body(h);
+    HCHK(hipStreamEndCapture(h->stream, &gr));
+    const hipError_t e = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+        h->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+        return MHD_ERR_HIP;
+    }
+    return MHD_OK;
+}
+
+static void drop_graphs(RunGraphs &g)
+{
+    if (g.first) (void)hipGraphExecDestroy(g.first);
+    if (g.next) (void)hipGraphExecDestroy(g.next);
+    g = RunGraphs{};
+}
+
+// the graphs of run r for its current pointer state (captured when missing)
+static int run_graphs(mhd_ensemble *e, int r, RunGraphs **out)
+{
+    mhd_handle *h = e->runs[r];
+    RunGraphs *slot = nullptr;
+    for (int k = 0; k < 2; k++) {
+        RunGraphs &g = e->graphs[2 * r + k];
+        if (g.x == h->x) { *out = &g; return MHD_OK; }
+        if (!slot && !g.x) slot = &g;
+    }
+    if (!slot) { slot = &e->graphs[2 * r]; drop_graphs(*slot); }
+    int rc = capture_pair(h, 1, &slot->first);
+    if (!rc) rc = capture_pair(h, 0, &slot->next);
+    if (rc) { drop_graphs(*slot); return rc; }
+    slot->x = h->x;
+    *out = slot;
+    return MHD_OK;
+}
+
+extern "C" {
+
+int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out)
+{
+    if (!runs || n <= 0 || !out) return MHD_ERR_ARG;
+    *out = nullptr;
+    for (int r = 0; r < n; r++)
+        if (!runs[r]) return MHD_ERR_ARG;
+    for (int r = 0; r < n; r++)
+        for (int q = 0; q < r; q++)
+            if (runs[q] == runs[r]) return MHD_ERR_ARG;   // a run appears once
+    mhd_ensemble *e = new mhd_ensemble();
+    e->runs.assign(runs, runs + n);
+    e->graphs.resize(2 * n);
+    e->status.assign(n, MHD_OK);
+    e->done.assign(n, nullptr);
+    *out = e;
+    hipError_t st = hipStreamCreate(&e->lead);
+    if (st == hipSuccess) st = hipEventCreate(&e->e0);
+    if (st == hipSuccess) st = hipEventCreate(&e->e1);
+    for (int r = 0; r < n && st == hipSuccess; r++) st = hipEventCreateWithFlags(&e->done[r], hipEventDisableTiming);
+    if (st != hipSuccess) { e->err = std::string("ensemble set-up: ") + hipGetErrorString(st); return MHD_ERR_HIP; }
+    return MHD_OK;
+}
+
+int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double *time, double *elapsed_ms)
+{
+    if (!e || nsteps < 0 || !dt0 || !time) return MHD_ERR_ARG;
+    const int n = (int)e->runs.size();
+    std::vector<RunGraphs *> gs(n, nullptr);
+    for (int r = 0; r < n; r++) {
+        mhd_handle *h = e->runs[r];
+        e->status[r] = MHD_OK;
+        if (!h->have_eq || h->prof) {
+            e->err = "run " + std::to_string(r) + (h->prof ? ": profiling is on" : ": equilibrium not set");
+            return MHD_ERR_ARG;
+        }
+        if (nsteps >= 2) {
+            const int rc = run_graphs(e, r, &gs[r]);
+            if (rc) { e->err = "run " + std::to_string(r) + ": " + h->err; return rc; }
+        }
+    }
+    auto hip_fail = [&](hipError_t st, const char *what) {
+        e->err = std::string(what) + ": " + hipGetErrorString(st);
+        return MHD_ERR_HIP;
+    };
+    hipError_t st;
+    for (int r = 0; r < n; r++) {
+        mhd_handle *h = e->runs[r];
+        const double init[8] = {0, time[r], dt0[r], 0, 0, 0, 0, 0};
+        if ((st = hipMemcpyAsync(h->dtv, init, sizeof(init), hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+            return hip_fail(st, "dt upload");
+        if ((st = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(st, "dt upload");
+    }
+    if ((st = hipEventRecord(e->e0, e->lead)) != hipSuccess) return hip_fail(st, "hipEventRecord");
+    for (int r = 0; r < n; r++)
+        if ((st = hipStreamWaitEvent(e->runs[r]->stream, e->e0, 0)) != hipSuccess) return hip_fail(st, "hipStreamWaitEvent");
+    // step pairs round-robin over the runs, then an odd last step eagerly
+    for (int p = 0; p < nsteps / 2; p++)
+        for (int r = 0; r < n; r++)
+            if ((st = hipGraphLaunch(p == 0 ? gs[r]->first : gs[r]->next, e->runs[r]->stream)) != hipSuccess)
+                return hip_fail(st, "hipGraphLaunch");
+    if (nsteps % 2)
+        for (int r = 0; r < n; r++) {
+            dt_kernels(e->runs[r], nsteps > 1 ? 1 : 0);
+            step_body(e->runs[r]);
+        }
+    for (int r = 0; r < n; r++) {
+        if ((st = hipEventRecord(e->done[r], e->runs[r]->stream)) != hipSuccess) return hip_fail(st, "hipEventRecord");
+        if ((st = hipStreamWaitEvent(e->lead, e->done[r], 0)) != hipSuccess) return hip_fail(st, "hipStreamWaitEvent");
+    }
+    if ((st = hipEventRecord(e->e1, e->lead)) != hipSuccess) return hip_fail(st, "hipEventRecord");
+    if ((st = hipEventSynchronize(e->e1)) != hipSuccess) return hip_fail(st, "ensemble run");
+    float ms = 0;
+    if ((st = hipEventElapsedTime(&ms, e->e0, e->e1)) != hipSuccess) return hip_fail(st, "hipEventElapsedTime");
+    if (elapsed_ms) *elapsed_ms = ms;
+    if ((st = hipGetLastError()) != hipSuccess) return hip_fail(st, "ensemble run");
+    int first = MHD_OK;
+    for (int r = 0; r < n; r++) {
+        mhd_handle *h = e->runs[r];
+        double v[8];
+        if ((st = hipMemcpy(v, h->dtv, sizeof(v), hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(st, "dt download");
+        time[r] = v[1] + (nsteps > 0 ? v[4] : 0.0);
+        if (v[3] != 0.0) {
+            set_err(h, "dt collapsed below 0.1*dt0");
+            e->status[r] = MHD_ERR_DT_COLLAPSE;
+            if (first == MHD_OK) { first = MHD_ERR_DT_COLLAPSE; e->err = "run " + std::to_string(r) + ": " + h->err; }
+        }
+    }
+    return first;
+}
+
+int mhd_ensemble_status(const mhd_ensemble *e, int *status)
+{
+    if (!e || !status) return MHD_ERR_ARG;
+    for (size_t r = 0; r < e->status.size(); r++) status[r] = e->status[r];
+    return MHD_OK;
+}
+
+const char *mhd_ensemble_error(const mhd_ensemble *e) { return e ? e->err.c_str() : "null ensemble"; }
+
+void mhd_ensemble_destroy(mhd_ensemble *e)
+{
+    if (!e) return;
+    if (e->lead) (void)hipStreamSynchronize(e->lead);
+    for (auto *h : e->runs) (void)hipStreamSynchronize(h->stream);
+    for (auto &g : e->graphs) drop_graphs(g);
+    for (auto ev : e->done)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->e0) (void)hipEventDestroy(e->e0);
+    if (e->e1) (void)hipEventDestroy(e->e1);
+    if (e->lead) (void)hipStreamDestroy(e->lead);
+    delete e;
+}
+
 void mhd_destroy(mhd_handle *h)
 {
     if (!h) return;

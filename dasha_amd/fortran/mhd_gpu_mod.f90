@@ -86,6 +86,35 @@ module mhd_gpu
        import :: c_ptr
        type(c_ptr), value :: h
      end subroutine mhd_destroy
+
+     ! ensemble of independent runs (handles created as above), advanced together
+     integer(c_int) function mhd_ensemble_create(runs, n, e) bind(C, name='mhd_ensemble_create')
+       import :: c_int, c_ptr
+       type(c_ptr), intent(in) :: runs(*)
+       integer(c_int), value :: n
+       type(c_ptr), intent(out) :: e
+     end function mhd_ensemble_create
+
+     integer(c_int) function mhd_ensemble_advance(e, nsteps, dt0, time, elapsed_ms) &
+          bind(C, name='mhd_ensemble_advance')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: e
+       integer(c_int), value :: nsteps
+       real(c_double), intent(in) :: dt0(*)
+       real(c_double), intent(inout) :: time(*)
+       real(c_double), intent(out) :: elapsed_ms
+     end function mhd_ensemble_advance
+
+     integer(c_int) function mhd_ensemble_status(e, status) bind(C, name='mhd_ensemble_status')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: e
+       integer(c_int), intent(out) :: status(*)
+     end function mhd_ensemble_status
+
+     subroutine mhd_ensemble_destroy(e) bind(C, name='mhd_ensemble_destroy')
+       import :: c_ptr
+       type(c_ptr), value :: e
+     end subroutine mhd_ensemble_destroy
   end interface
 
 contains

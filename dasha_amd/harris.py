@@ -233,4 +233,26 @@ def case_small(mx: int = 24, ny: int = 8, mz: int = 24, *, stretched: bool = Tru
                        **kw)
 
 
+def case_c4(seed: int = 4, nruns: int = 48) -> list:
+    """The batched ensemble: nruns Harris-sheet runs, uniform meshes on the
+    20 x 10 x 20 box of C1, size scale s in {1, 1.25, ..., 3} drawn from the
+    seed -> (64s, 32s, 64s) rounded to multiples of 16; eta ~ U[1e-3, 1e-2],
+    d_i ~ U[0, 0.5], perturbation amplitude ~ U[0.02, 0.2], sheet thickness
+    ~ U[0.3, 1.0]; 500 steps each."""
+    rng = np.random.default_rng(seed)
+
+    def r16(v: float) -> int:
+        return int(16 * np.floor(v / 16 + 0.5))
+
+    out = []
+    for r in range(nruns):
+        s = 1.0 + 0.25 * int(rng.integers(0, 9))
+        mx, ny = r16(64 * s), r16(32 * s)
+        eta, di = rng.uniform(1e-3, 1e-2), rng.uniform(0.0, 0.5)
+        amp, lam = rng.uniform(0.02, 0.2), rng.uniform(0.3, 1.0)
+        out.append(harris_case(f"C4_{r:02d}_{mx}x{ny}x{mx}", mx, ny, mx, stretched=False, lam=lam,
+                               beta_inf=0.2, eta0=eta, di=di, amp=amp, seed=100 + r, steps=500))
+    return out
+
+
 CASES = {"C1": case_c1, "C2": case_c2, "C3": case_c3}

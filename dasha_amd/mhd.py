@@ -26,7 +26,8 @@ EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state"
            "mhd_init_residual", "mhd_compute_dt", "mhd_step", "mhd_advance", "mhd_get_current",
            "mhd_diagnostics", "mhd_get_divb", "mhd_rhs", "mhd_get_xdif", "mhd_get_residual",
            "mhd_set_residual", "mhd_set_profiling", "mhd_kernel_times", "mhd_copy_rate",
-           "mhd_last_error", "mhd_destroy")
+           "mhd_last_error", "mhd_destroy", "mhd_ensemble_create", "mhd_ensemble_advance",
+           "mhd_ensemble_status", "mhd_ensemble_error", "mhd_ensemble_destroy")
 
 
 class MHDConfig(C.Structure):
@@ -80,6 +81,12 @@ def load_library(path: str = LIBPATH):
     L.mhd_last_error.argtypes = [P]
     L.mhd_last_error.restype = C.c_char_p
     L.mhd_destroy.argtypes = [P]
+    L.mhd_ensemble_create.argtypes = [C.POINTER(P), I, C.POINTER(P)]
+    L.mhd_ensemble_advance.argtypes = [P, I, D, D, D]
+    L.mhd_ensemble_status.argtypes = [P, C.POINTER(I)]
+    L.mhd_ensemble_error.argtypes = [P]
+    L.mhd_ensemble_error.restype = C.c_char_p
+    L.mhd_ensemble_destroy.argtypes = [P]
     _lib = L
     return L
 
@@ -221,3 +228,49 @@ class MHDStep:
             time += dt
             dts.append(dt)
         return time, dts
+
+
+class MHDEnsemble:
+    """Independent runs (MHDStep objects, owned by the caller) advanced
+    together: every run does setdt + stepon with its own device dt, all runs
+    concurrently on their own streams."""
+
+    def __init__(self, runs):
+        self.L = load_library()
+        self.runs = list(runs)
+        arr = (C.c_void_p * len(self.runs))(*[r.h for r in self.runs])
+        e = C.c_void_p()
+        rc = self.L.mhd_ensemble_create(arr, len(self.runs), C.byref(e))
+        self.e = e
+        if rc != MHD_OK:
+            msg = self.L.mhd_ensemble_error(e).decode() if e else ""
+            self.close()
+            raise MHDError(rc, msg)
+
+    def advance(self, nsteps: int, times, dt0s):
+        """nsteps for every run from times[r] (dt0s[r] arms the collapse
+        guard); returns (new times, elapsed ms of the batch)"""
+        t = np.ascontiguousarray(times, dtype=np.float64).copy()
+        d = np.ascontiguousarray(dt0s, dtype=np.float64)
+        assert t.shape == d.shape == (len(self.runs),)
+        ms = C.c_double()
+        rc = self.L.mhd_ensemble_advance(self.e, int(nsteps), _p(d), _p(t), C.byref(ms))
+        if rc != MHD_OK:
+            raise MHDError(rc, self.L.mhd_ensemble_error(self.e).decode())
+        return t, ms.value
+
+    def status(self):
+        st = (C.c_int * len(self.runs))()
+        self.L.mhd_ensemble_status(self.e, st)
+        return list(st)
+
+    def close(self):
+        if getattr(self, "e", None):
+            self.L.mhd_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -120,6 +120,25 @@ int mhd_copy_rate(long bytes, int iters, double *bytes_per_s);
 const char *mhd_last_error(const mhd_handle *h);
 void mhd_destroy(mhd_handle *h);
 
+/* Ensemble of independent runs (no counterpart in the reference, which runs
+ * one configuration per job; the batched form of the time loop above).
+ * The runs are handles the caller created and initialised (equilibrium,
+ * state, residual) as for mhd_advance; they stay owned by the caller and
+ * must outlive the ensemble.  mhd_ensemble_advance advances every run
+ * nsteps x (setdt; stepon; time += dt) with its own device dt, all runs
+ * concurrently: each run's step pair is a captured HIP graph replayed on
+ * the run's stream.  dt0[r] arms run r's collapse guard; time[r] is
+ * advanced in place; *elapsed_ms is the wall time of the whole batch
+ * (HIP events).  Returns the first failing run's status (its text via
+ * mhd_ensemble_error); mhd_ensemble_status gives every run's. */
+typedef struct mhd_ensemble mhd_ensemble;
+int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out);
+int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double *time,
+                         double *elapsed_ms);
+int mhd_ensemble_status(const mhd_ensemble *e, int *status);
+const char *mhd_ensemble_error(const mhd_ensemble *e);
+void mhd_ensemble_destroy(mhd_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif
