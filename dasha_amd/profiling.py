@@ -43,7 +43,15 @@ _FAMILIES = [
 
 
 # families whose kernels are launched together for one call of the family
-_TOGETHER = {"rhs_curlE_pressure"}
+_TOGETHER = {"rhs_curlE_pressure", "setdt"}
+
+
+def launch_group(kernel_name: str) -> str:
+    """kernels of one family launched together share a group: the smoothing
+    sweeps run as two launches, the windows without face rules (last
+    template argument 1) and those with them (2)"""
+    inst = re.sub(r"\(.*", "", kernel_name.replace("(anonymous namespace)::", ""))
+    return re.sub(r", [12]>$", ">", inst)
 
 
 def family(kernel_name: str) -> str | None:
@@ -89,22 +97,22 @@ def traffic(fetch_dir: str, write_dir: str, copy_bytes: int):
         raise RuntimeError("the copy kernel is missing from the counter runs")
     f_scale = copy_bytes / (fetch[ck[0]] * 1024.0)
     w_scale = copy_bytes / (write[ck[0]] * 1024.0)
-    fams = collections.defaultdict(lambda: [0.0, 0.0, 0])
+    fams = collections.defaultdict(lambda: collections.defaultdict(lambda: [0.0, 0.0]))
     for k in fetch:
         fam = family(k)
         if fam is None or fam == "copy_probe" or k not in write:
             continue
-        a = fams[fam]
+        a = fams[fam]["*" if fam in _TOGETHER else launch_group(k)]
         a[0] += fetch[k] * 1024.0 * f_scale
         a[1] += write[k] * 1024.0 * w_scale
-        a[2] += 1
-    # a family is either alternative kernels (one per launch: the mean) or
-    # kernels that run together inside one timed bracket (the sum)
+    # kernels launched together inside one timed bracket add up; the
+    # family's alternatives (RK stages) average
     res = {}
-    for fam, v in fams.items():
-        n = 1 if fam in _TOGETHER else v[2]
-        res[fam] = {"read_bytes_per_launch": v[0] / n, "write_bytes_per_launch": v[1] / n,
-                    "bytes_per_launch": (v[0] + v[1]) / n}
+    for fam, groups in fams.items():
+        n = len(groups)
+        rd = sum(v[0] for v in groups.values()) / n
+        wr = sum(v[1] for v in groups.values()) / n
+        res[fam] = {"read_bytes_per_launch": rd, "write_bytes_per_launch": wr, "bytes_per_launch": rd + wr}
     return res, f_scale, w_scale
 
 
@@ -127,6 +135,20 @@ def main():
         nm = re.sub(r"\(.*", "", r["name"].replace("(anonymous namespace)::", ""))
         lines.append(f"| {r['family'] or '-'} | `{nm}` | {r['calls']} | {r['avg_us']:.1f} | "
                      f"{r['total_ms']:.3f} | {r['pct']:.1f} |")
+    # per family: kernels launched together add up, alternatives count apart
+    fam_t = collections.defaultdict(lambda: collections.defaultdict(lambda: [0.0, 0]))
+    for r in ks:
+        if r["family"] is None or r["family"] == "copy_probe":
+            continue
+        g = fam_t[r["family"]]["*" if r["family"] in _TOGETHER else launch_group(r["name"])]
+        g[0] += r["total_ms"]
+        g[1] = max(g[1], r["calls"])
+    lines += ["", "Per launch of each kernel family (the kernels one call of the family runs; "
+              "what bench.py brackets with HIP events):", "",
+              "| kernel family | launches | avg ms |", "|---|---|---|"]
+    for fam, groups in sorted(fam_t.items(), key=lambda kv: -sum(v[0] for v in kv[1].values())):
+        calls = sum(v[1] for v in groups.values())
+        lines.append(f"| {fam} | {calls} | {sum(v[0] for v in groups.values()) / max(calls, 1):.3f} |")
     if args.fetch and args.write:
         tr, fs, ws = traffic(args.fetch, args.write, args.copy_bytes)
         lines += ["", "HBM traffic per launch (FETCH_SIZE x %.3f, WRITE_SIZE x %.3f: the factors that make "
