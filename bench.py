@@ -47,6 +47,7 @@ def per_cell_bytes(case):
         "avrg_radial_x1": 8 * (3 + 3),
         "avrgh_x2": 2 * 8 * (3 + 3),
         "avrgh_x1": 8 * (3 + 3),
+        "avrgh_x3_addE": 3 * 8 * (3 + 3),       # the last three avrgh passes (+ E, as in avrgh_x1)
         "avrg_state_x2": 2 * 8 * (8 + 8),
         "avrg_state_x1": 8 * (8 + 8),
         "avrg_state_x2_rk": 8 * (8 + 8) + rk2,  # avrg pass 1 + (pass 2 fused with the RK update)

@@ -347,6 +347,8 @@ struct mhd_handle {
     int zkc = 32;                          // interior planes per block of the temporal-blocked kernels
     int zord = 1;                          // block order of the temporal-blocked kernels (zs_tables)
     ZTab zt[2][2];                         // their block tables: [3 or 8 fields][without / with face rules]
+    ZTab zt3[2];                           // the same for three passes on 3 fields
+    int zs3 = 1;                           // last three avrgh passes (+ E) in one launch
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -432,12 +434,12 @@ static void prof_mark(mhd_handle *h, int tag)
 // kernel families (one name per kernel shape; _x2 = two passes in one launch)
 enum {
     PT_CURL, PT_RAD2, PT_RAD1, PT_EIG2, PT_EIG1, PT_ST2, PT_ST1, PT_ST2RK, PT_ST1RK, PT_RK, PT_FILTTT, PT_BC,
-    PT_RHSA, PT_RHSB, PT_POST, PT_DT, PT_TT2, PT_N
+    PT_RHSA, PT_RHSB, PT_POST, PT_DT, PT_TT2, PT_EIG3, PT_N
 };
 static const char *PT_NAMES[PT_N] = {"curl", "avrg_radial_x2", "avrg_radial_x1", "avrgh_x2", "avrgh_x1",
                                      "avrg_state_x2", "avrg_state_x1", "avrg_state_x2_rk", "avrg_state_x1_rk",
                                      "rk_update", "avrgtt", "bndrytt", "rhs_flux_E_hall", "rhs_curlE_pressure",
-                                     "post_add_eq", "setdt", "avrgtt_x2_eq"};
+                                     "post_add_eq", "setdt", "avrgtt_x2_eq", "avrgh_x3_addE"};
 
 static int prof_collect(mhd_handle *h)
 {
@@ -510,9 +512,9 @@ static void launch_sweep(mhd_handle *h, const double *a, double *b, const double
 // with the field fastest, then x tile, y tile, chunk: one tile's fields read
 // the same radial coefficients together, and neighbouring tiles share halo
 // columns and rows, in one L2.
-static int zs_tables(mhd_handle *h, int nf, ZTab *out)
+static int zs_tables(mhd_handle *h, int nf, int np, ZTab *out)
 {
-    const int np = 2, W = TX - 2 * np, H = ZB - 2 * np;
+    const int W = TX - 2 * np, H = ZB - 2 * np;
     const Geo &g = h->g;
     const unsigned gx = (unsigned)blocks(h->NX, W);
     const unsigned gy = (unsigned)blocks(g.yhi - g.ylo + 1, H);
@@ -579,17 +581,18 @@ static void launch_zs(mhd_handle *h, dim3 grid, const unsigned *tab, int nch, co
 // full march; the other sweeps in two launches, the windows without face
 // rules, then the windows with them (measured faster for each).
 template <int NF, int NP, int KIND, int TYPE, int RKS>
-static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
+static void launch_zsmooth(mhd_handle *h, const double *a, double *b, const double *addf = nullptr)
 {
     static_assert(NF == 3 || NF == 8, "block tables exist for 3 and 8 fields");
-    const RKArgs rk{h->dtv, h->xfold, h->xm, h->x, h->eq};
+    static_assert(NP == 2 || (NP == 3 && NF == 3), "three-pass tables exist for 3 fields");
+    const RKArgs rk{h->dtv, h->xfold, h->xm, h->x, RKS == RKS_ADD ? addf : h->eq};
     if constexpr (RKS >= 0) {
         const int W = TX - 2 * NP, H = ZB - 2 * NP;
         const int nch = blocks(h->NZ - 4, h->zkc);
         const dim3 grid((unsigned)blocks(h->NX, W), (unsigned)blocks(h->g.yhi - h->g.ylo + 1, H), (unsigned)(nch * NF));
         launch_zs<NP, KIND, TYPE, RKS, ZM_ALL>(h, grid, nullptr, nch, a, b, rk);
     } else {
-        const ZTab *zt = h->zt[NF == 8 ? 1 : 0];
+        const ZTab *zt = NP == 3 ? h->zt3 : h->zt[NF == 8 ? 1 : 0];
         if (zt[0].n) launch_zs<NP, KIND, TYPE, RKS, ZM_PLAIN>(h, dim3(zt[0].n), zt[0].d, 0, a, b, rk);
         if (zt[1].n) launch_zs<NP, KIND, TYPE, RKS, ZM_FACE>(h, dim3(zt[1].n), zt[1].d, 0, a, b, rk);
     }
@@ -601,6 +604,14 @@ static void launch_zsmooth(mhd_handle *h, const double *a, double *b)
 static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk, const double *addf = nullptr)
 {
     while (kk > 0) {
+        // the last three avrgh passes and the added field in one launch
+        if (h->zs && h->zs3 && kind == KIND_EIGHTH && addf && kk == 3) {
+            PROF_BEGIN(PT_EIG3);
+            launch_zsmooth<3, 3, KIND_EIGHTH, BC_AUX, RKS_ADD>(h, a, b, addf);
+            PROF_END(PT_EIG3);
+            std::swap(a, b);
+            break;
+        }
         const bool two = h->zs && kk >= 2 && !(addf && kk == 2);
         const int tag = kind == KIND_RADIAL ? (two ? PT_RAD2 : PT_RAD1) : (two ? PT_EIG2 : PT_EIG1);
         PROF_BEGIN(tag);
@@ -995,8 +1006,13 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     if (const char *e = getenv("MHD_ZKC")) h->zkc = atoi(e) == 64 ? 64 : 32;
     if (const char *e = getenv("MHD_RB2")) h->rb2 = atoi(e);
     if (const char *e = getenv("MHD_ZORD")) h->zord = std::min(1, std::max(0, atoi(e)));
+    if (const char *e = getenv("MHD_ZS3")) h->zs3 = atoi(e);
     for (int c = 0; c < 2; c++) {
-        int r = zs_tables(h, c ? 8 : 3, h->zt[c]);
+        int r = zs_tables(h, c ? 8 : 3, 2, h->zt[c]);
+        if (r) { *out = h; return r; }
+    }
+    {
+        int r = zs_tables(h, 3, 3, h->zt3);
         if (r) { *out = h; return r; }
     }
     h->pt.resize(PT_N);
@@ -1456,6 +1472,8 @@ void mhd_destroy(mhd_handle *h)
     for (auto &zc : h->zt)
         for (auto &t : zc)
             if (t.d) hipFree(t.d);
+    for (auto &t : h->zt3)
+        if (t.d) hipFree(t.d);
     for (auto e : h->pool) (void)hipEventDestroy(e);
     if (h->stage) (void)hipHostFree(h->stage);
     if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -317,17 +317,22 @@ constexpr int ZT_BY = 10, ZT_CH = 20, ZT_F = 29;
 // face windows' march needs more than 64 registers: that launch runs at 4
 // waves per SIMD (one block per CU) rather than spill.
 enum { ZM_ALL = 0, ZM_PLAIN = 1, ZM_FACE = 2 };
+// RKS value of a sweep that stores its output plus a field (E + xdh after the
+// last avrgh pass, t3dsdsc.f:1426-1433); the field comes in rk.EQ
+constexpr int RKS_ADD = -2;
 constexpr int ZPF = 3;            // planes loaded ahead
 
 template <int NP, int KIND, int TYPE, int RKS, int KC, int MODE>
 __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE == ZM_FACE ? 4 : 8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
                                                      RKArgs rk, const unsigned *__restrict__ ztab, int nch)
 {
-    static_assert(NP >= 1 && NP <= 2, "the radial coefficient ring holds planes t-2..t");
+    static_assert(NP >= 1 && (NP <= 2 || KIND == KIND_EIGHTH), "the radial coefficient ring holds planes t-2..t");
+    static_assert(NP <= 3 && (RKS != RKS_ADD || NP == 3), "passes per launch");
     constexpr int W = TX - 2 * NP, H = ZB - 2 * NP;
     constexpr int NZW = KC + 4 * NP;           // z-weight planes of a chunk
     constexpr bool RAD = KIND != KIND_EIGHTH;   // reads a smoothing coefficient stream
     constexpr bool TT = TYPE == BC_TT;          // post-step passes on x - eq (bndrytt fills)
+    constexpr bool ADD = RKS == RKS_ADD;        // store the output plus a field
     constexpr int XT = TT ? BC_AUX : TYPE;      // x-face rule of the pass
     __shared__ double s0[3][ZB][TX];           // level 0 by plane mod 3
     __shared__ double sl[NP][2][ZB][TX];       // level p (1..NP) by plane parity
@@ -392,6 +397,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
 
         const int kb = 2 + chunk * KC, ke = min(kb + KC, g.NZ - 2);
         const int t0 = kb - NP, t1 = ke - 1 + NP;
+        const int tl = min(t1, g.NZ - 1);           // last plane loaded (three passes reach past NZ-1)
         const int kz0 = t0 - NP;                    // lowest plane a level touches
         for (int n = threadIdx.x; n < NZW; n += TX * ZB) {
             const int k = kz0 + n;
@@ -407,6 +413,8 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         double *xo = RKS >= 0 ? rk.X + m * S : nullptr;
         // equilibrium of field m (rho, B, p; none for the momentum)
         const double *eqm = ((TT || RKS == 4) && (m == 0 || m >= 4)) ? rk.EQ + (m == 0 ? 0 : m - 3) * S : nullptr;
+        // the field added to the output (ADD: rk.EQ holds a 3-field array)
+        const double *addm = ADD ? rk.EQ + m * S : nullptr;
         // final value v of point e: store it (plus eq after the post-step
         // passes, t3dsdsc.f:1157-1164), or apply the RK stage update; RKS 4 is the
         // last stage followed by xm = x - eq (t3dsdsc.f:1131, 1138-1145)
@@ -414,14 +422,14 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         // put_with combines with v
         auto put_pre = [&](unsigned e, double &o1, double &o2) {
             o1 = o2 = 0.0;
-            if (RKS < 0) { if (TT && eqm) o1 = fld(eqm, e); }
+            if (RKS < 0) { if ((TT && eqm) || ADD) o1 = fld(ADD ? addm : eqm, e); }
             else if (RKS == 0) o1 = fld(xf, e);
             else if (RKS <= 2) { o1 = fld(xmo, e); o2 = fld(xf, e); }
             else if (RKS == 3) o1 = fld(xmo, e);
             else { o1 = fld(xmo, e); if (eqm) o2 = fld(eqm, e); }
         };
         auto put_with = [&](unsigned e, double v, double o1, double o2) {
-            if (RKS < 0) st(fldp(bo, e), (TT && eqm) ? v + o1 : v);
+            if (RKS < 0) st(fldp(bo, e), ((TT && eqm) || ADD) ? v + o1 : v);
             else if (RKS == 0) { st(fldp(xmo, e), o1 + v * dt / 6.); st(fldp(xo, e), o1 + v * dt / 2.); }
             else if (RKS == 1) { st(fldp(xmo, e), o1 + v * dt / 3.); st(fldp(xo, e), o2 + v * dt / 2.); }
             else if (RKS == 2) { st(fldp(xmo, e), o1 + v * dt / 3.); st(fldp(xo, e), o2 + v * dt); }
@@ -429,7 +437,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
             else { const double xn = o1 + v * dt / 6.; st(fldp(xmo, e), eqm ? xn - o2 : xn); }
         };
         auto put = [&](unsigned e, double v) {
-            if (RKS < 0) st(fldp(bo, e), (TT && eqm) ? v + fld(eqm, e) : v);
+            if (RKS < 0) st(fldp(bo, e), ((TT && eqm) || ADD) ? v + fld(ADD ? addm : eqm, e) : v);
             else if (RKS == 0) { const double f = fld(xf, e); st(fldp(xmo, e), f + v * dt / 6.); st(fldp(xo, e), f + v * dt / 2.); }
             else if (RKS == 1) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt / 2.); }
             else if (RKS == 2) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt); }
@@ -454,9 +462,12 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         if (RAD && t0 >= 1) cr[2] = fld(cbase, cq0 + csz * (unsigned)(t0 - 1));
         #pragma unroll
         for (int d = 0; d < ZPF; d++) {
-            const unsigned e = qc + sz * (unsigned)min(t0 + d, t1);
-            pf[d] = fld(a, e);
-            pc[d] = RAD ? fld(cbase, cq0 + csz * (unsigned)min(t0 + d, t1)) : 0.0;
+            // three passes start at plane t0 = -1 on the first chunk: planes
+            // below 0 load plane 0 (they feed only z-face planes, which the
+            // rules overwrite)
+            const unsigned td = (unsigned)min(max(t0 + d, 0), tl);
+            pf[d] = fld(a, qc + sz * td);
+            pc[d] = RAD ? fld(cbase, cq0 + csz * td) : 0.0;
         }
 
         // one plane of the march with every rule: x faces, image rows, z faces
@@ -464,12 +475,12 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
             const double v0 = ld ? pf[u] : 0.0;
             const double cv = pc[u];
             {
-                const unsigned tz = (unsigned)min(t + ZPF, t1);
+                const unsigned tz = (unsigned)min(t + ZPF, tl);
                 pf[u] = fld(a, qc + sz * tz);
                 if (RAD) pc[u] = fld(cbase, cq0 + csz * tz);
             }
             q[0][0] = q[0][1]; q[0][1] = q[0][2]; q[0][2] = v0;
-            s0[t % 3][r][lane] = v0;
+            s0[(t + 3) % 3][r][lane] = v0;
             if (RAD) { cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cv; }
             __syncthreads();
             #pragma unroll
@@ -491,7 +502,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
                 const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
                 double v = 0.0;
                 if (pin && kin) {
-                    const double c = KIND == KIND_EIGHTH ? 0.125 : cr[2 - p];
+                    const double c = KIND == KIND_EIGHTH ? 0.125 : cr[RAD ? 2 - p : 0];
                     const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
                                      difw(q[p - 1][0], f0, q[p - 1][2], szw[0][k - kz0], szw[1][k - kz0]);
                     v = f0 + c * s;
@@ -558,12 +569,12 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
             const double v0 = ld ? pf[u] : 0.0;
             const double cv = pc[u];
             {
-                const unsigned tz = (unsigned)min(t + ZPF, t1);
+                const unsigned tz = (unsigned)min(t + ZPF, tl);
                 pf[u] = fld(a, qc + sz * tz);
                 if (RAD) pc[u] = fld(cbase, cq0 + csz * tz);
             }
             q[0][u] = v0;
-            s0[t % 3][r][lane] = v0;
+            s0[(t + 3) % 3][r][lane] = v0;
             if (RAD) cr[u] = cv;
             __syncthreads();
             #pragma unroll
@@ -582,7 +593,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
                 }
                 const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
                 const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
-                const double c = KIND == KIND_EIGHTH ? 0.125 : cr[(u + 3 - p) % 3];
+                const double c = KIND == KIND_EIGHTH ? 0.125 : cr[RAD ? (u + 3 - p) % 3 : 0];
                 const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
                                  difw(zm, f0, zp, szw[0][k - kz0], szw[1][k - kz0]);
                 double v = pin ? f0 + c * s : 0.0;

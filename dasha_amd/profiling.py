@@ -26,6 +26,7 @@ _FAMILIES = [
     (r"k_sweepz<3, 0, 0, 3, 4, -1>", "avrg_radial_x1"),
     (r"k_zsmooth<2, 1, 0, -1[,>]", "avrgh_x2"),
     (r"k_sweepz<3, 1, 0, 3, 4, -1>", "avrgh_x1"),
+    (r"k_zsmooth<3, 1, 0, -2[,>]", "avrgh_x3_addE"),
     (r"k_zsmooth<2, 2, 1, -1[,>]", "avrg_state_x2"),
     (r"k_sweepz<8, 2, 1, 8, 4, -1>", "avrg_state_x1"),
     (r"k_zsmooth<2, 2, 1, [0-4][,>]", "avrg_state_x2_rk"),
