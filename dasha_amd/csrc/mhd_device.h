@@ -30,6 +30,8 @@ struct Geo {
     const double *xwp, *xwm, *ywp, *ywm, *zwp, *zwm;
     const double *xc, *yc, *zc;          // coordinates
     const double *dxs, *dys, *dzs;       // setdt spacings
+    const double *rdx, *rdy, *rdz;       // 1/|spacing| (rounded), for divc
+    const double *hdx, *hdy, *hdz;       // (eta + fmu) / spacing / 2 of setdt
     const double *cbx;                   // B-component smoothing coefficient per x
     const double *crad;                  // radial smoothing coefficient per point (0 off the interior)
     double ax3, bx3, cx3, ax4, bx4, cx4; // axp at jx = 3, 4 (x-left extrapolation)

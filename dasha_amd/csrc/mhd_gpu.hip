@@ -100,24 +100,28 @@ __global__ void k_dt_partial(Geo g, const double *__restrict__ X, double eta, do
         int i = (int)(t % ni); long r = t / ni;
         int j = j0 + (int)(r % nj); int k = (int)(r / nj);
         long q = IDX(g, i, j, k), S = g.S;
-        double x1 = X[q];
-        double vx = X[q + S] / x1, vy = X[q + 2 * S] / x1, vz = X[q + 3 * S] / x1;
+        // quotients with a shared or precomputed rounded reciprocal (divc:
+        // equal to the IEEE quotients); (eta + fmu) / dd / 2 is a per-index
+        // table computed the same way on the host
+        const double x1 = X[q], y1 = 1.0 / x1;
+        double vx = divc(X[q + S], x1, y1), vy = divc(X[q + 2 * S], x1, y1), vz = divc(X[q + 3 * S], x1, y1);
         double x5 = X[q + 4 * S], x6 = X[q + 5 * S], x7 = X[q + 6 * S];
         double b2 = x5 * x5 + x6 * x6 + x7 * x7;
-        double va2 = b2 / x1;
-        double cs2 = gam * X[q + 7 * S] / x1;
+        double va2 = divc(b2, x1, y1);
+        double cs2 = divc(gam * X[q + 7 * S], x1, y1);
         double vf = sqrt(fabs(va2 + cs2));
         double vpx = fabs(vx) + vf, vpy = fabs(vy) + vf, vpz = fabs(vz) + vf;
         double ddx = g.dxs[i], ddy = g.dys[j], ddz = g.dzs[k];
         if (wcoef != 0.0) {
-            double wb = wcoef * sqrt(b2) / x1;
-            vpx = vpx + wb / fabs(ddx);
-            vpy = vpy + wb / fabs(ddy);
-            vpz = vpz + wb / fabs(ddz);
+            double wb = divc(wcoef * sqrt(b2), x1, y1);
+            vpx = vpx + divc(wb, fabs(ddx), g.rdx[i]);
+            vpy = vpy + divc(wb, fabs(ddy), g.rdy[j]);
+            vpz = vpz + divc(wb, fabs(ddz), g.rdz[k]);
         }
-        double dtx = fabs(ddx) / (vpx / cfl + (eta + fmu) / ddx / 2.);
-        double dty = fabs(ddy) / (vpy / cfl + (eta + fmu) / ddy / 2.);
-        double dtz = fabs(ddz) / (vpz / cfl + (eta + fmu) / ddz / 2.);
+        const double rcfl = 1.0 / cfl;
+        double dtx = fabs(ddx) / (divc(vpx, cfl, rcfl) + g.hdx[i]);
+        double dty = fabs(ddy) / (divc(vpy, cfl, rcfl) + g.hdy[j]);
+        double dtz = fabs(ddz) / (divc(vpz, cfl, rcfl) + g.hdz[k]);
         double dt1 = dtx < dty ? dtx : dty;
         double dt2 = dtz < dt1 ? dtz : dt1;
         if (dt2 < best) best = dt2;
@@ -932,7 +936,13 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
         if (cfg->periodic_y) h->wy[j] = (j >= 2 && j <= NY - 3) ? Y[j + 1] - Y[j] : 0.0;
         else h->wy[j] = j == 0 ? 0.5 * (Y[1] - Y[0]) : j == NY - 1 ? 0.5 * (Y[j] - Y[j - 1]) : 0.5 * (Y[j + 1] - Y[j - 1]);
     }
-    size_t ngeo = cx.size() + cy.size() + cz.size() + 2 * (NX + NY + NZ) + NX + (NX + NY + NZ);
+    // setdt tables: 1/|d| and (eta + fmu) / d / 2 per index
+    std::vector<double> RDX(NX), RDY(NY), RDZ(NZ), HDX(NX), HDY(NY), HDZ(NZ);
+    const double em = cfg->eta0 + cfg->fmu0;
+    for (int i = 0; i < NX; i++) { RDX[i] = 1.0 / std::fabs(DXs[i]); HDX[i] = em / DXs[i] / 2.; }
+    for (int j = 0; j < NY; j++) { RDY[j] = 1.0 / std::fabs(DYs[j]); HDY[j] = em / DYs[j] / 2.; }
+    for (int k = 0; k < NZ; k++) { RDZ[k] = 1.0 / std::fabs(DZs[k]); HDZ[k] = em / DZs[k] / 2.; }
+    size_t ngeo = cx.size() + cy.size() + cz.size() + 2 * (NX + NY + NZ) + NX + (NX + NY + NZ) + 2 * (NX + NY + NZ);
     HCHK(hipMalloc(&h->geo, ngeo * sizeof(double)));
     std::vector<double> hg;
     hg.reserve(ngeo);
@@ -941,6 +951,7 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     size_t oX = push(X), oY = push(Y), oZ = push(Z), oDX = push(DXs), oDY = push(DYs), oDZ = push(DZs);
     size_t ocb = push(cbx);
     size_t owx = push(h->wx), owy = push(h->wy), owz = push(h->wz);
+    size_t orx = push(RDX), ory = push(RDY), orz = push(RDZ), ohx = push(HDX), ohy = push(HDY), ohz = push(HDZ);
     HCHK(hipMemcpy(h->geo, hg.data(), hg.size() * sizeof(double), hipMemcpyHostToDevice));
     Geo &g = h->g;
     g.NX = NX; g.NY = NY; g.NZ = NZ; g.sy = NX; g.sz = (long)NX * NY; g.S = h->S; g.periodic = cfg->periodic_y;
@@ -959,6 +970,8 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     g.zwp = g.z2d + 4 * NZ; g.zwm = g.zwp + NZ;
     g.xc = G0 + oX; g.yc = G0 + oY; g.zc = G0 + oZ;
     g.dxs = G0 + oDX; g.dys = G0 + oDY; g.dzs = G0 + oDZ;
+    g.rdx = G0 + orx; g.rdy = G0 + ory; g.rdz = G0 + orz;
+    g.hdx = G0 + ohx; g.hdy = G0 + ohy; g.hdz = G0 + ohz;
     g.cbx = G0 + ocb;
     h->dwx = h->geo + owx; h->dwy = h->geo + owy; h->dwz = h->geo + owz;
     coef_side(X.data(), 0, +1, &g.ax3, &g.bx3, &g.cx3);
