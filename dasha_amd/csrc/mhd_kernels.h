@@ -340,13 +340,24 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
     const int lane = threadIdx.x % TX;
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);   // uniform over the wavefront
     // the block's tile, chunk and field: from the grid position (ZM_ALL), or
-    // from the host's block table by dispatch position
+    // from the host's block table by dispatch position (work-groups go to
+    // XCD id % 8)
     int bx, by, chunk, m;
     if (MODE == ZM_ALL) {
-        bx = blockIdx.x;
-        by = blockIdx.y;
-        chunk = blockIdx.z % nch;
-        m = blockIdx.z / nch;
+        // the grid order (x tile fastest, then y tile, chunk, field) handed to
+        // the XCDs in contiguous ranges: neighbouring tiles of a field share
+        // halo rows in one L2
+        const unsigned gx = gridDim.x, gy = gridDim.y;
+        const unsigned T = gx * gy * gridDim.z;
+        const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+        const unsigned per = T / NXCD, rem = T % NXCD, xcd = L % NXCD;
+        unsigned V = xcd * per + (xcd < rem ? xcd : rem) + L / NXCD;
+        bx = __builtin_amdgcn_readfirstlane(V % gx);
+        V /= gx;
+        by = __builtin_amdgcn_readfirstlane(V % gy);
+        V /= gy;
+        chunk = __builtin_amdgcn_readfirstlane(V % (unsigned)nch);
+        m = __builtin_amdgcn_readfirstlane(V / (unsigned)nch);
     } else {
         const unsigned zw = ztab[blockIdx.x];
         bx = zw & ((1u << ZT_BY) - 1);
