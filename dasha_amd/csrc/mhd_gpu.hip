@@ -603,8 +603,9 @@ static void launch_zsmooth(mhd_handle *h, const double *a, double *b, const doub
 }
 
 // kk smoothing sweeps (+ fills) on a 3-field array; returns the result
-// buffer.  With addf, the result is the smoothed field plus addf (the last
-// sweep adds it; that sweep is a single pass)
+// buffer.  With addf, the result is the smoothed field plus addf, added by
+// the launch that runs the last pass (three passes for avrgh, else a
+// single-pass sweep)
 static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk, const double *addf = nullptr)
 {
     while (kk > 0) {
