@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # the ensemble runs each use a stream of their own: with 16 hardware queues
 # (HIP's default is 4) more of them execute at once (read at HIP start-up)
-os.environ["GPU_MAX_HW_QUEUES"] = "16"
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("MHD_HW_QUEUES", "16")
 
 HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
 B_ALG_HALL = 6256           # algorithmic bytes per cell per RK4 step (Hall on)
