@@ -12,12 +12,12 @@
 !   mhd<nst><000> restart record with the initial state (readin format)
 ! Outputs: fort.20 (dt, time, nstep per step, t3dsdsc.f:180-181),
 ! data/stepnm (nstep, time, dt, t3dsdsc.f:157-169), current.dat (diagn,
-! t3dsdsc.f:1228-1230), restart files mhd<nst><000> (recrd) and, at the
-! end of the run, the final state in the same format as mhdfinal.  Exit
-! status: 0 done, 2 dt collapse (t3dsdsc.f:142-154), 3 current blow-up
-! (t3dsdsc.f:1232), 1 any other error.  The reference's other snapshot
-! writers (recrd2/recrd3 plot files) are not part of the step and are not
-! written.
+! t3dsdsc.f:1228-1230), restart files mhd<nst><000> (recrd), the plot files
+! data/m2d<nst>.dat (recrd2: the y-midplane) and data/m3d<nst> (recrd3: the
+! volume) of rho, v = m/rho, B, p and J, and at the end of the run the final
+! state in the restart format as mhdfinal.  Exit status: 0 done, 2 dt
+! collapse (t3dsdsc.f:142-154), 3 current blow-up (t3dsdsc.f:1232), 1 any
+! other error.
 program mhd_driver
   use iso_c_binding
   use mhd_gpu
@@ -95,9 +95,9 @@ program mhd_driver
      call check(mhd_compute_dt(h, dt))
      write(*,*) 'main', nstep, time, dt
      if (dt < 0.1d0*dt0) then
-        ! t3dsdsc.f:142-154: snapshot and stop
+        ! t3dsdsc.f:142-154: midplane snapshot and stop
         nst = nst + nint
-        call recrd()
+        call recrd2()
         call mhd_destroy(h)
         stop 2
      end if
@@ -125,6 +125,8 @@ program mhd_driver
         nstp(nst) = nstep
         nst = nst + nint
         if (mod(nst, 3) == 1) call recrd()
+        call recrd2()
+        call recrd3()
      end if
      if (nstep > nstop) exit
      if (nst >= nend) exit
@@ -148,6 +150,78 @@ contains
     write(v) x
     close(v)
   end subroutine recrd
+
+  ! the eleven plot fields of physical point (jx, jy, jz) (1-based physical
+  ! indices): rho, vx, vy, vz, bx, by, bz, p, jx, jy, jz with v = m / rho
+  ! (t3dsdsc.f:5072-5076, 5331-5336)
+  subroutine plot_fields(cur, jx, jy, jz, f)
+    real(c_double), intent(in) :: cur(:,:,:,:)
+    integer, intent(in) :: jx, jy, jz
+    real(c_double), intent(out) :: f(11)
+    integer :: i
+    do i = 1, 8
+       f(i) = x(jx+2, jy, jz+2, i)
+    end do
+    do i = 2, 4
+       f(i) = f(i) / f(1)
+    end do
+    do i = 1, 3
+       f(8+i) = cur(jx+2, jy, jz+2, i)
+    end do
+  end subroutine plot_fields
+
+  subroutine recrd2()
+    ! t3dsdsc.f:4965-5164: the plane jy = my/2+1
+    real(c_double), allocatable :: cur(:,:,:,:), bl(:,:,:)
+    real(c_double) :: f(11)
+    integer :: v, jx, jz, i, nyh
+    allocate(cur(mx+4,my,mz+4,3), bl(mx,mz,11))
+    call check(mhd_get_state(h, x))
+    call check(mhd_get_current(h, cur))
+    nyh = my/2 + 1
+    do jz = 1, mz
+       do jx = 1, mx
+          call plot_fields(cur, jx, nyh, jz, f)
+          bl(jx,jz,:) = f
+       end do
+    end do
+    open(newunit=v, file='data/m2d'//cn(nst)//'.dat', status='replace', form='formatted')
+    write(v,*) 2, mx, mz
+    write(v,*) time
+    write(v,'(5(1x,e14.5E3))') xx(3:mx+2), zz(3:mz+2)
+    do i = 1, 11
+       write(v,*) i
+       write(v,'(5(1x,e14.5E3))') ((bl(jx,jz,i), jx=1,mx), jz=1,mz)
+    end do
+    close(v)
+  end subroutine recrd2
+
+  subroutine recrd3()
+    ! t3dsdsc.f:5167-5412: the volume, every y row
+    real(c_double), allocatable :: cur(:,:,:,:), bl(:,:,:,:)
+    real(c_double) :: f(11)
+    integer :: v, jx, jy, jz, i
+    allocate(cur(mx+4,my,mz+4,3), bl(mx,my,mz,11))
+    call check(mhd_get_state(h, x))
+    call check(mhd_get_current(h, cur))
+    do jz = 1, mz
+       do jy = 1, my
+          do jx = 1, mx
+             call plot_fields(cur, jx, jy, jz, f)
+             bl(jx,jy,jz,:) = f
+          end do
+       end do
+    end do
+    open(newunit=v, file='data/m3d'//cn(nst), status='replace', form='formatted')
+    write(v,*) 3, mx, my, mz
+    write(v,*) time
+    write(v,'(11(1x,e10.4))') xx(3:mx+2), yy, zz(3:mz+2)
+    do i = 1, 11
+       write(v,*) i
+       write(v,'(11(1x,e10.4))') (((bl(jx,jy,jz,i), jx=1,mx), jy=1,my), jz=1,mz)
+    end do
+    close(v)
+  end subroutine recrd3
 
   subroutine check(r)
     integer(c_int), intent(in) :: r

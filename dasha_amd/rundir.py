@@ -17,6 +17,7 @@ default).
 from __future__ import annotations
 
 import os
+import re
 import struct
 
 import numpy as np
@@ -77,3 +78,34 @@ def write_case(rundir: str, case, *, nstop: int, nst: int = 1, ncase: int = 1, d
         f.write(f" nstop={nstop}, nst={nst}, ncase={ncase}, dtime={dtime!r}, dnstep={dnstep}\n/\n")
     write_restart(os.path.join(rundir, "mhd" + cn(nst) + cn(0)), case.x0, ncase=ncase, nstep=0,
                   time=0.0, nst=nst)
+
+
+def read_plot(path: str):
+    """A recrd2 (m2d) or recrd3 (m3d) plot file (t3dsdsc.f:5117-5157,
+    5366-5401): returns (time, coordinate arrays, fields) with fields of shape
+    (11, mz, mx) or (11, mz, my, mx), C order: rho, vx, vy, vz, bx, by, bz, p,
+    jx, jy, jz."""
+    with open(path) as f:
+        tok = f.read().split()
+    # Ew.d drops the E of a three-digit exponent (0.1234-100)
+    fl = lambda t: float(re.sub(r"(?<=[0-9.])([+-])(?=[0-9]+$)", r"E\1", t))
+    ndim = int(tok[0])
+    dims = [int(t) for t in tok[1:1 + ndim]]
+    pos = 1 + ndim
+    time = fl(tok[pos])
+    pos += 1
+    coords = []
+    for n in dims:
+        coords.append(np.array([fl(t) for t in tok[pos:pos + n]]))
+        pos += n
+    npts = int(np.prod(dims))
+    shape = tuple(reversed(dims))
+    fields = np.empty((11,) + shape)
+    for i in range(11):
+        if int(tok[pos]) != i + 1:
+            raise ValueError(f"{path}: field label {tok[pos]} where {i + 1} was expected")
+        pos += 1
+        fields[i] = np.array([fl(t) for t in tok[pos:pos + npts]]).reshape(shape)
+        pos += npts
+    return time, coords, fields
+

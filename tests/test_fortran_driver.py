@@ -7,7 +7,8 @@ message (any format mismatch stops it earlier with a runtime I/O error).
 The GPU test runs the driver for 12 steps and requires its final state to
 be bit-identical to the same setdt/stepon sequence driven from Python
 through the same library, with the fort.20 and current.dat cadence of
-t3dsdsc.f:171-181 and 1174-1182.
+t3dsdsc.f:171-181 and 1174-1182; a second one checks the m2d/m3d plot
+files of an output step against the state and current at that step.
 """
 import os
 import subprocess
@@ -95,3 +96,38 @@ def test_driver_matches_python_loop(tmp_path):
     cur = [ln.split() for ln in (tmp_path / "current.dat").read_text().split("\n") if ln.strip()]
     assert len(cur) == 2                              # nstep 0 and 10
     assert all(len(c) == 7 for c in cur)
+
+
+@pytest.mark.gpu
+def test_driver_plot_files(tmp_path):
+    """recrd2 / recrd3 at an output step (dtime 0, dnstep 1: the cadence of
+    t3dsdsc.f:185-199 fires after the first step, nst 1 -> 2)"""
+    from dasha_amd.mhd import MHDStep
+    drv = _driver()
+    case = harris.case_small(24, 8, 24, seed=5)
+    rundir.write_case(str(tmp_path), case, nstop=2, dtime=0.0, dnstep=1)
+    print(f"[driver] plot files, case={case.name}", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "110", drv], cwd=str(tmp_path),
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    g = MHDStep(case)
+    t, _ = g.run(1)
+    x, cur = g.get_state(), g.get_current()
+    mx, mz = case.mx, case.mz
+    # physical points of the padded arrays [m][mz+4][my][mx+4]
+    phys = np.concatenate([x[:, 2:mz + 2, :, 2:mx + 2], cur[:, 2:mz + 2, :, 2:mx + 2]])
+    phys[1:4] /= phys[0]
+    t2, c2, f2 = rundir.read_plot(str(tmp_path / "data" / "m2d002.dat"))
+    assert t2 == pytest.approx(t, rel=1e-14)
+    # e14.5 keeps 5 significant digits (relative rounding <= 5e-5), e10.4
+    # keeps 4 (<= 5e-4)
+    r2, r3 = 5.01e-5, 5.01e-4
+    for got, want in ((c2[0], case.xx[2:mx + 2]), (c2[1], case.zz[2:mz + 2])):
+        assert np.allclose(got, want, rtol=r2, atol=0), (got[:6], want[:6])
+    nyh = case.my // 2                     # Fortran jy = my/2 + 1
+    assert np.allclose(f2, phys[:, :, nyh, :], rtol=r2, atol=0)
+    t3, c3, f3 = rundir.read_plot(str(tmp_path / "data" / "m3d002"))
+    assert len(c3) == 3 and np.allclose(c3[1], case.yy, rtol=r3, atol=0)
+    assert np.allclose(f3, phys, rtol=r3, atol=0)
+    assert not (tmp_path / ("mhd" + rundir.cn(2) + rundir.cn(0))).exists()   # recrd only when mod(nst,3) == 1
+
