@@ -960,3 +960,6 @@ void orc_energy(const orc *o, double *out)
     }
     out[0] = ek; out[1] = em; out[2] = et; out[3] = ep;
 }
+
+/* reverse mode (test infrastructure for the GPU adjoint) */
+#include "mhd_oracle_adj.c"

@@ -26,8 +26,8 @@ class OrcConfig(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(
-            os.path.join(HERE, "mhd_oracle.c")):
+    srcs = [os.path.join(HERE, f) for f in ("mhd_oracle.c", "mhd_oracle_adj.c")]
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(map(os.path.getmtime, srcs)):
         subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
 
@@ -61,6 +61,11 @@ def lib():
         L.orc_get_coeffs.argtypes = [P, C.c_int, D]
         L.orc_get_coeffs.restype = C.c_int
         L.orc_apply.argtypes = [P, C.c_int, D, C.c_int, C.c_int]
+        L.orc_adj_apply.argtypes = [P, C.c_int, D, C.c_int, C.c_int]
+        L.orc_adj_rhs.argtypes = [P, D, D, D, D, C.c_int, C.c_int]
+        L.orc_adj_step.argtypes = [P, C.c_double, C.c_double, D, D, D]
+        L.orc_adj_residual.argtypes = [P, D, D]
+        L.orc_adj_fold_y.argtypes = [P, D, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_max_threads.restype = C.c_int
         _lib = L
@@ -185,6 +190,44 @@ class Oracle:
         """0 bndry(xdif), 1 bndrytt(xm), 2 bndryc/E/h, 3 one smoothing sweep"""
         b = np.ascontiguousarray(a, dtype=np.float64).copy()
         lib().orc_apply(self.h, kind, _p(b), nf, coef_kind)
+        return b
+
+    # ---- reverse mode (oracle/mhd_oracle_adj.c) ----
+    def adj_apply(self, kind: int, a, nf: int = 3, coef_kind: int = 0):
+        """transpose of apply(kind, ...)"""
+        b = np.ascontiguousarray(a, dtype=np.float64).copy()
+        lib().orc_adj_apply(self.h, kind, _p(b), nf, coef_kind)
+        return b
+
+    def adj_rhs(self, abar, *, init: bool = False, smooth: bool = True):
+        """(xbar, rbar, etabar): vjp of rhs(t) (smooth = t != 0) or of the
+        residual capture (init) at the current state"""
+        ab = np.ascontiguousarray(abar, dtype=np.float64)
+        xb = np.zeros((8,) + self.shape3)
+        rb = np.zeros((3,) + self.shape3)
+        eb = np.zeros(1)
+        lib().orc_adj_rhs(self.h, _p(ab), _p(xb), _p(rb), _p(eb), int(init), int(smooth))
+        return xb, rb, float(eb[0])
+
+    def adj_step(self, time: float, dt: float, xbar, rbar, etabar: float = 0.0):
+        """advance one step (as step) and map xbar (adjoint of the new state)
+        to the adjoint of the old one; rbar accumulates in place"""
+        xb = np.ascontiguousarray(xbar, dtype=np.float64).copy()
+        assert rbar.dtype == np.float64 and rbar.flags.c_contiguous
+        eb = np.array([etabar])
+        lib().orc_adj_step(self.h, time, dt, _p(xb), _p(rbar), _p(eb))
+        return xb, float(eb[0])
+
+    def adj_residual(self, rbar, xbar):
+        """xbar += (d xresd / d x)^T rbar at the current state (in place)"""
+        rb = np.ascontiguousarray(rbar, dtype=np.float64)
+        assert xbar.dtype == np.float64 and xbar.flags.c_contiguous
+        lib().orc_adj_residual(self.h, _p(rb), _p(xbar))
+
+    def fold_y(self, a, nf: int):
+        """adjoint of the periodic image rows folded onto the rows they copy"""
+        b = np.ascontiguousarray(a, dtype=np.float64).copy()
+        lib().orc_adj_fold_y(self.h, _p(b), nf)
         return b
 
     def run(self, steps: int, *, time: float = 0.0, init: bool = True):
