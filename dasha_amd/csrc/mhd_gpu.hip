@@ -21,6 +21,7 @@
 
 #include "mhd_gpu.h"
 #include "mhd_kernels.h"
+#include "mhd_adjoint.h"
 
 using namespace mhd;
 
@@ -353,6 +354,10 @@ struct mhd_handle {
     ZTab zt[2][2];                         // their block tables: [3 or 8 fields][without / with face rules]
     ZTab zt3[2];                           // the same for three passes on 3 fields
     int zs3 = 1;                           // last three avrgh passes (+ E) in one launch
+    // stage recording of the adjoint's forward replay: when rec_x is set,
+    // right() copies its state and current into slot rec_n and advances it
+    double *rec_x = nullptr, *rec_c = nullptr;
+    int rec_n = 0;
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -369,6 +374,9 @@ struct mhd_handle {
             return MHD_ERR_HIP;                                                    \
         }                                                                          \
     } while (0)
+
+// launch-time errors of void helpers surface at the next synchronising call
+#define HCHK_V(call) (void)(call)
 
 namespace {
 
@@ -654,6 +662,13 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
     double eta = init ? 0.0 : c.eta0, fmu = init ? 0.0 : c.fmu0;
     double frho = init ? 0.0 : c.frho0, fpr = init ? 0.0 : c.fpr0;
     current(h, X);
+    if (h->rec_x) {
+        (void)hipMemcpyAsync(h->rec_x + (long)h->rec_n * 8 * h->S, X, 8 * h->S * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(h->rec_c + (long)h->rec_n * 3 * h->S, h->cur, 3 * h->S * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->stream);
+        h->rec_n++;
+    }
     PROF_BEGIN(PT_RHSA);
     hipLaunchKernelGGL(k_rhs_a, tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->eq + 4 * h->S, h->cur, h->xresd,
                        eta, c.hall, c.di, h->xdA, h->eA, h->hA, h->kc);
@@ -824,7 +839,7 @@ static int upload_fields(mhd_handle *h, double *dst, const double *src, int nf)
     return 0;
 }
 
-static int download_fields(mhd_handle *h, double *dst, const double *src, int nf)
+static int download_fields(mhd_handle *h, double *dst, const double *src, int nf, bool images = true)
 {
     const long px = h->cfg.mx + 4, plane = px * h->NY;
     const long total = plane * (h->cfg.mz + 4);
@@ -849,7 +864,7 @@ static int download_fields(mhd_handle *h, double *dst, const double *src, int nf
                 }
         }
     }
-    if (h->cfg.periodic_y) {
+    if (h->cfg.periodic_y && images) {
         // image rows (never computed on the device): 0,1,NY-2,NY-1 <- NY-4,NY-3,2,3
         int NY = h->NY;
         const int dstr[4] = {0, 1, NY - 2, NY - 1}, srcr[4] = {NY - 4, NY - 3, 2, 3};
@@ -861,6 +876,134 @@ static int download_fields(mhd_handle *h, double *dst, const double *src, int nf
                 }
     }
     return 0;
+}
+
+// ===========================================================================
+// reverse mode: discrete adjoint of the step (kernels in mhd_adjoint.h)
+// ===========================================================================
+struct AdjBufs {
+    double *ab = nullptr, *t8 = nullptr, *xb = nullptr, *xbm = nullptr, *xbf = nullptr, *tt = nullptr;  // 8 fields
+    double *eb = nullptr, *hb = nullptr, *jb = nullptr, *vb = nullptr, *t3 = nullptr, *rb = nullptr;    // 3 fields
+    double *xfin = nullptr;                // final state (8 fields)
+    double *acc = nullptr;                 // per-block slots: adjoint of eta, then objective
+    long nblk = 0;
+    double *ck = nullptr;                  // checkpoints, 8 fields each
+    double *tx = nullptr, *tc = nullptr;   // tape: stage states (8) and currents (3), 4 per step
+};
+
+static void adj_free(AdjBufs &A)
+{
+    double *p[] = {A.ab, A.t8, A.xb, A.xbm, A.xbf, A.tt, A.eb, A.hb, A.jb, A.vb, A.t3, A.rb, A.xfin, A.acc,
+                   A.ck, A.tx, A.tc};
+    for (double *q : p)
+        if (q) (void)hipFree(q);
+    A = AdjBufs();
+}
+
+static dim3 adj_grid(const mhd_handle *h)
+{
+    return dim3((unsigned)blocks(h->NX, 64), (unsigned)blocks(h->g.yhi - h->g.ylo + 1, 4), (unsigned)h->NZ);
+}
+
+static int adj_alloc(mhd_handle *h, AdjBufs &A, int nck, int K)
+{
+    const size_t fb = (size_t)h->S * sizeof(double);
+    const dim3 gr = adj_grid(h);
+    A.nblk = (long)gr.x * gr.y * gr.z;
+    struct Want {
+        double **p;
+        size_t bytes;
+    } w[] = {{&A.ab, 8 * fb}, {&A.t8, 8 * fb}, {&A.xb, 8 * fb}, {&A.xbm, 8 * fb}, {&A.xbf, 8 * fb},
+             {&A.tt, 8 * fb}, {&A.xfin, 8 * fb}, {&A.eb, 3 * fb}, {&A.hb, 3 * fb}, {&A.jb, 3 * fb},
+             {&A.vb, 3 * fb}, {&A.t3, 3 * fb}, {&A.rb, 3 * fb}, {&A.acc, 2 * A.nblk * sizeof(double)},
+             {&A.ck, (size_t)nck * 8 * fb}, {&A.tx, (size_t)K * 4 * 8 * fb}, {&A.tc, (size_t)K * 4 * 3 * fb}};
+    for (auto &x : w) {
+        if (x.bytes == 0) continue;
+        if (hipMalloc(x.p, x.bytes) != hipSuccess) {
+            set_err(h, "adjoint: device allocation failed");
+            return MHD_ERR_NOMEM;
+        }
+        HCHK(hipMemsetAsync(*x.p, 0, x.bytes, h->stream));
+    }
+    return 0;
+}
+
+// the two accumulated sums (adjoint of eta, objective), in a fixed order
+static int adj_sums(mhd_handle *h, const AdjBufs &A, double *eta_bar, double *obj)
+{
+    std::vector<double> v(2 * A.nblk);
+    HCHK(hipMemcpyAsync(v.data(), A.acc, v.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HCHK(hipStreamSynchronize(h->stream));
+    HCHK(hipGetLastError());
+    double e = 0.0, o = 0.0;
+    for (long b = 0; b < A.nblk; b++) { e += v[b]; o += v[A.nblk + b]; }
+    if (eta_bar) *eta_bar = e;
+    if (obj) *obj = o;
+    return 0;
+}
+
+// transposed fill: z rules, then x rules (the y images are implicit)
+static void adj_bc(mhd_handle *h, double *a, int nf, int type)
+{
+    const Geo &g = h->g;
+    const long nz = (long)g.NX * g.nyu * nf, nx = (long)g.nyu * g.NZ * nf;
+    hipLaunchKernelGGL(k_adj_bc_z, dim3(blocks(nz, 256)), dim3(256), 0, h->stream, g, a, nf, type == BC_TT ? 2 : 0);
+    hipLaunchKernelGGL(k_adj_bc_x, dim3(blocks(nx, 256)), dim3(256), 0, h->stream, g, a, nf, type == BC_STATE ? 0 : 1);
+}
+
+// kk transposed passes of a smoothing routine (each: fill^T, then sweep^T),
+// ping-ponging a and tmp; on return a holds the result, tmp is free
+template <int KIND>
+static void adj_smooth(mhd_handle *h, double *&a, double *&tmp, int nf, int kk, int bctype)
+{
+    for (int p = 0; p < kk; p++) {
+        adj_bc(h, a, nf, bctype);
+        hipLaunchKernelGGL(k_adj_filter<KIND>, adj_grid(h), dim3(64, 4), 0, h->stream, h->g, a, tmp, nf);
+        std::swap(a, tmp);
+    }
+}
+
+// reverse mode of right() at state X with current C: ab = adjoint of xdif
+// (destroyed), xb += (dright/dX)^T ab, rb -= part on xresd (if rb),
+// acc[0] += part on eta.  init: the residual capture (eta = 0, no avrg).
+static void adj_rhs(mhd_handle *h, AdjBufs &A, const double *X, const double *C, double *ab, double *xb,
+                    double *rb, bool init, bool smooth)
+{
+    const mhd_config &c = h->cfg;
+    const Geo &g = h->g;
+    const double eta = init ? 0.0 : c.eta0;
+    const dim3 grid = adj_grid(h), blk(64, 4);
+    double *a = ab, *t8 = A.t8;
+    if (smooth) adj_smooth<KIND_STATE>(h, a, t8, 8, 2, BC_STATE);   // avrg x2 (1669)
+    adj_bc(h, a, 8, BC_STATE);                                         // bndry (1665)
+    hipLaunchKernelGGL(k_adj_curle_p, grid, blk, 0, h->stream, g, a, X, C, c.gamma, eta, A.eb, A.vb, A.jb, xb,
+                       A.acc);
+    if (c.hall) {
+        HCHK_V(hipMemcpyAsync(A.hb, A.eb, 3 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        adj_smooth<KIND_EIGHTH>(h, A.hb, A.t3, 3, 5, BC_AUX);         // avrgh x5 (1424)
+        adj_bc(h, A.hb, 3, BC_AUX);                                    // bndryh (1423)
+        hipLaunchKernelGGL(k_adj_hall, grid, blk, 0, h->stream, g, A.hb, X, C, c.di, A.jb, xb);
+    }
+    adj_smooth<KIND_RADIAL>(h, A.eb, A.t3, 3, 4, BC_AUX);              // avrgE x4 (1389)
+    hipLaunchKernelGGL(k_adj_ext_flux, grid, blk, 0, h->stream, g, a, A.eb, A.vb, X, C, eta, xb, A.jb, rb, A.acc);
+    adj_smooth<KIND_RADIAL>(h, A.jb, A.t3, 3, 4, BC_AUX);              // avrgc x4 (2495)
+    adj_bc(h, A.jb, 3, BC_AUX);                                        // bndryc (2493)
+    hipLaunchKernelGGL(k_adj_curl, grid, blk, 0, h->stream, g, A.jb, xb);
+}
+
+// reverse mode of one step from tape slot `slot` (stage states and
+// currents of that step): A.xb = adjoint of x_{n+1} on entry, of x_n on exit
+static void adj_step(mhd_handle *h, AdjBufs &A, int slot, double dt)
+{
+    const long n8 = 8 * h->S, s8 = 8 * h->S, s3 = 3 * h->S;
+    const double *TX = A.tx + (long)slot * 4 * s8, *TC = A.tc + (long)slot * 4 * s3;
+    adj_smooth<KIND_STATE>(h, A.xb, A.t8, 8, 2, BC_TT);                // avrgtt x2 + bndrytt (1150)
+    const dim3 eg(blocks(n8, 256)), eb(256);
+    for (int s = 3; s >= 0; s--) {
+        hipLaunchKernelGGL(k_adj_rk, eg, eb, 0, h->stream, n8, s, dt, A.xb, A.xbm, A.xbf, A.ab, A.tt);
+        adj_rhs(h, A, TX + s * s8, TC + s * s3, A.ab, s == 0 ? A.xbf : A.tt, A.rb, false, true);
+    }
+    std::swap(A.xb, A.xbf);
 }
 
 // ===========================================================================
@@ -1162,6 +1305,108 @@ int mhd_get_current(mhd_handle *h, double *cur)
     if (!h || !cur) return MHD_ERR_ARG;
     current(h, h->x);
     return download_fields(h, cur, h->cur, 3);
+}
+
+int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective, int every, int flags,
+                    double *value, double *grad_x, double *grad_eta, double *ms)
+{
+    if (!h || nsteps < 1 || !dts || objective < 0 || objective > 1 || every < 0) return MHD_ERR_ARG;
+    if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
+    const mhd_config &c = h->cfg;
+    if (!c.periodic_y) { set_err(h, "adjoint: periodic y only"); return MHD_ERR_ARG; }
+    if (c.fmu0 != 0.0 || c.frho0 != 0.0 || c.fpr0 != 0.0) {
+        set_err(h, "adjoint: viscosity / diffusion terms are not differentiated");
+        return MHD_ERR_ARG;
+    }
+    for (int n = 0; n < nsteps; n++)
+        if (!(dts[n] > 0.0)) { set_err(h, "adjoint: dt must be positive"); return MHD_ERR_ARG; }
+    const int K = every > 0 ? std::min(every, nsteps) : (int)std::ceil(std::sqrt((double)nsteps));
+    const int nck = (nsteps + K - 1) / K;
+    // the two planes nearest z = 0 (the reconnection line of the sheet)
+    int k0 = -1;
+    {
+        std::vector<double> Z(h->NZ);
+        HCHK(hipMemcpy(Z.data(), h->g.zc, h->NZ * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k + 1 < h->NZ; k++)
+            if (Z[k] < 0.0 && Z[k + 1] >= 0.0) k0 = k;
+        if (objective == 1 && k0 < 0) { set_err(h, "adjoint: z = 0 is not inside the mesh"); return MHD_ERR_ARG; }
+    }
+    AdjBufs A;
+    int r = adj_alloc(h, A, nck, K);
+    if (r) { adj_free(A); return r; }
+    const size_t b8 = 8 * h->S * sizeof(double);
+    hipEvent_t e[3];
+    for (auto &x : e) HCHK(hipEventCreate(&x));
+    HCHK(hipEventRecord(e[0], h->stream));
+    // forward pass, keeping every K-th state
+    for (int n = 0; n < nsteps; n++) {
+        if (n % K == 0)
+            HCHK(hipMemcpyAsync(A.ck + (long)(n / K) * 8 * h->S, h->x, b8, hipMemcpyDeviceToDevice, h->stream));
+        hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, h->stream, h->dtv, dts[n]);
+        step_body(h);
+    }
+    HCHK(hipEventRecord(e[1], h->stream));
+    HCHK(hipMemcpyAsync(A.xfin, h->x, b8, hipMemcpyDeviceToDevice, h->stream));
+    hipLaunchKernelGGL(k_adj_objective, adj_grid(h), dim3(64, 4), 0, h->stream, h->g, h->x, objective, k0,
+                       h->dwx, h->dwy, h->dwz, A.xb, A.acc + A.nblk);
+    // reverse sweep, one checkpoint segment at a time: replay the segment
+    // recording stage states and currents, then run its steps backwards
+    for (int ci = nck - 1; ci >= 0; ci--) {
+        const int n0 = ci * K, n1 = std::min(nsteps, n0 + K);
+        HCHK(hipMemcpyAsync(h->x, A.ck + (long)ci * 8 * h->S, b8, hipMemcpyDeviceToDevice, h->stream));
+        for (int n = n0; n < n1; n++) {
+            h->rec_x = A.tx + (long)(n - n0) * 4 * 8 * h->S;
+            h->rec_c = A.tc + (long)(n - n0) * 4 * 3 * h->S;
+            h->rec_n = 0;
+            hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, h->stream, h->dtv, dts[n]);
+            step_body(h);
+        }
+        h->rec_x = h->rec_c = nullptr;
+        for (int n = n1 - 1; n >= n0; n--) adj_step(h, A, n - n0, dts[n]);
+    }
+    // the residual captured from x_0 (initia, t3dsdsc.f:816-834)
+    if (flags & MHD_ADJ_RESIDUAL) {
+        HCHK(hipMemcpyAsync(h->x, A.ck, b8, hipMemcpyDeviceToDevice, h->stream));
+        current(h, h->x);
+        HCHK(hipMemsetAsync(A.ab, 0, b8, h->stream));
+        HCHK(hipMemcpyAsync(A.ab + h->S, A.rb, 3 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        adj_rhs(h, A, h->x, h->cur, A.ab, A.xb, nullptr, true, false);
+    }
+    HCHK(hipEventRecord(e[2], h->stream));
+    HCHK(hipMemcpyAsync(h->x, A.xfin, b8, hipMemcpyDeviceToDevice, h->stream));
+    r = adj_sums(h, A, grad_eta, value);
+    if (r) { adj_free(A); return r; }
+    float f0 = 0, f1 = 0;
+    HCHK(hipEventElapsedTime(&f0, e[0], e[1]));
+    HCHK(hipEventElapsedTime(&f1, e[1], e[2]));
+    for (auto &x : e) (void)hipEventDestroy(x);
+    if (ms) { ms[0] = f0; ms[1] = f1; }
+    r = grad_x ? download_fields(h, grad_x, A.xb, 8, false) : 0;
+    adj_free(A);
+    return r;
+}
+
+int mhd_adjoint_rhs(mhd_handle *h, const double *xdif_bar, double *x_bar, double *xresd_bar, double *eta_bar,
+                    int init, int smooth)
+{
+    if (!h || !xdif_bar || !x_bar) return MHD_ERR_ARG;
+    const mhd_config &c = h->cfg;
+    if (!c.periodic_y || c.fmu0 != 0.0 || c.frho0 != 0.0 || c.fpr0 != 0.0) {
+        set_err(h, "adjoint: periodic y and zero viscosity/diffusion only");
+        return MHD_ERR_ARG;
+    }
+    AdjBufs A;
+    int r = adj_alloc(h, A, 0, 0);
+    if (!r) r = upload_fields(h, A.ab, xdif_bar, 8);
+    if (!r) {
+        current(h, h->x);
+        adj_rhs(h, A, h->x, h->cur, A.ab, A.xb, A.rb, init != 0, smooth != 0);
+        r = adj_sums(h, A, eta_bar, nullptr);
+    }
+    if (!r) r = download_fields(h, x_bar, A.xb, 8, false);
+    if (!r && xresd_bar) r = download_fields(h, xresd_bar, A.rb, 3, false);
+    adj_free(A);
+    return r;
 }
 
 int mhd_get_divb(mhd_handle *h, double *out)

@@ -27,7 +27,11 @@ EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state"
            "mhd_diagnostics", "mhd_get_divb", "mhd_rhs", "mhd_get_xdif", "mhd_get_residual",
            "mhd_set_residual", "mhd_set_profiling", "mhd_kernel_times", "mhd_copy_rate",
            "mhd_last_error", "mhd_destroy", "mhd_ensemble_create", "mhd_ensemble_advance",
-           "mhd_ensemble_status", "mhd_ensemble_error", "mhd_ensemble_destroy")
+           "mhd_ensemble_status", "mhd_ensemble_error", "mhd_ensemble_destroy",
+           "mhd_adjoint_run", "mhd_adjoint_rhs")
+
+MHD_ADJ_RESIDUAL = 1
+OBJECTIVES = {"magnetic_energy": 0, "reconnected_flux": 1}
 
 
 class MHDConfig(C.Structure):
@@ -87,6 +91,8 @@ def load_library(path: str = LIBPATH):
     L.mhd_ensemble_error.argtypes = [P]
     L.mhd_ensemble_error.restype = C.c_char_p
     L.mhd_ensemble_destroy.argtypes = [P]
+    L.mhd_adjoint_run.argtypes = [P, I, D, I, I, I, D, D, D, D]
+    L.mhd_adjoint_rhs.argtypes = [P, D, D, D, D, I, I]
     _lib = L
     return L
 
@@ -216,6 +222,33 @@ class MHDStep:
         calls = (C.c_int * n)()
         k = self.L.mhd_kernel_times(self.h, n, names, ms, calls)
         return {names[i].decode(): (ms[i], calls[i]) for i in range(k)}
+
+    # ---- reverse mode ----
+    def adjoint_rhs(self, xdif_bar, *, init: bool = False, smooth: bool = True):
+        """(x_bar, xresd_bar, eta_bar): vector-Jacobian product of one
+        right(t) at the current state (smooth = t != 0; init: the residual
+        capture)"""
+        ab = np.ascontiguousarray(xdif_bar, dtype=np.float64)
+        assert ab.shape == (8,) + self.shape3
+        xb = np.empty((8,) + self.shape3)
+        rb = np.empty((3,) + self.shape3)
+        eb = C.c_double()
+        self._check(self.L.mhd_adjoint_rhs(self.h, _p(ab), _p(xb), _p(rb), C.byref(eb), int(init), int(smooth)))
+        return xb, rb, eb.value
+
+    def adjoint_run(self, dts, objective="magnetic_energy", *, every: int = 0, residual: bool = True):
+        """Discrete adjoint of len(dts) steps from the current state with the
+        frozen dt sequence: returns (value, d value/d x0, d value/d eta0,
+        (forward ms, reverse ms)); the state ends at x_N."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        obj = OBJECTIVES[objective] if isinstance(objective, str) else int(objective)
+        gx = np.empty((8,) + self.shape3)
+        val, ge = C.c_double(), C.c_double()
+        ms = (C.c_double * 2)()
+        self._check(self.L.mhd_adjoint_run(self.h, int(d.size), _p(d), obj, int(every),
+                                           MHD_ADJ_RESIDUAL if residual else 0, C.byref(val), _p(gx),
+                                           C.byref(ge), ms))
+        return val.value, gx, ge.value, (ms[0], ms[1])
 
     def run(self, steps: int, *, time: float = 0.0, init: bool = True):
         """host loop mirroring t3dsdsc.f:136-176 (setdt, stepon, time += dt)"""

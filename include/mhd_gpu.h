@@ -139,6 +139,37 @@ int mhd_ensemble_status(const mhd_ensemble *e, int *status);
 const char *mhd_ensemble_error(const mhd_ensemble *e);
 void mhd_ensemble_destroy(mhd_ensemble *e);
 
+/* Discrete adjoint of the step (no counterpart in the reference, which has
+ * no derivatives; the reverse mode of stepon/right, t3dsdsc.f:1063-1673).
+ * From the current state x_0 (with its residual already captured), runs
+ * nsteps steps with the frozen dt sequence dts[nsteps] (setdt is not
+ * differentiated), evaluates the objective at x_N and returns
+ *   *value    objective 0: magnetic energy 1/2 sum |B|^2 dV (trapezoid
+ *             weights over unique physical points); objective 1:
+ *             reconnected flux 1/2 sum_x |Bz~| dx averaged over y, Bz~ the
+ *             mean of the two z planes nearest z = 0;
+ *   grad_x    d value / d x_0, (mx+4, my, mz+4, 8), image rows 0 (the
+ *             sensitivity of a periodic row is carried by its unique copy);
+ *   *grad_eta d value / d eta0;
+ *   ms[2]     HIP-event times of the forward pass and of the reverse sweep
+ *             (recomputation included).
+ * Reverse sweep: states are kept every `every` steps (0: ceil(sqrt(nsteps)));
+ * each segment is replayed recording stage states and currents, then
+ * traversed backwards.  flags & MHD_ADJ_RESIDUAL: x_0's residual capture
+ * (mhd_init_residual) is differentiated too; otherwise xresd is a constant.
+ * Periodic y and zero viscosity/diffusion only.  Any of value, grad_x,
+ * grad_eta, ms may be NULL.  On return the state is x_N. */
+enum { MHD_ADJ_RESIDUAL = 1 };
+/* stage-level access for parity checks of the adjoint: the vector-Jacobian
+ * product of one right(t) at the current state (smooth = t != 0; init: the
+ * residual capture, eta = 0, no final smoothing): given xdif_bar (8 fields)
+ * returns x_bar (8 fields), xresd_bar (3 fields, may be NULL) and *eta_bar
+ * (may be NULL), host layout, image rows 0. */
+int mhd_adjoint_rhs(mhd_handle *h, const double *xdif_bar, double *x_bar, double *xresd_bar, double *eta_bar,
+                    int init, int smooth);
+int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective, int every, int flags,
+                    double *value, double *grad_x, double *grad_eta, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -129,3 +129,97 @@ def test_step_adjoint_central_difference(small, nsteps):
         o2.close()
     fde = (vals[0] - vals[1]) / (2 * e)
     assert abs(fde - eb) <= 1e-6 * abs(eb)
+
+
+# ------------------------------------------------ control map and objectives
+def test_control_map_transposes():
+    from dasha_amd import adjoint
+    rng = np.random.default_rng(3)
+    c = np.cumsum(rng.uniform(0.1, 0.5, 17))
+    for axis in (0, 1):
+        shp = (17, 5) if axis == 0 else (5, 17)
+        f = rng.standard_normal(shp)
+        g = rng.standard_normal(shp)
+        lhs = np.vdot(adjoint.mesh_derivative(f, c, axis), g)
+        rhs = np.vdot(f, adjoint.mesh_derivative_t(g, c, axis))
+        assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    # the central quotient of a quadratic is exact on a uniform mesh
+    u = np.linspace(-1, 1, 11)
+    d = adjoint.mesh_derivative((u ** 2)[:, None] * np.ones((1, 3)), u, 0)
+    assert np.allclose(d[1:-1, 0], 2 * u[1:-1], atol=1e-13)
+    ac = adjoint.small_case(16, 8, 16)
+    ay = rng.standard_normal((16, 16))
+    gx = rng.standard_normal((8,) + ac.case.shape3)
+    dx0 = ac.state(ay) - ac.state(np.zeros_like(ay))
+    assert abs(np.vdot(dx0, gx) - np.vdot(ay, adjoint.ay_gradient(ac.case, gx))) <= 1e-11 * abs(np.vdot(dx0, gx))
+
+
+@pytest.mark.parametrize("obj", ["magnetic_energy", "reconnected_flux"])
+def test_objective_seed_central_difference(obj):
+    from dasha_amd import adjoint
+    ac = adjoint.small_case(16, 8, 16)
+    x = ac.state()
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal(x.shape) * 1e-3
+    s = adjoint.objective_seed(ac.case, x, obj)
+    eps = 1e-3
+    fd = (adjoint.objective_of_state(ac.case, x + eps * v, obj) -
+          adjoint.objective_of_state(ac.case, x - eps * v, obj)) / (2 * eps)
+    assert abs(fd - np.vdot(s, v)) <= 1e-9 * abs(fd)
+
+
+def oracle_adjoint(ac, dts, obj, ay=None):
+    """(value, d value / d x0 folded onto unique rows, d value / d eta) of the
+    CPU oracle: forward run, objective seed, reverse steps, residual vjp"""
+    from dasha_amd import adjoint
+    o = orc.Oracle(ac.case)
+    x0 = ac.state(ay)
+    o.set_state(x0)
+    o.init_residual()
+    states, ts, t = [], [], 0.0
+    for dt in dts:
+        states.append(o.get_state())
+        ts.append(t)
+        o.step(t, dt)
+        t += dt
+    xN = o.get_state()
+    val = adjoint.objective_of_state(ac.case, xN, obj)
+    xb = adjoint.objective_seed(ac.case, xN, obj)
+    rb = np.zeros((3,) + o.shape3)
+    eb = 0.0
+    for n in reversed(range(len(dts))):
+        o.set_state(states[n])
+        xb, eb = o.adj_step(ts[n], dts[n], xb, rb, eb)
+    o.set_state(x0)
+    o.adj_residual(rb, xb)
+    xb = o.fold_y(xb, 8)
+    o.close()
+    return val, xb, eb
+
+
+@pytest.mark.parametrize("obj", ["magnetic_energy", "reconnected_flux"])
+def test_oracle_ay_gradient_central_difference(obj):
+    """the whole chain on the CPU: control map, steps, objective"""
+    from dasha_amd import adjoint
+    ac = adjoint.small_case(16, 8, 16)
+    dts = [0.0056, 0.0055]
+    val, xb, eb = oracle_adjoint(ac, dts, obj)
+    gay = adjoint.ay_gradient(ac.case, xb)
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal(ac.ay0.shape) * 0.01
+
+    def J(ay):
+        o = orc.Oracle(ac.case)
+        o.set_state(ac.state(ay))
+        o.init_residual()
+        t = 0.0
+        for dt in dts:
+            o.step(t, dt)
+            t += dt
+        r = adjoint.objective_of_state(ac.case, o.get_state(), obj)
+        o.close()
+        return r
+
+    eps = 1e-3
+    fd = (J(ac.ay0 + eps * v) - J(ac.ay0 - eps * v)) / (2 * eps)
+    assert abs(fd - np.vdot(gay, v)) <= 1e-7 * abs(fd)
