@@ -14,6 +14,8 @@ Prints ONE JSON line:
   cpu_baseline   the CPU oracle (C restatement, OpenMP) on a bounded sample
   C3             the same timed loop on 512x256x512
   ensemble       48 uneven Harris runs advanced together vs one after another
+  adjoint        discrete adjoint of 500 steps at 128x64x128 (sqrt(N)
+                 checkpointing): reverse sweep incl. recomputation / forward
 """
 from __future__ import annotations
 
@@ -152,6 +154,28 @@ def time_ensemble(steps: int, nruns: int = 48):
             "speedup": seq_ms / ens_ms, "ms": ens_ms, "sequential_ms": seq_ms}
 
 
+def time_adjoint(steps: int = 500):
+    """gradient of the magnetic energy at T after `steps` steps of the
+    128x64x128 stretched Harris sheet (eta 0.005, d_i 0.15) with respect to
+    the initial dAy and eta: forward pass (states kept every ceil(sqrt(N))
+    steps) and reverse sweep (segment replays + transposed kernels), HIP
+    events on the library stream"""
+    from dasha_amd import adjoint
+    t0 = time.perf_counter()
+    ac = adjoint.case_c5(steps=steps)
+    dts = adjoint.dt_sequence(ac)
+    val, gay, geta, (fwd_ms, rev_ms) = adjoint.objective(ac, dts, obj="magnetic_energy")
+    print(f"[bench] adjoint: forward {fwd_ms:.1f} ms, reverse {rev_ms:.1f} ms "
+          f"(wall {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    return {"workload": f"{ac.case.name} stretched Harris sheet, eta {ac.case.eta0}, d_i {ac.case.di}, "
+                        f"{steps} RK4 steps, objective magnetic energy at T, controls dAy (mz x mx) and eta",
+            "steps": steps, "checkpoint_every": adjoint.sqrt_checkpoints(steps),
+            "forward_ms": fwd_ms, "reverse_ms": rev_ms,
+            "forward_ms_per_step": fwd_ms / steps, "reverse_ms_per_step": rev_ms / steps,
+            "ratio_reverse_to_forward": rev_ms / fwd_ms, "objective": val, "grad_eta": geta,
+            "grad_ay_norm": float((gay ** 2).sum() ** 0.5)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +191,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble timing")
     ap.add_argument("--ensemble-only", action="store_true", help="time the batched ensemble alone")
     ap.add_argument("--ensemble-runs", type=int, default=48)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete adjoint timing")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
@@ -256,6 +282,8 @@ def main():
         out["C3"] = time_config("C3", 10, 2)
     if args.config == "C2" and not args.no_ensemble:
         out["ensemble"] = time_ensemble(10, args.ensemble_runs)
+    if args.config == "C2" and not args.no_adjoint:
+        out["adjoint"] = time_adjoint(args.adjoint_steps)
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_steps, threads)

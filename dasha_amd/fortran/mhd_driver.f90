@@ -18,15 +18,20 @@
 ! state in the restart format as mhdfinal.  Exit status: 0 done, 2 dt
 ! collapse (t3dsdsc.f:142-154), 3 current blow-up (t3dsdsc.f:1232), 1 any
 ! other error.
+! With nadj > 0 in the namelist the program instead differentiates nadj
+! steps from the initial state: it records the dt sequence of a plain run
+! (setdt per step), restores the initial state and calls mhd_adjoint_run
+! (objective adjobj: 0 magnetic energy, 1 reconnected flux), writing
+! adjoint.bin = (value, d value/d eta0, d value/d x(mx+4,my,mz+4,8)).
 program mhd_driver
   use iso_c_binding
   use mhd_gpu
   implicit none
-  integer :: mx, my, mz, nstop, nst, ncase, dnstep, nend, nint
+  integer :: mx, my, mz, nstop, nst, ncase, dnstep, nend, nint, nadj, adjobj
   logical :: periody, hall, whistler
   real(c_double) :: di, eta0, fmu0, frho0, fpr0, gamma, cfl, dtime
   namelist /run/ mx, my, mz, periody, hall, whistler, di, eta0, fmu0, frho0, fpr0, gamma, cfl, &
-       nstop, nst, ncase, dtime, dnstep, nend, nint
+       nstop, nst, ncase, dtime, dnstep, nend, nint, nadj, adjobj
   type(mhd_config) :: cfg
   type(mhd_diag) :: dg
   type(c_ptr) :: h
@@ -42,6 +47,7 @@ program mhd_driver
   periody = .true.; hall = .true.; whistler = .true.
   di = 0; eta0 = 0; fmu0 = 0; frho0 = 0; fpr0 = 0; gamma = 5.d0/3.d0; cfl = 0.4d0
   nstop = 10; nst = 1; ncase = 1; dtime = 1.d0; dnstep = 10; nend = 5011; nint = 1
+  nadj = 0; adjobj = 0
   nstp = 0
   open(newunit=u, file='mhd.nml', status='old', action='read', iostat=ios)
   if (ios /= 0) call fail('cannot open mhd.nml', 1)
@@ -87,6 +93,11 @@ program mhd_driver
   dt0 = dt
   timein = time
   nstin = nst
+  if (nadj > 0) then
+     call adjoint_run()
+     call mhd_destroy(h)
+     stop
+  end if
   inquire(file='data/.', exist=exist)
   if (.not. exist) call execute_command_line('mkdir -p data')
 
@@ -222,6 +233,29 @@ contains
     end do
     close(v)
   end subroutine recrd3
+
+  subroutine adjoint_run()
+    ! gradients of the objective after nadj steps (frozen dt sequence)
+    real(c_double), allocatable :: dts(:), x0(:,:,:,:), gx(:,:,:,:)
+    real(c_double) :: val, geta, ms(2), t
+    integer :: n, v
+    allocate(dts(nadj), x0(mx+4,my,mz+4,8), gx(mx+4,my,mz+4,8))
+    call check(mhd_get_state(h, x0))
+    t = time
+    do n = 1, nadj
+       call check(mhd_compute_dt(h, dts(n)))
+       call check(mhd_step(h, t, dts(n)))
+       t = t + dts(n)
+    end do
+    call check(mhd_set_state(h, x0))
+    call check(mhd_init_residual(h))
+    call check(mhd_adjoint_run(h, nadj, dts, adjobj, 0, MHD_ADJ_RESIDUAL, val, gx, geta, ms))
+    write(*,*) 'adjoint', nadj, val, geta, ms
+    open(newunit=v, file='adjoint.bin', form='unformatted', status='replace', action='write')
+    write(v) val, geta
+    write(v) gx
+    close(v)
+  end subroutine adjoint_run
 
   subroutine check(r)
     integer(c_int), intent(in) :: r

@@ -20,6 +20,7 @@ module mhd_gpu
 
   integer(c_int), parameter :: MHD_OK = 0, MHD_ERR_ARG = -1, MHD_ERR_HIP = -2, &
        MHD_ERR_NOMEM = -3, MHD_ERR_DT_COLLAPSE = -4, MHD_ERR_BLOWUP = -5
+  integer(c_int), parameter :: MHD_ADJ_RESIDUAL = 1
 
   interface
      integer(c_int) function mhd_create(cfg, xx, yy, zz, dx, dy, dz, h) bind(C, name='mhd_create')
@@ -110,6 +111,18 @@ module mhd_gpu
        type(c_ptr), value :: e
        integer(c_int), intent(out) :: status(*)
      end function mhd_ensemble_status
+
+     ! discrete adjoint of nsteps steps with the frozen dt sequence dts
+     ! (objective 0: magnetic energy, 1: reconnected flux); grad_x has the
+     ! layout of x, ms(2) the forward / reverse times
+     integer(c_int) function mhd_adjoint_run(h, nsteps, dts, objective, every, flags, value, grad_x, &
+          grad_eta, ms) bind(C, name='mhd_adjoint_run')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: nsteps, objective, every, flags
+       real(c_double), intent(in) :: dts(*)
+       real(c_double), intent(out) :: value, grad_x(*), grad_eta, ms(2)
+     end function mhd_adjoint_run
 
      subroutine mhd_ensemble_destroy(e) bind(C, name='mhd_ensemble_destroy')
        import :: c_ptr

@@ -40,6 +40,15 @@ _FAMILIES = [
     (r"k_add_eq\b", "post_add_eq"),
     (r"k_dt_(partial|final)\b", "setdt"),
     (r"k_copy\b", "copy_probe"),
+    # discrete adjoint (reverse sweep)
+    (r"k_adj_filter<", "adj_sweep"),
+    (r"k_adj_bc_[xz]\b", "adj_fill"),
+    (r"k_adj_curle_p\b", "adj_curlE_pressure"),
+    (r"k_adj_hall\b", "adj_hall"),
+    (r"k_adj_ext_flux\b", "adj_ext_flux"),
+    (r"k_adj_curl\b", "adj_curl"),
+    (r"k_adj_rk\b", "adj_rk"),
+    (r"k_adj_objective\b", "adj_objective"),
 ]
 
 

@@ -58,8 +58,9 @@ def read_restart(path: str, shape):
 
 
 def write_case(rundir: str, case, *, nstop: int, nst: int = 1, ncase: int = 1, dtime: float = 1.0,
-               dnstep: int = 10):
-    """Write grid.bin, equil.bin, mhd.nml and the initial restart file."""
+               dnstep: int = 10, nadj: int = 0, adjobj: int = 0):
+    """Write grid.bin, equil.bin, mhd.nml and the initial restart file
+    (nadj > 0: the driver differentiates nadj steps instead of running)."""
     os.makedirs(rundir, exist_ok=True)
     with open(os.path.join(rundir, "grid.bin"), "wb") as f:
         for a in (case.xx, case.yy, case.zz, case.dx, case.dy, case.dz):
@@ -75,9 +76,20 @@ def write_case(rundir: str, case, *, nstop: int, nst: int = 1, ncase: int = 1, d
         f.write(f" di={case.di!r}, eta0={case.eta0!r}, fmu0={case.fmu0!r}, frho0={case.frho0!r}, "
                 f"fpr0={case.fpr0!r},\n")
         f.write(f" gamma={case.gamma!r}, cfl={case.cfl!r},\n")
-        f.write(f" nstop={nstop}, nst={nst}, ncase={ncase}, dtime={dtime!r}, dnstep={dnstep}\n/\n")
+        f.write(f" nstop={nstop}, nst={nst}, ncase={ncase}, dtime={dtime!r}, dnstep={dnstep},\n")
+        f.write(f" nadj={nadj}, adjobj={adjobj}\n/\n")
     write_restart(os.path.join(rundir, "mhd" + cn(nst) + cn(0)), case.x0, ncase=ncase, nstep=0,
                   time=0.0, nst=nst)
+
+
+def read_adjoint(path: str, shape3):
+    """adjoint.bin of the driver: (value, d value/d eta0, d value/d x) with
+    the gradient as a C array (8,) + shape3"""
+    with open(path, "rb") as f:
+        recs = [_read_record(f), _read_record(f)]
+    val, geta = np.frombuffer(recs[0], dtype="<f8")
+    gx = np.frombuffer(recs[1], dtype="<f8").reshape((8,) + tuple(shape3))
+    return float(val), float(geta), gx
 
 
 def read_plot(path: str):

@@ -8,7 +8,9 @@ The GPU test runs the driver for 12 steps and requires its final state to
 be bit-identical to the same setdt/stepon sequence driven from Python
 through the same library, with the fort.20 and current.dat cadence of
 t3dsdsc.f:171-181 and 1174-1182; a second one checks the m2d/m3d plot
-files of an output step against the state and current at that step.
+files of an output step against the state and current at that step; a third
+runs the driver's adjoint mode (mhd_adjoint_run called from Fortran) and
+requires the Python call's value and gradients bit for bit.
 """
 import os
 import subprocess
@@ -131,3 +133,29 @@ def test_driver_plot_files(tmp_path):
     assert np.allclose(f3, phys, rtol=r3, atol=0)
     assert not (tmp_path / ("mhd" + rundir.cn(2) + rundir.cn(0))).exists()   # recrd only when mod(nst,3) == 1
 
+
+
+@pytest.mark.gpu
+def test_driver_adjoint_matches_python(tmp_path):
+    """nadj = 4: the driver records the dt sequence, restores x0 and calls
+    mhd_adjoint_run; same numbers as the Python binding of the same call"""
+    import dataclasses
+    from dasha_amd import adjoint
+    from dasha_amd.mhd import MHDStep
+    drv = _driver()
+    ac = adjoint.small_case(24, 8, 24, steps=4, seed=16)
+    case = dataclasses.replace(ac.case, x0=ac.state())
+    rundir.write_case(str(tmp_path), case, nstop=1, nadj=4, adjobj=1)
+    print(f"[driver] adjoint, case={case.name}", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "110", drv], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    val, geta, gx = rundir.read_adjoint(str(tmp_path / "adjoint.bin"), case.shape3)
+    dts = adjoint.dt_sequence(ac)
+    g = MHDStep(case)
+    try:
+        g.init_residual()
+        v2, gx2, ge2, _ = g.adjoint_run(dts, "reconnected_flux")
+    finally:
+        g.close()
+    assert val == v2 and geta == ge2
+    assert np.array_equal(gx, gx2)

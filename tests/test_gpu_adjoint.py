@@ -148,6 +148,34 @@ def test_torch_function():
     val = adjoint.AdjointObjective.apply(ay, eta, ac, dts, "reconnected_flux")
     (2.0 * val).backward()
     v, gay, geta, _ = adjoint.objective(ac, dts, obj="reconnected_flux")
-    assert float(val) == v
+    assert float(val.detach()) == v
     assert np.array_equal(ay.grad.numpy(), 2.0 * gay)
     assert float(eta.grad) == 2.0 * geta
+
+
+@pytest.mark.parametrize("obj", ["magnetic_energy", "reconnected_flux"])
+def test_c5_central_difference(obj):
+    """the full adjoint workload: 128x64x128 stretched, eta 0.005, d_i 0.15,
+    500 steps, sqrt(N) checkpoints; gradients to 1e-6 of central differences"""
+    ac = adjoint.case_c5()
+    print(f"[adjoint] C5 central differences {obj} {ac.case.name} {ac.steps} steps", flush=True)
+    dts = adjoint.dt_sequence(ac)
+    val, gay, geta, ms = adjoint.objective(ac, dts, obj=obj)
+    print(f"forward {ms[0]:.1f} ms reverse {ms[1]:.1f} ms", flush=True)
+    assert ms[1] <= 4.0 * ms[0]
+    rng = np.random.default_rng(7)
+    if obj == "magnetic_energy":
+        v = rng.standard_normal(ac.ay0.shape) * 0.01
+    else:
+        v = ac.ay0 * (1.0 + 0.5 * rng.uniform(-1.0, 1.0, ac.ay0.shape))
+    eps = 1e-4
+    fd = (adjoint.forward_value(ac, dts, ac.ay0 + eps * v, obj=obj) -
+          adjoint.forward_value(ac, dts, ac.ay0 - eps * v, obj=obj)) / (2 * eps)
+    ad = float(np.vdot(gay, v))
+    print("ay", fd, ad, abs(fd - ad) / abs(fd), flush=True)
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
+    e = 1e-3 * ac.case.eta0
+    fde = (adjoint.forward_value(ac, dts, eta=ac.case.eta0 + e, obj=obj) -
+           adjoint.forward_value(ac, dts, eta=ac.case.eta0 - e, obj=obj)) / (2 * e)
+    print("eta", fde, geta, abs(fde - geta) / abs(geta), flush=True)
+    assert abs(fde - geta) <= 1e-6 * abs(geta)
