@@ -582,10 +582,8 @@ template <int NP, int KIND, int TYPE, int RKS, int MODE>
 static void launch_zs(mhd_handle *h, dim3 grid, const unsigned *tab, int nch, const double *a, double *b,
                       const RKArgs &rk)
 {
-    if (h->zkc == 64)
-        hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, 64, MODE>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, tab, nch);
-    else
-        hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, 32, MODE>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, tab, nch);
+    hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, MODE>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, tab,
+                       nch, h->zkc);
 }
 
 // NP smoothing passes of NF fields a -> b.  The RK stage sweeps (two more
@@ -1150,17 +1148,32 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     HCHK(hipMemset(h->dtv, 0, 8 * sizeof(double)));
     h->xd = h->xdA;
     h->cur = h->cA;
-    // z planes per chunk of the tile kernels: 32 on volumes below 2^25
-    // points, 16 above (measured at 256x128x256 and 512x256x512)
-    h->kc = h->S < (1L << 25) ? 32 : 16;
+    // z planes per chunk of the tile kernels: 16 on volumes of 2^25 points
+    // and more; below, the longest of 32, 16, 8, 4 that still gives 400
+    // work-groups (measured at 64x32x64 to 512x256x512: short chunks pay
+    // for their z halo, too few work-groups leave CUs idle)
+    {
+        const long tiles = (long)blocks(NX, TX) * blocks(h->g.yhi - h->g.ylo + 1, TY);
+        h->kc = 32;
+        while (h->kc > 4 && tiles * blocks(NZ - 4, h->kc) < 400) h->kc /= 2;
+        if (h->S >= (1L << 25)) h->kc = 16;
+    }
     if (const char *e = getenv("MHD_KC")) h->kc = std::max(1, atoi(e));
     if (const char *e = getenv("MHD_FUSE_RK")) h->fuse_rk = atoi(e);
     if (const char *e = getenv("MHD_ZS")) h->zs = atoi(e);
     // windowed right-part-2 kernels pay off on large volumes (measured at
     // 512x256x512 against 256x128x256)
     h->rb2 = h->S >= (1L << 25) ? 1 : 0;
-    h->zkc = h->S >= (1L << 25) ? 64 : 32;
-    if (const char *e = getenv("MHD_ZKC")) h->zkc = atoi(e) == 64 ? 64 : 32;
+    // interior planes per chunk of the smoothing kernels: 64 on volumes of
+    // 2^25 points and more; below, the longest of 32, 16, 8 that gives 600
+    // work-groups for a 3-field two-pass launch (same measurements)
+    {
+        const long wins = (long)blocks(NX, TX - 4) * blocks(h->g.yhi - h->g.ylo + 1, ZB - 4) * 3;
+        h->zkc = 32;
+        while (h->zkc > 8 && wins * blocks(NZ - 4, h->zkc) < 600) h->zkc /= 2;
+        if (h->S >= (1L << 25)) h->zkc = 64;
+    }
+    if (const char *e = getenv("MHD_ZKC")) h->zkc = std::min(ZKC_MAX, std::max(4, atoi(e)));
     if (const char *e = getenv("MHD_RB2")) h->rb2 = atoi(e);
     if (const char *e = getenv("MHD_ZORD")) h->zord = std::min(1, std::max(0, atoi(e)));
     if (const char *e = getenv("MHD_ZS3")) h->zs3 = atoi(e);
@@ -1553,6 +1566,7 @@ struct RunGraphs {
 
 struct mhd_ensemble {
     std::vector<mhd_handle *> runs;
+    std::vector<std::pair<int, int>> chunks;   // each run's own (kc, zkc), restored at destroy
     std::vector<RunGraphs> graphs;         // two per run
     std::vector<int> status;
     std::vector<hipEvent_t> done;          // end of each run's work
@@ -1560,6 +1574,30 @@ struct mhd_ensemble {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::string err;
 };
+
+// z chunk lengths of a handle's tile and smoothing kernels (the smoothing
+// block tables are rebuilt).  Results do not depend on them.
+static int set_chunks(mhd_handle *h, int kc, int zkc)
+{
+    if (h->kc == kc && h->zkc == zkc) return 0;
+    HCHK(hipStreamSynchronize(h->stream));
+    h->kc = kc;
+    h->zkc = zkc;
+    for (auto &zc : h->zt)
+        for (auto &t : zc) {
+            if (t.d) (void)hipFree(t.d);
+            t = ZTab();
+        }
+    for (auto &t : h->zt3) {
+        if (t.d) (void)hipFree(t.d);
+        t = ZTab();
+    }
+    for (int c = 0; c < 2; c++) {
+        int r = zs_tables(h, c ? 8 : 3, 2, h->zt[c]);
+        if (r) return r;
+    }
+    return zs_tables(h, 3, 3, h->zt3);
+}
 
 static int capture_pair(mhd_handle *h, int opening, hipGraphExec_t *out)
 {
@@ -1627,6 +1665,15 @@ int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out)
     if (st == hipSuccess) st = hipEventCreate(&e->e1);
     for (int r = 0; r < n && st == hipSuccess; r++) st = hipEventCreateWithFlags(&e->done[r], hipEventDisableTiming);
     if (st != hipSuccess) { e->err = std::string("ensemble set-up: ") + hipGetErrorString(st); return MHD_ERR_HIP; }
+    // runs that share the device do not need short chunks to fill it: the
+    // long ones (less z halo per plane) measured faster for the batch
+    for (int r = 0; r < n; r++) {
+        mhd_handle *h = e->runs[r];
+        e->chunks.emplace_back(h->kc, h->zkc);
+        const bool big = h->S >= (1L << 25);
+        int rc = set_chunks(h, big ? 16 : 32, big ? 64 : 32);
+        if (rc) { e->err = h->err; return rc; }
+    }
     return MHD_OK;
 }
 
@@ -1712,6 +1759,7 @@ void mhd_ensemble_destroy(mhd_ensemble *e)
     if (e->lead) (void)hipStreamSynchronize(e->lead);
     for (auto *h : e->runs) (void)hipStreamSynchronize(h->stream);
     for (auto &g : e->graphs) drop_graphs(g);
+    for (size_t r = 0; r < e->chunks.size(); r++) (void)set_chunks(e->runs[r], e->chunks[r].first, e->chunks[r].second);
     for (auto ev : e->done)
         if (ev) (void)hipEventDestroy(ev);
     if (e->e0) (void)hipEventDestroy(e->e0);

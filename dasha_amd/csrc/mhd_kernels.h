@@ -154,6 +154,9 @@ __device__ inline double row_xrule_rl(const Geo &g, int i, int i0, bool apply, d
 constexpr int BY = 8;
 constexpr int SWEEP_UZ = 4;
 
+// interior planes per chunk of k_zsmooth: a launch argument up to this
+constexpr int ZKC_MAX = 64;
+
 struct RKArgs {
     const double *dtp;     // device dt
     const double *XF;      // xfold
@@ -322,14 +325,14 @@ enum { ZM_ALL = 0, ZM_PLAIN = 1, ZM_FACE = 2 };
 constexpr int RKS_ADD = -2;
 constexpr int ZPF = 3;            // planes loaded ahead
 
-template <int NP, int KIND, int TYPE, int RKS, int KC, int MODE>
+template <int NP, int KIND, int TYPE, int RKS, int MODE>
 __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE == ZM_FACE ? 4 : 8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
-                                                     RKArgs rk, const unsigned *__restrict__ ztab, int nch)
+                                                     RKArgs rk, const unsigned *__restrict__ ztab, int nch, int kc)
 {
     static_assert(NP >= 1 && (NP <= 2 || KIND == KIND_EIGHTH), "the radial coefficient ring holds planes t-2..t");
     static_assert(NP <= 3 && (RKS != RKS_ADD || NP == 3), "passes per launch");
     constexpr int W = TX - 2 * NP, H = ZB - 2 * NP;
-    constexpr int NZW = KC + 4 * NP;           // z-weight planes of a chunk
+    constexpr int NZW = ZKC_MAX + 4 * NP;      // z-weight planes of the longest chunk
     constexpr bool RAD = KIND != KIND_EIGHTH;   // reads a smoothing coefficient stream
     constexpr bool TT = TYPE == BC_TT;          // post-step passes on x - eq (bndrytt fills)
     constexpr bool ADD = RKS == RKS_ADD;        // store the output plus a field
@@ -406,11 +409,11 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         const double *cbase = bcomp ? g.cbx : crad;
         const unsigned cq0 = bcomp ? (unsigned)i : qc, csz = bcomp ? 0u : sz;
 
-        const int kb = 2 + chunk * KC, ke = min(kb + KC, g.NZ - 2);
+        const int kb = 2 + chunk * kc, ke = min(kb + kc, g.NZ - 2);
         const int t0 = kb - NP, t1 = ke - 1 + NP;
         const int tl = min(t1, g.NZ - 1);           // last plane loaded (three passes reach past NZ-1)
         const int kz0 = t0 - NP;                    // lowest plane a level touches
-        for (int n = threadIdx.x; n < NZW; n += TX * ZB) {
+        for (int n = threadIdx.x; n < kc + 4 * NP; n += TX * ZB) {
             const int k = kz0 + n;
             const bool kin = k >= 2 && k <= g.NZ - 3;
             szw[0][n] = kin ? g.zwp[k] : 0.0;

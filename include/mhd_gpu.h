@@ -130,7 +130,10 @@ void mhd_destroy(mhd_handle *h);
  * the run's stream.  dt0[r] arms run r's collapse guard; time[r] is
  * advanced in place; *elapsed_ms is the wall time of the whole batch
  * (HIP events).  Returns the first failing run's status (its text via
- * mhd_ensemble_error); mhd_ensemble_status gives every run's. */
+ * mhd_ensemble_error); mhd_ensemble_status gives every run's.  While
+ * grouped, a run's kernels march longer z chunks (the runs fill the device
+ * together); results do not change, and the run's own chunking returns at
+ * mhd_ensemble_destroy. */
 typedef struct mhd_ensemble mhd_ensemble;
 int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out);
 int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double *time,
