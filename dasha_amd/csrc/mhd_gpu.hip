@@ -1567,6 +1567,7 @@ struct RunGraphs {
 struct mhd_ensemble {
     std::vector<mhd_handle *> runs;
     std::vector<std::pair<int, int>> chunks;   // each run's own (kc, zkc), restored at destroy
+    int policy = 0;                            // z-chunk policy in force (ENS_POLICIES)
     std::vector<RunGraphs> graphs;         // two per run
     std::vector<int> status;
     std::vector<hipEvent_t> done;          // end of each run's work
@@ -1643,6 +1644,33 @@ static int run_graphs(mhd_ensemble *e, int r, RunGraphs **out)
     return MHD_OK;
 }
 
+// z-chunk policies the ensemble tuner times: 0 long chunks, 1 each run's
+// own (sized to fill the device alone), 2 16/16, 3 8/8 (tile / smoothing)
+static constexpr int ENS_POLICIES = 4;
+
+static std::pair<int, int> policy_chunks(const mhd_ensemble *e, int r, int p)
+{
+    const bool big = e->runs[r]->S >= (1L << 25);
+    switch (p) {
+    case 0: return {big ? 16 : 32, big ? 64 : 32};
+    case 1: return e->chunks[r];
+    case 2: return {16, 16};
+    default: return {8, 8};
+    }
+}
+
+static int apply_policy(mhd_ensemble *e, int p)
+{
+    for (auto &g : e->graphs) drop_graphs(g);   // captured with the old launch arguments
+    for (size_t r = 0; r < e->runs.size(); r++) {
+        const auto c = policy_chunks(e, (int)r, p);
+        const int rc = set_chunks(e->runs[r], c.first, c.second);
+        if (rc) { e->err = "run " + std::to_string(r) + ": " + e->runs[r]->err; return rc; }
+    }
+    e->policy = p;
+    return MHD_OK;
+}
+
 extern "C" {
 
 int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out)
@@ -1666,15 +1694,10 @@ int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out)
     for (int r = 0; r < n && st == hipSuccess; r++) st = hipEventCreateWithFlags(&e->done[r], hipEventDisableTiming);
     if (st != hipSuccess) { e->err = std::string("ensemble set-up: ") + hipGetErrorString(st); return MHD_ERR_HIP; }
     // runs that share the device do not need short chunks to fill it: the
-    // long ones (less z halo per plane) measured faster for the batch
-    for (int r = 0; r < n; r++) {
-        mhd_handle *h = e->runs[r];
-        e->chunks.emplace_back(h->kc, h->zkc);
-        const bool big = h->S >= (1L << 25);
-        int rc = set_chunks(h, big ? 16 : 32, big ? 64 : 32);
-        if (rc) { e->err = h->err; return rc; }
-    }
-    return MHD_OK;
+    // long ones (less z halo per plane) measured faster for the C4 batch;
+    // mhd_ensemble_tune picks by measurement
+    for (int r = 0; r < n; r++) e->chunks.emplace_back(e->runs[r]->kc, e->runs[r]->zkc);
+    return apply_policy(e, 0);
 }
 
 int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double *time, double *elapsed_ms)
@@ -1742,6 +1765,53 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
         }
     }
     return first;
+}
+
+int mhd_ensemble_tune(mhd_ensemble *e, int trial, const double *dt0, const double *time, int *best, double *ms)
+{
+    if (!e || trial < 1 || !dt0 || !time) return MHD_ERR_ARG;
+    const int n = (int)e->runs.size();
+    std::vector<double *> keep(n, nullptr);
+    auto release = [&]() {
+        for (auto p : keep)
+            if (p) (void)hipFree(p);
+    };
+    for (int r = 0; r < n; r++) {
+        mhd_handle *h = e->runs[r];
+        if (hipMalloc(&keep[r], 8 * h->S * sizeof(double)) != hipSuccess ||
+            hipMemcpyAsync(keep[r], h->x, 8 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            release();
+            e->err = "tune: saving the states failed";
+            return MHD_ERR_NOMEM;
+        }
+    }
+    double tms[ENS_POLICIES];
+    int pick = 0;
+    for (int p = 0; p < ENS_POLICIES; p++) {
+        int rc = apply_policy(e, p);
+        std::vector<double> t(time, time + n);
+        double el = 0;
+        // two steps capture the graphs; the trial that follows is timed
+        if (!rc) rc = mhd_ensemble_advance(e, 2, dt0, t.data(), nullptr);
+        if (!rc) rc = mhd_ensemble_advance(e, trial, dt0, t.data(), &el);
+        tms[p] = (rc == MHD_OK) ? el : 1e30;
+        for (int r = 0; r < n; r++) {
+            mhd_handle *h = e->runs[r];
+            if (hipMemcpyAsync(h->x, keep[r], 8 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess) {
+                release();
+                e->err = "tune: restoring the states failed";
+                return MHD_ERR_HIP;
+            }
+        }
+        if (rc != MHD_OK && rc != MHD_ERR_DT_COLLAPSE) { release(); return rc; }
+        if (tms[p] < tms[pick]) pick = p;
+        if (ms) ms[p] = tms[p];
+    }
+    release();
+    if (best) *best = pick;
+    return apply_policy(e, pick);
 }
 
 int mhd_ensemble_status(const mhd_ensemble *e, int *status)

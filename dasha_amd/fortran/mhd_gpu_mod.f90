@@ -124,6 +124,15 @@ module mhd_gpu
        real(c_double), intent(out) :: value, grad_x(*), grad_eta, ms(2)
      end function mhd_adjoint_run
 
+     integer(c_int) function mhd_ensemble_tune(e, trial, dt0, time, best, ms) bind(C, name='mhd_ensemble_tune')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: e
+       integer(c_int), value :: trial
+       real(c_double), intent(in) :: dt0(*), time(*)
+       integer(c_int), intent(out) :: best
+       real(c_double), intent(out) :: ms(4)
+     end function mhd_ensemble_tune
+
      subroutine mhd_ensemble_destroy(e) bind(C, name='mhd_ensemble_destroy')
        import :: c_ptr
        type(c_ptr), value :: e

@@ -28,7 +28,7 @@ EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state"
            "mhd_set_residual", "mhd_set_profiling", "mhd_kernel_times", "mhd_copy_rate",
            "mhd_last_error", "mhd_destroy", "mhd_ensemble_create", "mhd_ensemble_advance",
            "mhd_ensemble_status", "mhd_ensemble_error", "mhd_ensemble_destroy",
-           "mhd_adjoint_run", "mhd_adjoint_rhs")
+           "mhd_adjoint_run", "mhd_adjoint_rhs", "mhd_ensemble_tune")
 
 MHD_ADJ_RESIDUAL = 1
 OBJECTIVES = {"magnetic_energy": 0, "reconnected_flux": 1}
@@ -92,6 +92,7 @@ def load_library(path: str = LIBPATH):
     L.mhd_ensemble_error.restype = C.c_char_p
     L.mhd_ensemble_destroy.argtypes = [P]
     L.mhd_adjoint_run.argtypes = [P, I, D, I, I, I, D, D, D, D]
+    L.mhd_ensemble_tune.argtypes = [P, I, D, D, C.POINTER(I), D]
     L.mhd_adjoint_rhs.argtypes = [P, D, D, D, D, I, I]
     _lib = L
     return L
@@ -291,6 +292,19 @@ class MHDEnsemble:
         if rc != MHD_OK:
             raise MHDError(rc, self.L.mhd_ensemble_error(self.e).decode())
         return t, ms.value
+
+    def tune(self, trial: int, times, dt0s):
+        """time `trial` batch steps under each z-chunk policy and keep the
+        fastest; returns (choice, trial ms per policy).  States and times
+        are unchanged."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        d = np.ascontiguousarray(dt0s, dtype=np.float64)
+        best = C.c_int()
+        ms = np.zeros(4)
+        rc = self.L.mhd_ensemble_tune(self.e, int(trial), _p(d), _p(t), C.byref(best), _p(ms))
+        if rc != MHD_OK:
+            raise MHDError(rc, self.L.mhd_ensemble_error(self.e).decode())
+        return best.value, [float(m) for m in ms]
 
     def status(self):
         st = (C.c_int * len(self.runs))()

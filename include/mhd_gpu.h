@@ -141,6 +141,15 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
 int mhd_ensemble_status(const mhd_ensemble *e, int *status);
 const char *mhd_ensemble_error(const mhd_ensemble *e);
 void mhd_ensemble_destroy(mhd_ensemble *e);
+/* Tile-shape autotuner of the ensemble (no counterpart in the reference):
+ * times `trial` steps of the whole batch (after two untimed steps that
+ * capture the graphs) under each z-chunk policy -- 0 long chunks, 1 each
+ * run's own, 2 16/16, 3 8/8 planes (tile / smoothing kernels) -- restores
+ * every run's state after each trial, and keeps the fastest.  ms[4] (may
+ * be NULL) receives the trial times, *best (may be NULL) the choice.  The
+ * runs' states and the caller's times are unchanged on return. */
+int mhd_ensemble_tune(mhd_ensemble *e, int trial, const double *dt0, const double *time, int *best,
+                      double *ms);
 
 /* Discrete adjoint of the step (no counterpart in the reference, which has
  * no derivatives; the reverse mode of stepon/right, t3dsdsc.f:1063-1673).

@@ -51,3 +51,30 @@ def test_ensemble_rejects_bad_input():
         MHDEnsemble([g, g])          # a run appears once
     with pytest.raises(MHDError):
         MHDEnsemble([])
+
+
+def test_tuned_ensemble_matches_single_runs():
+    """the tuner's trials leave states and times alone, and whatever chunk
+    policy it keeps, results stay bit-identical to the runs alone"""
+    from dasha_amd.mhd import MHDEnsemble, MHDStep
+    cases = _cases()
+    print("[ensemble] tune", [c.name for c in cases], flush=True)
+    singles = [MHDStep(c) for c in cases]
+    members = [MHDStep(c) for c in cases]
+    for g in singles + members:
+        g.init_residual()
+    dt0 = [g.compute_dt() for g in singles]
+    x_before = [g.get_state() for g in members]
+    ens = MHDEnsemble(members)
+    best, ms = ens.tune(3, np.zeros(len(cases)), dt0)
+    print(f"[ensemble] tuned policy {best}, trial ms {ms}", flush=True)
+    assert 0 <= best < 4 and all(m > 0 for m in ms)
+    for g, x in zip(members, x_before):
+        assert np.array_equal(g.get_state(), x)
+    t_ens, _ = ens.advance(5, np.zeros(len(cases)), dt0)
+    for g, d, t in zip(singles, dt0, t_ens):
+        tn, _, _ = g.advance(5, 0.0, d)
+        assert tn == t
+    for a, b in zip(singles, members):
+        assert np.array_equal(a.get_state(), b.get_state())
+    ens.close()
