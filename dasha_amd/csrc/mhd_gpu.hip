@@ -1344,12 +1344,26 @@ int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective,
             if (Z[k] < 0.0 && Z[k + 1] >= 0.0) k0 = k;
         if (objective == 1 && k0 < 0) { set_err(h, "adjoint: z = 0 is not inside the mesh"); return MHD_ERR_ARG; }
     }
-    AdjBufs A;
+    // buffers, events and the recording hook are released on every return
+    struct Scope {
+        mhd_handle *h;
+        AdjBufs A;
+        hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+        ~Scope()
+        {
+            h->rec_x = h->rec_c = nullptr;
+            if (h->stream) (void)hipStreamSynchronize(h->stream);
+            adj_free(A);
+            for (auto x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } sc{h};
+    AdjBufs &A = sc.A;
+    hipEvent_t *e = sc.e;
     int r = adj_alloc(h, A, nck, K);
-    if (r) { adj_free(A); return r; }
+    if (r) return r;
     const size_t b8 = 8 * h->S * sizeof(double);
-    hipEvent_t e[3];
-    for (auto &x : e) HCHK(hipEventCreate(&x));
+    for (int i = 0; i < 3; i++) HCHK(hipEventCreate(&e[i]));
     HCHK(hipEventRecord(e[0], h->stream));
     // forward pass, keeping every K-th state
     for (int n = 0; n < nsteps; n++) {
@@ -1388,15 +1402,12 @@ int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective,
     HCHK(hipEventRecord(e[2], h->stream));
     HCHK(hipMemcpyAsync(h->x, A.xfin, b8, hipMemcpyDeviceToDevice, h->stream));
     r = adj_sums(h, A, grad_eta, value);
-    if (r) { adj_free(A); return r; }
+    if (r) return r;
     float f0 = 0, f1 = 0;
     HCHK(hipEventElapsedTime(&f0, e[0], e[1]));
     HCHK(hipEventElapsedTime(&f1, e[1], e[2]));
-    for (auto &x : e) (void)hipEventDestroy(x);
     if (ms) { ms[0] = f0; ms[1] = f1; }
-    r = grad_x ? download_fields(h, grad_x, A.xb, 8, false) : 0;
-    adj_free(A);
-    return r;
+    return grad_x ? download_fields(h, grad_x, A.xb, 8, false) : 0;
 }
 
 int mhd_adjoint_rhs(mhd_handle *h, const double *xdif_bar, double *x_bar, double *xresd_bar, double *eta_bar,

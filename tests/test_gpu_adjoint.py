@@ -33,10 +33,18 @@ def _unique_rows(a):
     return b
 
 
-@pytest.mark.parametrize("mode", ["smooth", "nosmooth", "init"])
-def test_rhs_vjp_matches_oracle(mode):
-    c = harris.case_small(24, 8, 24, seed=11)
-    print(f"[adjoint] rhs vjp {mode} {c.name}", flush=True)
+@pytest.mark.parametrize("mode,variant", [("smooth", "stretched"), ("nosmooth", "stretched"),
+                                          ("init", "stretched"), ("smooth", "no_hall"),
+                                          ("smooth", "uniform")])
+def test_rhs_vjp_matches_oracle(mode, variant):
+    if variant == "no_hall":
+        c = harris.case_small(24, 8, 24, seed=11, di=0.0)
+    elif variant == "uniform":
+        c = harris.case_small(20, 8, 28, stretched=False, seed=11)
+    else:
+        c = harris.case_small(24, 8, 24, seed=11)
+    assert c.hall == (variant != "no_hall")
+    print(f"[adjoint] rhs vjp {mode} {variant} {c.name}", flush=True)
     orc.build()
     o = orc.Oracle(c)
     g = _mhd()(c)
