@@ -1164,14 +1164,18 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     // windowed right-part-2 kernels pay off on large volumes (measured at
     // 512x256x512 against 256x128x256)
     h->rb2 = h->S >= (1L << 25) ? 1 : 0;
-    // interior planes per chunk of the smoothing kernels: 64 on volumes of
-    // 2^25 points and more; below, the longest of 32, 16, 8 that gives 600
-    // work-groups for a 3-field two-pass launch (same measurements)
+    // interior planes per chunk of the smoothing kernels, the chunks of a
+    // column of equal length: about 72 planes on volumes of 2^25 points and
+    // more (73 at 512x256x512 measured 0.7% faster than 64 and 85);
+    // below, about the longest of 32, 16, 8 that gives 600 work-groups to a
+    // 3-field two-pass launch (same measurements)
     {
         const long wins = (long)blocks(NX, TX - 4) * blocks(h->g.yhi - h->g.ylo + 1, ZB - 4) * 3;
-        h->zkc = 32;
-        while (h->zkc > 8 && wins * blocks(NZ - 4, h->zkc) < 600) h->zkc /= 2;
-        if (h->S >= (1L << 25)) h->zkc = 64;
+        int z = 32;
+        while (z > 8 && wins * blocks(NZ - 4, z) < 600) z /= 2;
+        int nch = blocks(NZ - 4, z);
+        if (h->S >= (1L << 25)) nch = std::max(1, (int)std::lround((NZ - 4) / 72.0));
+        h->zkc = std::min(ZKC_MAX, blocks(NZ - 4, nch));
     }
     if (const char *e = getenv("MHD_ZKC")) h->zkc = std::min(ZKC_MAX, std::max(4, atoi(e)));
     if (const char *e = getenv("MHD_RB2")) h->rb2 = atoi(e);

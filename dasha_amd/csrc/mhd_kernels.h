@@ -155,7 +155,7 @@ constexpr int BY = 8;
 constexpr int SWEEP_UZ = 4;
 
 // interior planes per chunk of k_zsmooth: a launch argument up to this
-constexpr int ZKC_MAX = 64;
+constexpr int ZKC_MAX = 96;
 
 struct RKArgs {
     const double *dtp;     // device dt
