@@ -35,12 +35,14 @@ def _unique_rows(a):
 
 @pytest.mark.parametrize("mode,variant", [("smooth", "stretched"), ("nosmooth", "stretched"),
                                           ("init", "stretched"), ("smooth", "no_hall"),
-                                          ("smooth", "uniform")])
+                                          ("smooth", "uniform"), ("smooth", "ny12")])
 def test_rhs_vjp_matches_oracle(mode, variant):
     if variant == "no_hall":
         c = harris.case_small(24, 8, 24, seed=11, di=0.0)
     elif variant == "uniform":
         c = harris.case_small(20, 8, 28, stretched=False, seed=11)
+    elif variant == "ny12":          # a partial tile of unique rows
+        c = harris.case_small(24, 12, 24, seed=11)
     else:
         c = harris.case_small(24, 8, 24, seed=11)
     assert c.hall == (variant != "no_hall")
