@@ -11,6 +11,7 @@ Prints ONE JSON line:
   value          cell-updates/s = mx*ny*mz*steps / T (physical cells)
   roofline       dominant kernel family: algorithmic bytes / HIP-event time
   step_roofline  whole step: value * 6256 B (5104 B without Hall) / 8 TB/s
+  divb_drift     max and rms of div B (diverB + bndrydiv) before and after the run
   cpu_baseline   the CPU oracle (C restatement, OpenMP) on a bounded sample
   C3             the same timed loop on 512x256x512
   ensemble       48 uneven Harris runs advanced together vs one after another
@@ -219,6 +220,7 @@ def main():
     dt0 = g.compute_dt()
     print(f"[bench] {case.name}: setup {time.perf_counter() - t_setup:.1f}s, dt0 {dt0:.4e}",
           file=sys.stderr, flush=True)
+    d_start = g.diagnostics()
     # warm-up steps (untimed)
     tm, _, _ = g.advance(args.warmup, 0.0, dt0)
     # timed region: K steps of setdt+stepon, HIP events on the library stream
@@ -228,6 +230,9 @@ def main():
     print(f"[bench] timed {args.steps} steps: {ms:.1f} ms ({ms_per_step:.3f} ms/step)",
           file=sys.stderr, flush=True)
 
+    d_end = g.diagnostics()
+    divb = {"steps": args.warmup + args.steps, "max_start": d_start["divb_max"], "max_end": d_end["divb_max"],
+            "rms_start": d_start["divb_rms"], "rms_end": d_end["divb_rms"]}
     # the state crossing PCIe (restart / output points only, never per step)
     t0 = time.perf_counter()
     xs = g.get_state()
@@ -278,6 +283,7 @@ def main():
                           "frac": value * b_alg / HBM_PEAK},
         "kernel_families_ms": {k: {"total_ms": v[0], "calls": v[1]} for k, v in fam.items()},
         "host_io": host_io,
+        "divb_drift": divb,
         "cpu_baseline": None,
     }
     if args.copy_probe:
