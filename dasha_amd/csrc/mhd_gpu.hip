@@ -354,6 +354,7 @@ struct mhd_handle {
     ZTab zt[2][2];                         // their block tables: [3 or 8 fields][without / with face rules]
     ZTab zt3[2];                           // the same for three passes on 3 fields
     int zs3 = 1;                           // last three avrgh passes (+ E) in one launch
+    int face12 = 1;                        // 12-row blocks for the face windows (periodic y)
     // stage recording of the adjoint's forward replay: when rec_x is set,
     // right() copies its state and current into slot rec_n and advances it
     double *rec_x = nullptr, *rec_c = nullptr;
@@ -524,14 +525,20 @@ static void launch_sweep(mhd_handle *h, const double *a, double *b, const double
 // with the field fastest, then x tile, y tile, chunk: one tile's fields read
 // the same radial coefficients together, and neighbouring tiles share halo
 // columns and rows, in one L2.
+// rows per block of the face-window launch: 12 with periodic y (the face
+// windows are whole x-tile columns then, tiled in rows of their own), else
+// the rows of the other windows
+static int face_rows(const mhd_handle *h) { return h->g.periodic && h->face12 ? 12 : ZB; }
+
 static int zs_tables(mhd_handle *h, int nf, int np, ZTab *out)
 {
-    const int W = TX - 2 * np, H = ZB - 2 * np;
+    const int W = TX - 2 * np, H = ZB - 2 * np, HF = face_rows(h) - 2 * np;
     const Geo &g = h->g;
     const unsigned gx = (unsigned)blocks(h->NX, W);
     const unsigned gy = (unsigned)blocks(g.yhi - g.ylo + 1, H);
+    const unsigned gyf = (unsigned)blocks(g.yhi - g.ylo + 1, HF);
     const unsigned nch = (unsigned)blocks(h->NZ - 4, h->zkc);
-    if (gx >= (1u << ZT_BY) || gy >= (1u << (ZT_CH - ZT_BY)) || nch >= (1u << (ZT_F - ZT_CH)) ||
+    if (gx >= (1u << ZT_BY) || std::max(gy, gyf) >= (1u << (ZT_CH - ZT_BY)) || nch >= (1u << (ZT_F - ZT_CH)) ||
         nf > (1 << (32 - ZT_F))) {
         set_err(h, "grid too large for the smoothing kernels' block table");
         return MHD_ERR_ARG;
@@ -548,19 +555,25 @@ static int zs_tables(mhd_handle *h, int nf, int np, ZTab *out)
         return f;
     };
     std::vector<unsigned> list[2];
-    auto add = [&](unsigned bx, unsigned by, unsigned ch, unsigned m) {
-        list[faces(bx, by) ? 1 : 0].push_back(bx | by << ZT_BY | ch << ZT_CH | m << ZT_F);
+    // class c of the windows: with periodic y a window holds a face exactly
+    // when its x tile does, and the face class is tiled in rows of HF
+    auto add = [&](int c, unsigned bx, unsigned by, unsigned ch, unsigned m) {
+        const bool f = g.periodic ? faces(bx, 0) : faces(bx, by);
+        if ((f ? 1 : 0) == c) list[c].push_back(bx | by << ZT_BY | ch << ZT_CH | m << ZT_F);
     };
-    if (h->zord == 0) {
-        for (unsigned m = 0; m < (unsigned)nf; m++)
+    for (int c = 0; c < 2; c++) {
+        const unsigned ny = (c == 1 && g.periodic) ? gyf : gy;
+        if (h->zord == 0) {
+            for (unsigned m = 0; m < (unsigned)nf; m++)
+                for (unsigned ch = 0; ch < nch; ch++)
+                    for (unsigned by = 0; by < ny; by++)
+                        for (unsigned bx = 0; bx < gx; bx++) add(c, bx, by, ch, m);
+        } else {
             for (unsigned ch = 0; ch < nch; ch++)
-                for (unsigned by = 0; by < gy; by++)
-                    for (unsigned bx = 0; bx < gx; bx++) add(bx, by, ch, m);
-    } else {
-        for (unsigned ch = 0; ch < nch; ch++)
-            for (unsigned by = 0; by < gy; by++)
-                for (unsigned bx = 0; bx < gx; bx++)
-                    for (unsigned m = 0; m < (unsigned)nf; m++) add(bx, by, ch, m);
+                for (unsigned by = 0; by < ny; by++)
+                    for (unsigned bx = 0; bx < gx; bx++)
+                        for (unsigned m = 0; m < (unsigned)nf; m++) add(c, bx, by, ch, m);
+        }
     }
     for (int c = 0; c < 2; c++) {
         const unsigned T = (unsigned)list[c].size();
@@ -582,6 +595,13 @@ template <int NP, int KIND, int TYPE, int RKS, int MODE>
 static void launch_zs(mhd_handle *h, dim3 grid, const unsigned *tab, int nch, const double *a, double *b,
                       const RKArgs &rk)
 {
+    if constexpr (MODE == ZM_FACE) {
+        if (face_rows(h) == 12) {
+            hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, MODE, 12>), grid, dim3(TX * 12), 0, h->stream, h->g, a,
+                               b, rk, tab, nch, h->zkc);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, MODE>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, tab,
                        nch, h->zkc);
 }
@@ -1181,6 +1201,7 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     if (const char *e = getenv("MHD_RB2")) h->rb2 = atoi(e);
     if (const char *e = getenv("MHD_ZORD")) h->zord = std::min(1, std::max(0, atoi(e)));
     if (const char *e = getenv("MHD_ZS3")) h->zs3 = atoi(e);
+    if (const char *e = getenv("MHD_FACE12")) h->face12 = atoi(e);
     for (int c = 0; c < 2; c++) {
         int r = zs_tables(h, c ? 8 : 3, 2, h->zt[c]);
         if (r) { *out = h; return r; }

@@ -317,28 +317,28 @@ constexpr int ZT_BY = 10, ZT_CH = 20, ZT_F = 29;
 // launch modes: every window from a gx x gy x (nch * nf) grid with the full
 // march; from a block table, the windows without x faces or image rows, or
 // the windows with them.  The table modes use the fast plane groups.  The
-// face windows' march needs more than 64 registers: that launch runs at 4
-// waves per SIMD (one block per CU) rather than spill.
+// face windows' march needs more than 64 registers (6 waves per SIMD): with
+// periodic y their blocks have RB = 12 rows, so two fit a CU.
 enum { ZM_ALL = 0, ZM_PLAIN = 1, ZM_FACE = 2 };
 // RKS value of a sweep that stores its output plus a field (E + xdh after the
 // last avrgh pass, t3dsdsc.f:1426-1433); the field comes in rk.EQ
 constexpr int RKS_ADD = -2;
 constexpr int ZPF = 3;            // planes loaded ahead
 
-template <int NP, int KIND, int TYPE, int RKS, int MODE>
-__global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE == ZM_FACE ? 4 : 8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
+template <int NP, int KIND, int TYPE, int RKS, int MODE, int RB = ZB>
+__global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE == ZM_FACE ? 4 : 8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
                                                      RKArgs rk, const unsigned *__restrict__ ztab, int nch, int kc)
 {
     static_assert(NP >= 1 && (NP <= 2 || KIND == KIND_EIGHTH), "the radial coefficient ring holds planes t-2..t");
     static_assert(NP <= 3 && (RKS != RKS_ADD || NP == 3), "passes per launch");
-    constexpr int W = TX - 2 * NP, H = ZB - 2 * NP;
+    constexpr int W = TX - 2 * NP, H = RB - 2 * NP;
     constexpr int NZW = ZKC_MAX + 4 * NP;      // z-weight planes of the longest chunk
     constexpr bool RAD = KIND != KIND_EIGHTH;   // reads a smoothing coefficient stream
     constexpr bool TT = TYPE == BC_TT;          // post-step passes on x - eq (bndrytt fills)
     constexpr bool ADD = RKS == RKS_ADD;        // store the output plus a field
     constexpr int XT = TT ? BC_AUX : TYPE;      // x-face rule of the pass
-    __shared__ double s0[3][ZB][TX];           // level 0 by plane mod 3
-    __shared__ double sl[NP][2][ZB][TX];       // level p (1..NP) by plane parity
+    __shared__ double s0[3][RB][TX];           // level 0 by plane mod 3
+    __shared__ double sl[NP][2][RB][TX];       // level p (1..NP) by plane parity
     __shared__ double szw[2][NZW];             // z weights of the chunk's planes
     const int lane = threadIdx.x % TX;
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);   // uniform over the wavefront
@@ -381,8 +381,8 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         const bool ownx = vx && i >= nomx && i < nomx + W;
         // rows: wavefront r holds logical row j0 + r (periodic rows map through rmap)
         const int nomy = g.ylo + by * H;
-        const int j0 = g.periodic ? max(g.ylo - NP, min(nomy - NP, g.yhi + NP + 1 - ZB))
-                                  : max(0, min(nomy - NP, g.NY - ZB));
+        const int j0 = g.periodic ? max(g.ylo - NP, min(nomy - NP, g.yhi + NP + 1 - RB))
+                                  : max(0, min(nomy - NP, g.NY - RB));
         const int j = j0 + r;
         const bool vy = !FACES || g.periodic || j < g.NY;
         const bool own = ownx && vy && j >= nomy && j < nomy + H && j <= g.yhi;
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
         const int t0 = kb - NP, t1 = ke - 1 + NP;
         const int tl = min(t1, g.NZ - 1);           // last plane loaded (three passes reach past NZ-1)
         const int kz0 = t0 - NP;                    // lowest plane a level touches
-        for (int n = threadIdx.x; n < kc + 4 * NP; n += TX * ZB) {
+        for (int n = threadIdx.x; n < kc + 4 * NP; n += TX * RB) {
             const int k = kz0 + n;
             const bool kin = k >= 2 && k <= g.NZ - 3;
             szw[0][n] = kin ? g.zwp[k] : 0.0;
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
             for (int p = 1; p <= NP; p++) {
                 const int k = t - p;                        // plane of level p
                 const bool kin = k >= 2 && k <= g.NZ - 3;
-                const int rm = r > 0 ? r - 1 : 0, rp = r < ZB - 1 ? r + 1 : ZB - 1;
+                const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
                 const double f0 = q[p - 1][1];              // level p-1 at plane k
                 double ym, yp;
                 if (p == 1) {
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(TX * ZB) __attribute__((amdgpu_waves_per_eu(MODE ==
             #pragma unroll
             for (int p = 1; p <= NP; p++) {
                 const int k = t - p;
-                const int rm = r > 0 ? r - 1 : 0, rp = r < ZB - 1 ? r + 1 : ZB - 1;
+                const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
                 const double zm = q[p - 1][(u + 1) % 3], f0 = q[p - 1][(u + 2) % 3], zp = q[p - 1][u];
                 double ym, yp;
                 if (p == 1) {
