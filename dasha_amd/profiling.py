@@ -58,10 +58,11 @@ _TOGETHER = {"rhs_curlE_pressure", "setdt"}
 
 def launch_group(kernel_name: str) -> str:
     """kernels of one family launched together share a group: the smoothing
-    sweeps run as two launches, the windows without face rules (last
-    template argument 1) and those with them (2)"""
+    sweeps run as two launches, the windows without face rules (launch-mode
+    template argument 1) and those with them (2, optionally followed by the
+    face launch's row count)"""
     inst = re.sub(r"\(.*", "", kernel_name.replace("(anonymous namespace)::", ""))
-    return re.sub(r", [12]>$", ">", inst)
+    return re.sub(r", [12](, \d+)?>$", ">", inst)
 
 
 def family(kernel_name: str) -> str | None:
