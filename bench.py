@@ -202,7 +202,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the step runs on one GPU's domain).", file=sys.stderr)
         sys.exit(2)
 

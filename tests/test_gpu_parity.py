@@ -4,8 +4,8 @@ Every test builds the same Harris case for both, runs the same sequence of
 calls and compares arrays.  Tolerances: the two sides evaluate the same
 expressions in the same order without FMA contraction; they differ only
 where the device tanh/sqrt differ from glibc by an ulp (radial smoothing
-coefficient), so single calls agree to ~1e-15 and the north-star bound is
-1e-10 relative L2 per field after 100 steps.
+coefficient), so single calls agree to ~1e-15 and the bound after 100 steps
+is 1e-10 relative L2 per field.
 """
 import sys
 
@@ -18,7 +18,7 @@ from tests.util import field_errors, rel_l2
 pytestmark = pytest.mark.gpu
 
 TIGHT = 1e-12      # one operator call / one step
-NORTH = 1e-10      # relative L2 per field after 100 steps
+BOUND_100 = 1e-10  # relative L2 per field after 100 steps
 
 
 def _pair(case, **okw):
@@ -136,7 +136,7 @@ def test_diagnostics_match():
     print("diag", d)
 
 
-def test_c1_100_steps_north_star():
+def test_c1_100_steps():
     """C1 (64x32x64 uniform Harris, eta 0.005, d_i 0.1): 100 RK4 steps,
     relative L2 per field <= 1e-10, div B drift reported."""
     from oracle import oracle as orc_mod
@@ -150,5 +150,5 @@ def test_c1_100_steps_north_star():
     print(f"C1 100 steps: time {tg:.6f}  rel L2 per field {['%.2e' % v for v in e]}", flush=True)
     print(f"C1 div B max {d0['divb_max']:.3e} -> {d1['divb_max']:.3e}, rms {d0['divb_rms']:.3e} -> "
           f"{d1['divb_rms']:.3e}; oracle threads {orc_mod.max_threads()}", flush=True)
-    assert max(e) <= NORTH
+    assert max(e) <= BOUND_100
     assert np.allclose(dg, do, rtol=1e-10, atol=0)
