@@ -84,7 +84,10 @@ int mhd_compute_dt(mhd_handle *h, double *dt);
 int mhd_step(mhd_handle *h, double time, double dt);
 /* the time loop body of program main (t3dsdsc.f:139-176) without I/O:
  * nsteps x (setdt; stepon; time += dt) with dt kept on the device.
- * dt0 > 0 arms the collapse guard dt < 0.1*dt0 (t3dsdsc.f:142).
+ * dt0 > 0 arms the collapse guard dt < 0.1*dt0 (t3dsdsc.f:142): a device
+ * flag, reported as MHD_ERR_DT_COLLAPSE when the call returns (the steps
+ * queued after the collapsing one have run too; a caller that must stop
+ * at that step, as the reference does, drives setdt/stepon per step).
  * On return *time is advanced, *dt_last holds the last dt and
  * *elapsed_ms the HIP-event time of the loop on the handle's stream. */
 int mhd_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_last,

@@ -1,0 +1,100 @@
+"""The reference's abort guards as status codes, and argument checks.
+
+* setdt's collapse guard (t3dsdsc.f:142-154): dt < 0.1 dt0 -> the device
+  flag -> MHD_ERR_DT_COLLAPSE from mhd_advance, and per run from the
+  ensemble;
+* diagn's blow-up guard (t3dsdsc.f:1232): max Jy on the plane jy = my/2+1
+  >= 1e7 -> MHD_ERR_BLOWUP from mhd_diagnostics (after filling the values),
+  and exit status 3 of the Fortran driver;
+* bad arguments come back as MHD_ERR_ARG with a message.
+"""
+import dataclasses
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from dasha_amd import harris, rundir
+
+pytestmark = pytest.mark.gpu
+
+
+def _blowup_state(case):
+    """a sheet of 1e8 jump in Bx at z = 0: Jy = dBx/dz far above 1e7"""
+    x = case.x0.copy()
+    z = case.zz[:, None, None]
+    x[4] += 1e8 * np.tanh(z / 0.05)
+    x[4][:2] = 0.0
+    x[4][-2:] = 0.0
+    return x
+
+
+def test_dt_collapse_guard():
+    from dasha_amd.mhd import MHDEnsemble, MHDError, MHDStep
+    c = harris.case_small(24, 8, 24, seed=31)
+    print(f"[guards] dt collapse {c.name}", flush=True)
+    g = MHDStep(c)
+    g.init_residual()
+    dt = g.compute_dt()
+    t, dl, _ = g.advance(2, 0.0, dt)            # armed at the real dt: no collapse
+    assert t > 0.0 and dl > 0.0
+    with pytest.raises(MHDError) as e:
+        g.advance(1, t, 20.0 * dt)              # dt < 0.1 * dt0
+    assert e.value.code == -4
+    h = MHDStep(c)
+    h.init_residual()
+    ens = MHDEnsemble([g, h])
+    with pytest.raises(MHDError):
+        ens.advance(2, [0.0, 0.0], [dt, 20.0 * dt])
+    assert ens.status() == [0, -4]
+    ens.close()
+    g.close()
+    h.close()
+
+
+def test_current_blowup_guard():
+    from dasha_amd.mhd import MHDStep
+    c = harris.case_small(24, 8, 24, seed=32)
+    print(f"[guards] current blow-up {c.name}", flush=True)
+    g = MHDStep(c)
+    d = g.diagnostics()
+    assert d["status"] == 0 and abs(d["cymax"]) < 1e7
+    g.set_state(_blowup_state(c))
+    d = g.diagnostics()
+    assert d["status"] == -5 and d["cymax"] >= 1e7
+    g.close()
+
+
+def test_driver_stops_on_blowup(tmp_path):
+    """diagn after the first step (nstep 0) sees the blow-up: exit status 3
+    with current.dat written (t3dsdsc.f:1174-1182, 1228-1232)"""
+    from tests.test_fortran_driver import _driver
+    drv = _driver()
+    c = harris.case_small(24, 8, 24, seed=33)
+    case = dataclasses.replace(c, x0=_blowup_state(c))
+    rundir.write_case(str(tmp_path), case, nstop=5, dtime=1e9)
+    print(f"[guards] driver blow-up {case.name}", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "110", drv], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode == 3, (r.returncode, r.stdout[-1000:], r.stderr[-1000:])
+    rows = [ln.split() for ln in (tmp_path / "current.dat").read_text().split("\n") if ln.strip()]
+    assert len(rows) == 1 and float(rows[0][2]) >= 1e7
+
+
+def test_bad_arguments():
+    from dasha_amd.mhd import MHDError, MHDStep
+    c = harris.case_small(24, 8, 24, seed=34)
+    g = MHDStep(c)
+    with pytest.raises(MHDError) as e:
+        g.step(0.0, 0.0)                        # dt must be positive
+    assert e.value.code == -1
+    g.close()
+    with pytest.raises(MHDError):
+        MHDStep(dataclasses.replace(c, mx=6))   # fewer than 8 points
+    nonper = harris.case_small(24, 12, 20, periodic_y=False, seed=35)
+    h = MHDStep(nonper)
+    h.init_residual()
+    with pytest.raises(MHDError) as e:
+        h.adjoint_run([0.005], "magnetic_energy")   # the adjoint needs periodic y
+    assert e.value.code == -1 and "periodic" in str(e.value)
+    h.close()
