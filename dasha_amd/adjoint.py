@@ -159,51 +159,51 @@ def forward_value(ac: AdjointCase, dts, ay=None, eta=None, obj="magnetic_energy"
     return objective_of_state(c.case, x, obj)
 
 
+def _trapezoid(c: np.ndarray) -> np.ndarray:
+    w = np.empty(c.size)
+    w[1:-1] = 0.5 * (c[2:] - c[:-2])
+    w[0] = 0.5 * (c[1] - c[0])
+    w[-1] = 0.5 * (c[-1] - c[-2])
+    return w
+
+
+def _weights(case):
+    """(wx, rows, wy, wz, k0): trapezoid weights over the unique physical
+    points (periodic y: the rows 2..my-3, each dy wide), and the lower of the
+    two z planes nearest z = 0"""
+    px, pz = physical_coords(case)
+    if case.periodic_y:
+        rows = slice(2, case.my - 2)
+        wy = np.full(case.ny, case.yy[3] - case.yy[2])
+    else:
+        rows = slice(0, case.my)
+        wy = _trapezoid(case.yy)
+    hits = np.nonzero((pz[:-1] < 0.0) & (pz[1:] >= 0.0))[0]
+    k0 = int(hits[0]) if hits.size else -1
+    return _trapezoid(px), rows, wy, _trapezoid(pz), k0
+
+
 def objective_of_state(case, x: np.ndarray, obj="magnetic_energy") -> float:
     """host evaluation of the objectives from a state in the host layout"""
-    px, pz = physical_coords(case)
-    ny = case.ny
-    wx = np.empty(case.mx)
-    wx[1:-1] = 0.5 * (px[2:] - px[:-2])
-    wx[0] = 0.5 * (px[1] - px[0])
-    wx[-1] = 0.5 * (px[-1] - px[-2])
-    rows = slice(2, case.my - 2) if case.periodic_y else slice(0, case.my)
+    wx, rows, wy, wz, k0 = _weights(case)
     if obj == "magnetic_energy":
-        wz = np.empty(case.mz)
-        wz[1:-1] = 0.5 * (pz[2:] - pz[:-2])
-        wz[0] = 0.5 * (pz[1] - pz[0])
-        wz[-1] = 0.5 * (pz[-1] - pz[-2])
-        dy = case.yy[3] - case.yy[2]
         b = x[4:7, 2:-2, rows, 2:-2]
-        w = wz[:, None, None] * dy * wx[None, None, :]
+        w = wz[:, None, None] * wy[None, :, None] * wx[None, None, :]
         return float(0.5 * np.sum(w[None] * b * b))
-    k0 = int(np.nonzero((pz[:-1] < 0.0) & (pz[1:] >= 0.0))[0][0])
     bz = 0.5 * (x[6, 2 + k0, rows, 2:-2] + x[6, 3 + k0, rows, 2:-2])
-    return float(np.sum(0.5 * wx[None, :] * np.abs(bz)) / ny)
+    return float(np.sum(0.5 * wx[None, :] * np.abs(bz)) / case.ny)
 
 
 def objective_seed(case, x: np.ndarray, obj="magnetic_energy") -> np.ndarray:
     """d objective_of_state / d x in the host layout (image rows 0)"""
-    px, pz = physical_coords(case)
-    ny = case.ny
-    wx = np.empty(case.mx)
-    wx[1:-1] = 0.5 * (px[2:] - px[:-2])
-    wx[0] = 0.5 * (px[1] - px[0])
-    wx[-1] = 0.5 * (px[-1] - px[-2])
-    rows = slice(2, case.my - 2) if case.periodic_y else slice(0, case.my)
+    wx, rows, wy, wz, k0 = _weights(case)
     s = np.zeros_like(x)
     if obj == "magnetic_energy":
-        wz = np.empty(case.mz)
-        wz[1:-1] = 0.5 * (pz[2:] - pz[:-2])
-        wz[0] = 0.5 * (pz[1] - pz[0])
-        wz[-1] = 0.5 * (pz[-1] - pz[-2])
-        dy = case.yy[3] - case.yy[2]
-        w = wz[:, None, None] * dy * wx[None, None, :]
+        w = wz[:, None, None] * wy[None, :, None] * wx[None, None, :]
         s[4:7, 2:-2, rows, 2:-2] = w[None] * x[4:7, 2:-2, rows, 2:-2]
         return s
-    k0 = int(np.nonzero((pz[:-1] < 0.0) & (pz[1:] >= 0.0))[0][0])
     bz = 0.5 * (x[6, 2 + k0, rows, 2:-2] + x[6, 3 + k0, rows, 2:-2])
-    g = 0.5 * wx[None, :] * np.sign(bz) * 0.5 / ny
+    g = 0.5 * wx[None, :] * np.sign(bz) * 0.5 / case.ny
     s[6, 2 + k0, rows, 2:-2] = g
     s[6, 3 + k0, rows, 2:-2] = g
     return s

@@ -13,7 +13,7 @@
 // w(p -> q) * a(p) over the interior points p whose stencil reads q, with
 // the coefficients of p.  One thread per point of the unique range
 // (periodic y: rows 2..NY-3, the image rows are implicit, as in the forward
-// kernels).  Reference routines: right t3dsdsc.f:1297-1673, current
+// kernels; non-periodic: every row).  Reference routines: right t3dsdsc.f:1297-1673, current
 // 2452-2500, bndry 1677-1985, bndrytt 1989-2284, bndryc/h/E 2539-3027,
 // avrg* 5611-6054, stepon 1063-1185.
 #pragma once
@@ -110,20 +110,22 @@ __device__ inline double fdz(const Geo &g, const double *f, int i, int j, int k)
 }
 
 // ---------------------------------------------------------------------------
-// boundary rules, transposed.  Forward order x, then (periodic) y, then z;
-// the y images are implicit, so the transpose is z then x.
+// boundary rules, transposed.  Forward order x, then y, then z: the
+// transpose is z, then (non-periodic) y, then x; periodic y images are
+// implicit.
 // ---------------------------------------------------------------------------
 // z rules (f0 = 0, f1 = f2/2; f(NZ-1) = 0, f(NZ-2) = f(NZ-3)/2) on columns
 // i >= i0 (0 for bndry/bndryc/h/E, 2 for bndrytt: t3dsdsc.f:2130-2168)
 __global__ void k_adj_bc_z(Geo g, double *__restrict__ a, int nf, int i0)
 {
+    const int nr = g.yhi - g.ylo + 1;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long n = (long)g.NX * g.nyu * nf;
+    long n = (long)g.NX * nr * nf;
     if (t >= n) return;
     int i = (int)(t % g.NX);
     long r = t / g.NX;
-    int j = g.ylo + (int)(r % g.nyu);
-    int m = (int)(r / g.nyu);
+    int j = g.ylo + (int)(r % nr);
+    int m = (int)(r / nr);
     if (i < i0) return;
     double *f = a + m * g.S + i + g.sy * j;
     const long s = g.sz;
@@ -135,15 +137,38 @@ __global__ void k_adj_bc_z(Geo g, double *__restrict__ a, int nf, int i0)
     f[0] = 0.0;
 }
 
+// non-periodic y rules (f0 = 0, f1 = f2/2; f(NY-1) = 0, f(NY-2) = f(NY-3)/2)
+// on columns i >= i0 (t3dsdsc.f:1800-1819, 2592-2612; bndrytt 2108-2128)
+__global__ void k_adj_bc_y(Geo g, double *__restrict__ a, int nf, int i0)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)g.NX * g.NZ * nf;
+    if (t >= n) return;
+    int i = (int)(t % g.NX);
+    long r = t / g.NX;
+    int k = (int)(r % g.NZ);
+    int m = (int)(r / g.NZ);
+    if (i < i0) return;
+    double *f = a + m * g.S + i + g.sz * k;
+    const long s = g.sy;
+    f[(g.NY - 3) * s] += 0.5 * f[(g.NY - 2) * s];
+    f[(g.NY - 2) * s] = 0.0;
+    f[(g.NY - 1) * s] = 0.0;
+    f[2 * s] += 0.5 * f[s];
+    f[s] = 0.0;
+    f[0] = 0.0;
+}
+
 // x rules: right f(NX-1) = 0, f(NX-2) = f(NX-3)/2; left the one-sided
 // extrapolation (extrap, bndryc/h/E and bndrytt) or 0 / half (bndry)
 __global__ void k_adj_bc_x(Geo g, double *__restrict__ a, int nf, int extrap)
 {
+    const int nr = g.yhi - g.ylo + 1;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long n = (long)g.nyu * g.NZ * nf;
+    long n = (long)nr * g.NZ * nf;
     if (t >= n) return;
-    int j = g.ylo + (int)(t % g.nyu);
-    long r = t / g.nyu;
+    int j = g.ylo + (int)(t % nr);
+    long r = t / nr;
     int k = (int)(r % g.NZ);
     int m = (int)(r / g.NZ);
     double *f = a + m * g.S + g.sy * j + g.sz * k;
@@ -473,7 +498,7 @@ __global__ __launch_bounds__(256) void k_adj_objective(Geo g, const double *__re
         } else if (k == k0 || k == k0 + 1) {
             const long q0 = IDX(g, i, j, k0);
             const double bt = 0.5 * (X[q0 + 6 * S] + X[q0 + g.sz + 6 * S]);
-            const double cw = 0.5 * wx[i] / g.nyu;
+            const double cw = 0.5 * wx[i] / (g.yhi - g.ylo + 1);
             if (k == k0) v = cw * fabs(bt);
             s[6] = cw * (bt > 0.0 ? 0.5 : bt < 0.0 ? -0.5 : 0.0);
         }

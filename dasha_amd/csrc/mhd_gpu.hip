@@ -964,8 +964,12 @@ static int adj_sums(mhd_handle *h, const AdjBufs &A, double *eta_bar, double *ob
 static void adj_bc(mhd_handle *h, double *a, int nf, int type)
 {
     const Geo &g = h->g;
-    const long nz = (long)g.NX * g.nyu * nf, nx = (long)g.nyu * g.NZ * nf;
-    hipLaunchKernelGGL(k_adj_bc_z, dim3(blocks(nz, 256)), dim3(256), 0, h->stream, g, a, nf, type == BC_TT ? 2 : 0);
+    const long nr = g.yhi - g.ylo + 1;
+    const long nz = (long)g.NX * nr * nf, nx = nr * g.NZ * nf, ny = (long)g.NX * g.NZ * nf;
+    const int i0 = type == BC_TT ? 2 : 0;
+    hipLaunchKernelGGL(k_adj_bc_z, dim3(blocks(nz, 256)), dim3(256), 0, h->stream, g, a, nf, i0);
+    if (!g.periodic)
+        hipLaunchKernelGGL(k_adj_bc_y, dim3(blocks(ny, 256)), dim3(256), 0, h->stream, g, a, nf, i0);
     hipLaunchKernelGGL(k_adj_bc_x, dim3(blocks(nx, 256)), dim3(256), 0, h->stream, g, a, nf, type == BC_STATE ? 0 : 1);
 }
 
@@ -1351,7 +1355,6 @@ int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective,
     if (!h || nsteps < 1 || !dts || objective < 0 || objective > 1 || every < 0) return MHD_ERR_ARG;
     if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
     const mhd_config &c = h->cfg;
-    if (!c.periodic_y) { set_err(h, "adjoint: periodic y only"); return MHD_ERR_ARG; }
     if (c.fmu0 != 0.0 || c.frho0 != 0.0 || c.fpr0 != 0.0) {
         set_err(h, "adjoint: viscosity / diffusion terms are not differentiated");
         return MHD_ERR_ARG;
@@ -1440,8 +1443,8 @@ int mhd_adjoint_rhs(mhd_handle *h, const double *xdif_bar, double *x_bar, double
 {
     if (!h || !xdif_bar || !x_bar) return MHD_ERR_ARG;
     const mhd_config &c = h->cfg;
-    if (!c.periodic_y || c.fmu0 != 0.0 || c.frho0 != 0.0 || c.fpr0 != 0.0) {
-        set_err(h, "adjoint: periodic y and zero viscosity/diffusion only");
+    if (c.fmu0 != 0.0 || c.frho0 != 0.0 || c.fpr0 != 0.0) {
+        set_err(h, "adjoint: viscosity / diffusion terms are not differentiated");
         return MHD_ERR_ARG;
     }
     AdjBufs A;

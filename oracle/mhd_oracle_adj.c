@@ -10,7 +10,7 @@
  *     term, the pressure equation) contribute the transpose of their
  *     Jacobian at the forward state.
  * The time-step size is an input (the dt sequence is frozen), so setdt is
- * not differentiated.  Periodic y only.  Parity pins (tests/test_adjoint.py):
+ * not differentiated.  Parity pins (tests/test_adjoint.py):
  * <A u, w> = <u, A^T w> for every linear primitive against the forward
  * primitive, and central differences of right(), stepon() and multi-step
  * objectives.
@@ -81,39 +81,50 @@ static void adj_xrules(orc *o, double *a, int nf, int extrap)
             }
 }
 
-static void adj_require_periodic(const orc *o)
+/* transpose of the non-periodic y rules (f1 = 0, f2 = f3/2; f(my) = 0,
+ * f(my-1) = f(my-2)/2) over columns jx0..x_last+2 (bndryc/h/E, bndry:
+ * t3dsdsc.f:1800-1819, 2592-2612; bndrytt from x_first, 2108-2128) */
+static void adj_yrules(orc *o, double *a, int nf, int jx0)
 {
-    if (!o->c.periodic_y) {
-        fprintf(stderr, "oracle adjoint: periodic y only\n");
-        abort();
-    }
+    int m, jx, jz, my = o->ny;
+    for (m = 1; m <= nf; m++)
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jx = jx0; jx <= o->x_l + 2; jx++) {
+                FM(a, jx, my - 2, jz, m) += 0.5 * FM(a, jx, my - 1, jz, m); FM(a, jx, my - 1, jz, m) = 0;
+                FM(a, jx, my, jz, m) = 0;
+                FM(a, jx, 3, jz, m) += 0.5 * FM(a, jx, 2, jz, m); FM(a, jx, 2, jz, m) = 0;
+                FM(a, jx, 1, jz, m) = 0;
+            }
+}
+
+static void adj_y(orc *o, double *a, int nf, int jx0)
+{
+    if (o->c.periodic_y) adj_periodic_y(o, a, nf);
+    else adj_yrules(o, a, nf, jx0);
 }
 
 /* transposes of bndry_aux, bndry and bndrytt (x, then y, then z forward) */
 static void adj_bndry_aux(orc *o, double *a)
 {
-    adj_require_periodic(o);
     if (!o->c.bcs) return;
     adj_zrules(o, a, 3, o->x_f - 2);
-    adj_periodic_y(o, a, 3);
+    adj_y(o, a, 3, o->x_f - 2);
     adj_xrules(o, a, 3, 1);
 }
 
 static void adj_bndry(orc *o, double *a)
 {
-    adj_require_periodic(o);
     if (!o->c.bcs) return;
     adj_zrules(o, a, 8, o->x_f - 2);
-    adj_periodic_y(o, a, 8);
+    adj_y(o, a, 8, o->x_f - 2);
     adj_xrules(o, a, 8, 0);
 }
 
 static void adj_bndrytt(orc *o, double *a)
 {
-    adj_require_periodic(o);
     if (!o->c.bcs) return;
     adj_zrules(o, a, 8, o->x_f);
-    adj_periodic_y(o, a, 8);
+    adj_y(o, a, 8, o->x_f);
     adj_xrules(o, a, 8, 1);
 }
 
@@ -531,4 +542,7 @@ void orc_adj_residual(orc *o, const double *rb, double *xb)
 
 /* transpose of periodic_y on nf fields (folds image-row adjoints into the
  * rows they copy) */
-void orc_adj_fold_y(orc *o, double *a, int nf) { adj_periodic_y(o, a, nf); }
+void orc_adj_fold_y(orc *o, double *a, int nf)
+{
+    if (o->c.periodic_y) adj_periodic_y(o, a, nf);
+}

@@ -31,16 +31,23 @@ def _direction(x0, rng, scale=1e-3):
     return v
 
 
+@pytest.mark.parametrize("periodic", [True, False])
 @pytest.mark.parametrize("kind,nf,ck", [(0, 8, 0), (1, 8, 0), (2, 3, 0), (3, 3, 0), (3, 3, 1), (3, 3, 2)])
-def test_primitive_transposes(small, kind, nf, ck):
-    """bndry, bndrytt, bndryc/E/h and the three smoothing coefficients"""
+def test_primitive_transposes(small, kind, nf, ck, periodic):
+    """bndry, bndrytt, bndryc/E/h and the three smoothing coefficients, with
+    periodic and with non-periodic y"""
     c, o = small
+    if not periodic:
+        c = harris.case_small(16, 12, 16, periodic_y=False, seed=3)
+        o = orc.Oracle(c)
     rng = np.random.default_rng(10 + kind + ck)
     u = rng.standard_normal((nf,) + o.shape3)
     w = rng.standard_normal((nf,) + o.shape3)
     lhs = np.vdot(o.apply(kind, u, nf, ck), w)
     rhs = np.vdot(u, o.adj_apply(kind, w, nf, ck))
     assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    if not periodic:
+        o.close()
 
 
 @pytest.mark.parametrize("t,smooth", [(1.0, True), (0.0, False)])
@@ -106,13 +113,15 @@ def _run(o, x0, dts, A):
     return np.vdot(A, o.get_state())
 
 
-@pytest.mark.parametrize("nsteps,variant", [(1, "stretched"), (3, "stretched"), (2, "no_hall"), (2, "uniform")])
+@pytest.mark.parametrize("nsteps,variant", [(1, "stretched"), (3, "stretched"), (2, "no_hall"), (2, "uniform"),
+                                            (2, "nonperiodic")])
 def test_step_adjoint_central_difference(small, nsteps, variant):
     """J = <A, x_N>, residual captured from x_0, frozen dt sequence"""
     c, o = small
     if variant != "stretched":
-        c = (harris.case_small(16, 8, 16, seed=3, di=0.0) if variant == "no_hall"
-             else harris.case_small(16, 8, 20, stretched=False, seed=3))
+        c = {"no_hall": lambda: harris.case_small(16, 8, 16, seed=3, di=0.0),
+             "uniform": lambda: harris.case_small(16, 8, 20, stretched=False, seed=3),
+             "nonperiodic": lambda: harris.case_small(16, 12, 16, periodic_y=False, seed=3)}[variant]()
         o = orc.Oracle(c)
     rng = np.random.default_rng(5 + nsteps)
     x0 = c.x0.copy()
@@ -160,10 +169,11 @@ def test_control_map_transposes():
     assert abs(np.vdot(dx0, gx) - np.vdot(ay, adjoint.ay_gradient(ac.case, gx))) <= 1e-11 * abs(np.vdot(dx0, gx))
 
 
+@pytest.mark.parametrize("periodic", [True, False])
 @pytest.mark.parametrize("obj", ["magnetic_energy", "reconnected_flux"])
-def test_objective_seed_central_difference(obj):
+def test_objective_seed_central_difference(obj, periodic):
     from dasha_amd import adjoint
-    ac = adjoint.small_case(16, 8, 16)
+    ac = adjoint.small_case(16, 8 if periodic else 12, 16, periodic_y=periodic)
     x = ac.state()
     rng = np.random.default_rng(4)
     v = rng.standard_normal(x.shape) * 1e-3
