@@ -105,9 +105,11 @@ def case_c5(steps: int = 500, mx: int = 128, ny: int = 64, mz: int = 128, *, amp
     return AdjointCase(case=base, ay0=ay0, steps=steps)
 
 
-def small_case(mx: int = 24, ny: int = 8, mz: int = 24, steps: int = 3, **kw) -> AdjointCase:
-    """a small stretched case of the same form (tests)"""
-    base = harris.case_small(mx, ny, mz, amp=0.0, **kw)
+def small_case(mx: int = 24, ny: int = 8, mz: int = 24, steps: int = 3, *, fmu0: float = 0.0,
+               frho0: float = 0.0, fpr0: float = 0.0, **kw) -> AdjointCase:
+    """a small stretched case of the same form (tests); fmu0, frho0, fpr0
+    switch on viscosity and diffusion"""
+    base = dataclasses.replace(harris.case_small(mx, ny, mz, amp=0.0, **kw), fmu0=fmu0, frho0=frho0, fpr0=fpr0)
     px, pz = physical_coords(base)
     ay0 = 0.1 * np.exp(-(px[None, :] ** 2 + pz[:, None] ** 2))
     return AdjointCase(case=base, ay0=ay0, steps=steps)

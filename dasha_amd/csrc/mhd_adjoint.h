@@ -91,6 +91,26 @@ __device__ inline double dTz(const Geo &g, int i, int j, int k, V &&v)
     return s;
 }
 
+// (L^T v)(q) of the 5-point Laplacian D2X + D2Y + D2Z (second-derivative
+// coefficients), v supported on the interior
+template <class V>
+__device__ inline double lapT(const Geo &g, int i, int j, int k, V &&v)
+{
+    double s = 0.0;
+    const bool xin = ax_in(i, g.NX), yin = g.periodic || ax_in(j, g.NY), zin = ax_in(k, g.NZ);
+#pragma unroll
+    for (int d = -2; d <= 2; d++) {
+        const int px = i - d, py = rmap(g, j - d), pz = k - d;
+        if (yin && zin && px >= 2 && px <= g.NX - 3)
+            s += adj_dw(d, g.x2a[px], g.x2b[px], g.x2c[px], g.x2d[px]) * v(px, j, k);
+        if (xin && zin && (g.periodic || ax_in(py, g.NY)))
+            s += adj_dw(d, g.y2a[py], g.y2b[py], g.y2c[py], g.y2d[py]) * v(i, py, k);
+        if (xin && yin && pz >= 2 && pz <= g.NZ - 3)
+            s += adj_dw(d, g.z2a[pz], g.z2b[pz], g.z2c[pz], g.z2d[pz]) * v(i, j, pz);
+    }
+    return s;
+}
+
 // forward first derivatives of a field at an interior point
 __device__ inline double fdx(const Geo &g, const double *f, int i, int j, int k)
 {
@@ -322,6 +342,50 @@ __global__ __launch_bounds__(256) void k_adj_curle_p(Geo g, const double *__rest
         xb[q + 7 * S] += p8;
     }
     block_add(es, etab);
+}
+
+// ---------------------------------------------------------------------------
+// viscosity (t3dsdsc.f:1585-1621): xdif(2..4) += fmu L(v), xdif8 +=
+// fmu v.L(v); diffusion (1623-1663): xdif1 += frho L(rho), xdif8 += fpr
+// L(p); transposed.  vb += adjoint of v, xb(0), xb(7) += .
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_adj_visc(Geo g, const double *__restrict__ ab,
+                                                  const double *__restrict__ X, double fmu, double frho,
+                                                  double fpr, double *__restrict__ vb, double *__restrict__ xb)
+{
+    AdjPt pt = adj_point(g);
+    if (!pt.ok) return;
+    const int i = pt.i, j = pt.j, k = pt.k;
+    const long S = g.S, q = IDX(g, i, j, k);
+    const double *a8 = ab + 7 * S;
+    auto vel = [&](long r, int m) { return X[r + (m + 1) * S] / X[r]; };
+    if (fmu != 0.0) {
+        const bool qi = in3(g, i, j, k);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const double *am = ab + (m + 1) * S;
+            double t = lapT(g, i, j, k, [&](int x, int y, int z) {
+                const long r = IDX(g, x, y, z);
+                return fmu * (am[r] + a8[r] * vel(r, m));
+            });
+            if (qi) {
+                const long b = i + g.sz * k, sz = g.sz;
+                const double lx = fd5(vel(q - 2, m), vel(q - 1, m), vel(q, m), vel(q + 1, m), vel(q + 2, m),
+                                      g.x2a[i], g.x2b[i], g.x2c[i], g.x2d[i]);
+                const double ly = fd5(vel(b + g.sy * rmap(g, j - 2), m), vel(b + g.sy * rmap(g, j - 1), m), vel(q, m),
+                                      vel(b + g.sy * rmap(g, j + 1), m), vel(b + g.sy * rmap(g, j + 2), m),
+                                      g.y2a[j], g.y2b[j], g.y2c[j], g.y2d[j]);
+                const double lz = fd5(vel(q - 2 * sz, m), vel(q - sz, m), vel(q, m), vel(q + sz, m), vel(q + 2 * sz, m),
+                                      g.z2a[k], g.z2b[k], g.z2c[k], g.z2d[k]);
+                t += fmu * a8[q] * (lx + ly + lz);
+            }
+            vb[q + m * S] += t;
+        }
+    }
+    if (frho != 0.0 || fpr != 0.0) {
+        xb[q] += lapT(g, i, j, k, [&](int x, int y, int z) { return frho * ab[IDX(g, x, y, z)]; });
+        xb[q + 7 * S] += lapT(g, i, j, k, [&](int x, int y, int z) { return fpr * a8[IDX(g, x, y, z)]; });
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -1000,6 +1000,9 @@ static void adj_rhs(mhd_handle *h, AdjBufs &A, const double *X, const double *C,
     adj_bc(h, a, 8, BC_STATE);                                         // bndry (1665)
     hipLaunchKernelGGL(k_adj_curle_p, grid, blk, 0, h->stream, g, a, X, C, c.gamma, eta, A.eb, A.vb, A.jb, xb,
                        A.acc);
+    const double fmu = init ? 0.0 : c.fmu0, frho = init ? 0.0 : c.frho0, fpr = init ? 0.0 : c.fpr0;
+    if (fmu != 0.0 || frho != 0.0 || fpr != 0.0)   // viscosity, diffusion (1585-1663)
+        hipLaunchKernelGGL(k_adj_visc, grid, blk, 0, h->stream, g, a, X, fmu, frho, fpr, A.vb, xb);
     if (c.hall) {
         HCHK_V(hipMemcpyAsync(A.hb, A.eb, 3 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
         adj_smooth<KIND_EIGHTH>(h, A.hb, A.t3, 3, 5, BC_AUX);         // avrgh x5 (1424)
@@ -1355,10 +1358,6 @@ int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective,
     if (!h || nsteps < 1 || !dts || objective < 0 || objective > 1 || every < 0) return MHD_ERR_ARG;
     if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
     const mhd_config &c = h->cfg;
-    if (c.fmu0 != 0.0 || c.frho0 != 0.0 || c.fpr0 != 0.0) {
-        set_err(h, "adjoint: viscosity / diffusion terms are not differentiated");
-        return MHD_ERR_ARG;
-    }
     for (int n = 0; n < nsteps; n++)
         if (!(dts[n] > 0.0)) { set_err(h, "adjoint: dt must be positive"); return MHD_ERR_ARG; }
     const int K = every > 0 ? std::min(every, nsteps) : (int)std::ceil(std::sqrt((double)nsteps));
@@ -1442,11 +1441,6 @@ int mhd_adjoint_rhs(mhd_handle *h, const double *xdif_bar, double *x_bar, double
                     int init, int smooth)
 {
     if (!h || !xdif_bar || !x_bar) return MHD_ERR_ARG;
-    const mhd_config &c = h->cfg;
-    if (c.fmu0 != 0.0 || c.frho0 != 0.0 || c.fpr0 != 0.0) {
-        set_err(h, "adjoint: viscosity / diffusion terms are not differentiated");
-        return MHD_ERR_ARG;
-    }
     AdjBufs A;
     int r = adj_alloc(h, A, 0, 0);
     if (!r) r = upload_fields(h, A.ab, xdif_bar, 8);

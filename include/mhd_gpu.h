@@ -172,7 +172,7 @@ int mhd_ensemble_tune(mhd_ensemble *e, int trial, const double *dt0, const doubl
  * each segment is replayed recording stage states and currents, then
  * traversed backwards.  flags & MHD_ADJ_RESIDUAL: x_0's residual capture
  * (mhd_init_residual) is differentiated too; otherwise xresd is a constant.
- * Zero viscosity/diffusion only (fmu0 = frho0 = fpr0 = 0).  Any of value, grad_x,
+ * Any of value, grad_x,
  * grad_eta, ms may be NULL.  On return the state is x_N. */
 enum { MHD_ADJ_RESIDUAL = 1 };
 /* stage-level access for parity checks of the adjoint: the vector-Jacobian

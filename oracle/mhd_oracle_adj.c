@@ -223,6 +223,24 @@ static double adj_dz(const orc *o, const double *g, int jx, int jy, int jz)
     return s;
 }
 
+/* (L^T g)(q) of the 5-point Laplacian D2X + D2Y + D2Z (second-derivative
+ * coefficients) for g supported on the interior */
+static double adj_lap(const orc *o, const double *g, int jx, int jy, int jz)
+{
+    double s = 0;
+    int p;
+    for (p = jx - 2; p <= jx + 2; p++)
+        if (p >= o->x_f && p <= o->x_l)
+            s += adj_dw(jx - p, V1(o->ax2, p), V1(o->bx2, p), V1(o->cx2, p), V1(o->dx2, p)) * F(g, p, jy, jz);
+    for (p = jy - 2; p <= jy + 2; p++)
+        if (p >= 3 && p <= o->ny - 2)
+            s += adj_dw(jy - p, V1(o->ay2, p), V1(o->by2, p), V1(o->cy2, p), V1(o->dy2, p)) * F(g, jx, p, jz);
+    for (p = jz - 2; p <= jz + 2; p++)
+        if (p >= o->z_f && p <= o->z_l)
+            s += adj_dw(jz - p, V1(o->az2, p), V1(o->bz2, p), V1(o->cz2, p), V1(o->dz2, p)) * F(g, jx, jy, p);
+    return s;
+}
+
 #define ADJ_INT_LOOP for (jz = o->z_f; jz <= o->z_l; jz++) for (jy = 3; jy <= o->ny - 2; jy++) \
                      for (jx = o->x_f; jx <= o->x_l; jx++)
 #define ADJ_EXT_LOOP for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++) for (jy = 1; jy <= o->ny; jy++) \
@@ -245,10 +263,7 @@ static void adj_right(orc *o, const double *X, const double *Cu, double *ab, dou
     const double *a1 = ab, *a2 = ab + n3, *a3 = ab + 2 * n3, *a4 = ab + 3 * n3;
     const double *a5 = ab + 4 * n3, *a6 = ab + 5 * n3, *a7 = ab + 6 * n3, *a8 = ab + 7 * n3;
     const double *x8 = X + 7 * n3;
-    if (o->c.fmu0 != 0.0 || o->c.frho0 != 0.0 || o->c.fpr0 != 0.0) {
-        fprintf(stderr, "oracle adjoint: viscosity / diffusion terms are not differentiated\n");
-        abort();
-    }
+    const double fmu = init ? 0.0 : o->fmu, frho = init ? 0.0 : o->frho, fpr = init ? 0.0 : o->fpr;
     /* final avrg (1669) and bndry (1665) */
     if (smooth) adj_avrg_state(o, ab, 2, 0, w);
     adj_bndry(o, ab);
@@ -307,6 +322,41 @@ static void adj_right(orc *o, const double *X, const double *Cu, double *ab, dou
         FM(jb, jx, jy, jz, 2) += a * (gam - 1.) * eta * 2. * c2;
         FM(jb, jx, jy, jz, 3) += a * (gam - 1.) * eta * 2. * c3;
         es += a * (gam - 1.) * (c1 * c1 + c2 * c2 + c3 * c3);
+    }
+
+    /* viscosity (1585-1621): xdif(2..4) += fmu L(v), xdif8 += fmu v.L(v);
+     * diffusion (1623-1663): xdif1 += frho L(rho), xdif8 += fpr L(p) */
+    if (fmu != 0.0) {
+        const double *vv[3] = {fs, gs, hs}, *aa[3] = {a2, a3, a4};
+        for (m = 0; m < 3; m++) {
+            memset(g, 0, sizeof(double) * n3);
+            ADJ_INT_LOOP F(g1, jx, jy, jz) = fmu * (F(aa[m], jx, jy, jz) + F(a8, jx, jy, jz) * F(vv[m], jx, jy, jz));
+#pragma omp parallel for private(jy, jx)
+            for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+                for (jy = 1; jy <= o->ny; jy++)
+                    for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                        double t = adj_lap(o, g1, jx, jy, jz);
+                        if (adj_interior(o, jx, jy, jz)) {
+                            const double *f = vv[m];
+                            t += fmu * F(a8, jx, jy, jz) * (D2X(f) + D2Y(f) + D2Z(f));
+                        }
+                        FM(vb, jx, jy, jz, m + 1) += t;
+                    }
+        }
+    }
+    if (frho != 0.0 || fpr != 0.0) {
+        memset(g, 0, sizeof(double) * 2 * n3);
+        ADJ_INT_LOOP {
+            F(g1, jx, jy, jz) = frho * F(a1, jx, jy, jz);
+            F(g2, jx, jy, jz) = fpr * F(a8, jx, jy, jz);
+        }
+#pragma omp parallel for private(jy, jx)
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jy = 1; jy <= o->ny; jy++)
+                for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++) {
+                    FM(xb, jx, jy, jz, 1) += adj_lap(o, g1, jx, jy, jz);
+                    FM(xb, jx, jy, jz, 8) += adj_lap(o, g2, jx, jy, jz);
+                }
     }
 
     /* Hall term (1391-1433): E += avrg(bndryh(xdh)) with

@@ -114,14 +114,16 @@ def _run(o, x0, dts, A):
 
 
 @pytest.mark.parametrize("nsteps,variant", [(1, "stretched"), (3, "stretched"), (2, "no_hall"), (2, "uniform"),
-                                            (2, "nonperiodic")])
+                                            (2, "nonperiodic"), (2, "viscous")])
 def test_step_adjoint_central_difference(small, nsteps, variant):
     """J = <A, x_N>, residual captured from x_0, frozen dt sequence"""
     c, o = small
     if variant != "stretched":
         c = {"no_hall": lambda: harris.case_small(16, 8, 16, seed=3, di=0.0),
              "uniform": lambda: harris.case_small(16, 8, 20, stretched=False, seed=3),
-             "nonperiodic": lambda: harris.case_small(16, 12, 16, periodic_y=False, seed=3)}[variant]()
+             "nonperiodic": lambda: harris.case_small(16, 12, 16, periodic_y=False, seed=3),
+             "viscous": lambda: dataclasses.replace(harris.case_small(16, 8, 16, seed=3), fmu0=2e-3,
+                                                    frho0=1e-3, fpr0=1.5e-3)}[variant]()
         o = orc.Oracle(c)
     rng = np.random.default_rng(5 + nsteps)
     x0 = c.x0.copy()

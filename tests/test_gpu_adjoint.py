@@ -10,6 +10,8 @@ central differences of the HIP forward step.
 * d/d(dAy) and d/deta == central differences of the GPU forward run;
 * the torch.autograd.Function returns the library's gradients.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -38,7 +40,8 @@ def _unique_rows(a, periodic=True):
 @pytest.mark.parametrize("mode,variant", [("smooth", "stretched"), ("nosmooth", "stretched"),
                                           ("init", "stretched"), ("smooth", "no_hall"),
                                           ("smooth", "uniform"), ("smooth", "ny12"),
-                                          ("smooth", "nonperiodic"), ("init", "nonperiodic")])
+                                          ("smooth", "nonperiodic"), ("init", "nonperiodic"),
+                                          ("smooth", "viscous")])
 def test_rhs_vjp_matches_oracle(mode, variant):
     if variant == "no_hall":
         c = harris.case_small(24, 8, 24, seed=11, di=0.0)
@@ -48,6 +51,8 @@ def test_rhs_vjp_matches_oracle(mode, variant):
         c = harris.case_small(24, 12, 24, seed=11)
     elif variant == "nonperiodic":
         c = harris.case_small(24, 12, 20, periodic_y=False, seed=11)
+    elif variant == "viscous":
+        c = dataclasses.replace(harris.case_small(24, 8, 24, seed=11), fmu0=2e-3, frho0=1e-3, fpr0=1.5e-3)
     else:
         c = harris.case_small(24, 8, 24, seed=11)
     assert c.hall == (variant != "no_hall")
@@ -81,11 +86,13 @@ def test_rhs_vjp_matches_oracle(mode, variant):
         o.close()
 
 
-@pytest.mark.parametrize("obj,periodic", [("magnetic_energy", True), ("reconnected_flux", True),
-                                          ("magnetic_energy", False), ("reconnected_flux", False)])
-def test_adjoint_run_matches_oracle(obj, periodic):
-    ac = (adjoint.small_case(24, 8, 24, seed=12) if periodic
-          else adjoint.small_case(24, 12, 24, seed=12, periodic_y=False))
+@pytest.mark.parametrize("obj,variant", [("magnetic_energy", "periodic"), ("reconnected_flux", "periodic"),
+                                         ("magnetic_energy", "nonperiodic"), ("reconnected_flux", "nonperiodic"),
+                                         ("magnetic_energy", "viscous")])
+def test_adjoint_run_matches_oracle(obj, variant):
+    ac = {"periodic": lambda: adjoint.small_case(24, 8, 24, seed=12),
+          "nonperiodic": lambda: adjoint.small_case(24, 12, 24, seed=12, periodic_y=False),
+          "viscous": lambda: adjoint.small_case(24, 8, 24, seed=12, fmu0=2e-3, frho0=1e-3, fpr0=1.5e-3)}[variant]()
     dts = [0.0050, 0.0052, 0.0049]
     print(f"[adjoint] run {obj} {ac.case.name} {len(dts)} steps", flush=True)
     orc.build()

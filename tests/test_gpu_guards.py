@@ -91,10 +91,11 @@ def test_bad_arguments():
     g.close()
     with pytest.raises(MHDError):
         MHDStep(dataclasses.replace(c, mx=6))   # fewer than 8 points
-    visc = dataclasses.replace(harris.case_small(24, 8, 24, seed=35), fmu0=1e-3)
-    h = MHDStep(visc)
+    h = MHDStep(c)
     h.init_residual()
     with pytest.raises(MHDError) as e:
-        h.adjoint_run([0.005], "magnetic_energy")   # viscosity is not differentiated
-    assert e.value.code == -1 and "viscosity" in str(e.value)
+        h.adjoint_run([0.005], 7)                   # no such objective
+    assert e.value.code == -1
+    with pytest.raises(MHDError):
+        h.adjoint_run([0.005, -1.0], 0)             # dt must be positive
     h.close()
