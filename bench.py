@@ -174,13 +174,25 @@ def time_adjoint(steps: int = 500):
     val, gay, geta, (fwd_ms, rev_ms) = adjoint.objective(ac, dts, obj="magnetic_energy")
     print(f"[bench] adjoint: forward {fwd_ms:.1f} ms, reverse {rev_ms:.1f} ms "
           f"(wall {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    # the same gradient with every step's stage states kept in HBM (44 fields
+    # per step: about 196 GB at 128x64x128 x 500 steps), nothing recomputed
+    from dasha_amd.mhd import MHDError
+    try:
+        v2, gay2, ge2, (fwd2, rev2) = adjoint.objective(ac, dts, obj="magnetic_energy", every=-1)
+        same = bool(v2 == val and ge2 == geta and (gay2 == gay).all())
+        store_all = {"forward_ms": fwd2, "reverse_ms": rev2, "ratio_reverse_to_forward": rev2 / fwd2,
+                     "gradients_identical": same}
+        print(f"[bench] adjoint, every step kept: forward {fwd2:.1f} ms, reverse {rev2:.1f} ms", file=sys.stderr,
+              flush=True)
+    except MHDError as e:
+        store_all = {"error": str(e)}
     return {"workload": f"{ac.case.name} stretched Harris sheet, eta {ac.case.eta0}, d_i {ac.case.di}, "
                         f"{steps} RK4 steps, objective magnetic energy at T, controls dAy (mz x mx) and eta",
             "steps": steps, "checkpoint_every": adjoint.sqrt_checkpoints(steps),
             "forward_ms": fwd_ms, "reverse_ms": rev_ms,
             "forward_ms_per_step": fwd_ms / steps, "reverse_ms_per_step": rev_ms / steps,
             "ratio_reverse_to_forward": rev_ms / fwd_ms, "objective": val, "grad_eta": geta,
-            "grad_ay_norm": float((gay ** 2).sum() ** 0.5)}
+            "grad_ay_norm": float((gay ** 2).sum() ** 0.5), "every_step_kept": store_all}
 
 
 def main():

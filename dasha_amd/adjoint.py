@@ -131,14 +131,15 @@ def with_eta(ac: AdjointCase, eta: float) -> AdjointCase:
     return dataclasses.replace(ac, case=dataclasses.replace(ac.case, eta0=float(eta)))
 
 
-def objective(ac: AdjointCase, dts, ay=None, eta=None, obj="magnetic_energy"):
-    """(value, d value/d ay, d value/d eta, (forward ms, reverse ms))"""
+def objective(ac: AdjointCase, dts, ay=None, eta=None, obj="magnetic_energy", every: int = 0):
+    """(value, d value/d ay, d value/d eta, (forward ms, reverse ms));
+    every: checkpoint spacing (0: ceil(sqrt(N)), < 0: keep every step)"""
     c = ac if eta is None else with_eta(ac, eta)
     g = MHDStep(c.case)
     try:
         g.set_state(c.state(ay))
         g.init_residual()
-        val, gx, ge, ms = g.adjoint_run(dts, obj)
+        val, gx, ge, ms = g.adjoint_run(dts, obj, every=every)
     finally:
         g.close()
     return val, ay_gradient(c.case, gx), ge, ms

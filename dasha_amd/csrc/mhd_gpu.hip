@@ -941,7 +941,8 @@ static int adj_alloc(mhd_handle *h, AdjBufs &A, int nck, int K)
             set_err(h, "adjoint: device allocation failed");
             return MHD_ERR_NOMEM;
         }
-        HCHK(hipMemsetAsync(*x.p, 0, x.bytes, h->stream));
+        // checkpoints and tape are written before they are read
+        if (x.p != &A.ck && x.p != &A.tx && x.p != &A.tc) HCHK(hipMemsetAsync(*x.p, 0, x.bytes, h->stream));
     }
     return 0;
 }
@@ -1355,12 +1356,15 @@ int mhd_get_current(mhd_handle *h, double *cur)
 int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective, int every, int flags,
                     double *value, double *grad_x, double *grad_eta, double *ms)
 {
-    if (!h || nsteps < 1 || !dts || objective < 0 || objective > 1 || every < 0) return MHD_ERR_ARG;
+    if (!h || nsteps < 1 || !dts || objective < 0 || objective > 1) return MHD_ERR_ARG;
     if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
     const mhd_config &c = h->cfg;
     for (int n = 0; n < nsteps; n++)
         if (!(dts[n] > 0.0)) { set_err(h, "adjoint: dt must be positive"); return MHD_ERR_ARG; }
-    const int K = every > 0 ? std::min(every, nsteps) : (int)std::ceil(std::sqrt((double)nsteps));
+    // every < 0: the forward pass records every step (no recomputation)
+    const bool store_all = every < 0;
+    const int K = store_all ? nsteps
+                            : every > 0 ? std::min(every, nsteps) : (int)std::ceil(std::sqrt((double)nsteps));
     const int nck = (nsteps + K - 1) / K;
     // the two planes nearest z = 0 (the reconnection line of the sheet)
     int k0 = -1;
@@ -1396,9 +1400,15 @@ int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective,
     for (int n = 0; n < nsteps; n++) {
         if (n % K == 0)
             HCHK(hipMemcpyAsync(A.ck + (long)(n / K) * 8 * h->S, h->x, b8, hipMemcpyDeviceToDevice, h->stream));
+        if (store_all) {
+            h->rec_x = A.tx + (long)n * 4 * 8 * h->S;
+            h->rec_c = A.tc + (long)n * 4 * 3 * h->S;
+            h->rec_n = 0;
+        }
         hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, h->stream, h->dtv, dts[n]);
         step_body(h);
     }
+    h->rec_x = h->rec_c = nullptr;
     HCHK(hipEventRecord(e[1], h->stream));
     HCHK(hipMemcpyAsync(A.xfin, h->x, b8, hipMemcpyDeviceToDevice, h->stream));
     hipLaunchKernelGGL(k_adj_objective, adj_grid(h), dim3(64, 4), 0, h->stream, h->g, h->x, objective, k0,
@@ -1408,7 +1418,7 @@ int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective,
     for (int ci = nck - 1; ci >= 0; ci--) {
         const int n0 = ci * K, n1 = std::min(nsteps, n0 + K);
         HCHK(hipMemcpyAsync(h->x, A.ck + (long)ci * 8 * h->S, b8, hipMemcpyDeviceToDevice, h->stream));
-        for (int n = n0; n < n1; n++) {
+        for (int n = n0; n < n1 && !store_all; n++) {
             h->rec_x = A.tx + (long)(n - n0) * 4 * 8 * h->S;
             h->rec_c = A.tc + (long)(n - n0) * 4 * 3 * h->S;
             h->rec_n = 0;

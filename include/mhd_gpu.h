@@ -170,7 +170,9 @@ int mhd_ensemble_tune(mhd_ensemble *e, int trial, const double *dt0, const doubl
  *             (recomputation included).
  * Reverse sweep: states are kept every `every` steps (0: ceil(sqrt(nsteps)));
  * each segment is replayed recording stage states and currents, then
- * traversed backwards.  flags & MHD_ADJ_RESIDUAL: x_0's residual capture
+ * traversed backwards.  every < 0: the forward pass records the stage states
+ * and currents of every step (44 fields per step of device memory) and
+ * nothing is recomputed.  flags & MHD_ADJ_RESIDUAL: x_0's residual capture
  * (mhd_init_residual) is differentiated too; otherwise xresd is a constant.
  * Any of value, grad_x,
  * grad_eta, ms may be NULL.  On return the state is x_N. */

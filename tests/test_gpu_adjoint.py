@@ -116,7 +116,7 @@ def test_checkpoint_segments_agree():
     dts = [0.005] * 7
     print(f"[adjoint] checkpoint segments {ac.case.name}", flush=True)
     out = []
-    for every in (1, 3, 7, 0):
+    for every in (1, 3, 7, 0, -1):
         g = _mhd()(ac.case)
         try:
             g.set_state(ac.state())
