@@ -145,8 +145,8 @@ def objective(ac: AdjointCase, dts, ay=None, eta=None, obj="magnetic_energy", ev
     return val, ay_gradient(c.case, gx), ge, ms
 
 
-def forward_value(ac: AdjointCase, dts, ay=None, eta=None, obj="magnetic_energy") -> float:
-    """the objective alone, from a plain forward run (finite differences)"""
+def forward_state(ac: AdjointCase, dts, ay=None, eta=None) -> np.ndarray:
+    """the state after a plain forward run over the frozen dt sequence"""
     c = ac if eta is None else with_eta(ac, eta)
     g = MHDStep(c.case)
     try:
@@ -156,10 +156,42 @@ def forward_value(ac: AdjointCase, dts, ay=None, eta=None, obj="magnetic_energy"
         for dt in dts:
             g.step(t, dt)
             t += dt
-        x = g.get_state()
+        return g.get_state()
     finally:
         g.close()
-    return objective_of_state(c.case, x, obj)
+
+
+def forward_value(ac: AdjointCase, dts, ay=None, eta=None, obj="magnetic_energy") -> float:
+    """the objective alone, from a plain forward run (finite differences)"""
+    return objective_of_state(ac.case, forward_state(ac, dts, ay, eta), obj)
+
+
+def fd_direction(ac: AdjointCase, obj: str, seed: int = 7) -> np.ndarray:
+    """a direction in dAy for central differences, scaled to max |v| = 1.
+    magnetic_energy: white noise; reconnected_flux: the control's shape with
+    a +-50% random modulation, which keeps the sign of Bz~ at every point
+    under a small step (|Bz~| has a kink where Bz~ = 0)"""
+    rng = np.random.default_rng(seed)
+    if obj == "magnetic_energy":
+        v = rng.standard_normal(ac.ay0.shape)
+    else:
+        v = ac.ay0 * (1.0 + 0.5 * rng.uniform(-1.0, 1.0, ac.ay0.shape))
+    return v / np.abs(v).max()
+
+
+def central_difference(case, xp: np.ndarray, xm: np.ndarray, h: float, obj="magnetic_energy") -> float:
+    """(J(xp) - J(xm)) / 2h, the difference taken inside one weighted sum
+    (1/2 w (b+ - b-)(b+ + b-), or w (|bz+| - |bz-|)) instead of between two
+    sums of ~1e6 terms, whose own rounding (~1e-12 of J) would otherwise
+    dominate the quotient at h = 1e-6"""
+    wx, rows, wy, wz, k0 = _weights(case)
+    if obj == "magnetic_energy":
+        bp, bm = xp[4:7, 2:-2, rows, 2:-2], xm[4:7, 2:-2, rows, 2:-2]
+        w = wz[:, None, None] * wy[None, :, None] * wx[None, None, :]
+        return float(0.5 * np.sum(w[None] * (bp - bm) * (bp + bm))) / (2.0 * h)
+    zp = 0.5 * (xp[6, 2 + k0, rows, 2:-2] + xp[6, 3 + k0, rows, 2:-2])
+    zm = 0.5 * (xm[6, 2 + k0, rows, 2:-2] + xm[6, 3 + k0, rows, 2:-2])
+    return float(np.sum(0.5 * wx[None, :] * (np.abs(zp) - np.abs(zm))) / case.ny) / (2.0 * h)
 
 
 def _trapezoid(c: np.ndarray) -> np.ndarray:

@@ -241,3 +241,55 @@ def test_oracle_ay_gradient_central_difference(obj):
     eps = 1e-3
     fd = (J(ac.ay0 + eps * v) - J(ac.ay0 - eps * v)) / (2 * eps)
     assert abs(fd - np.vdot(gay, v)) <= 1e-7 * abs(fd)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_central_difference_closed_forms(periodic):
+    """adjoint.central_difference: the magnetic energy is quadratic, so its
+    central difference along d is exactly sum w b.d for any h; the flux is
+    |Bz~|, linear while no sign changes, so its quotient is seed.d"""
+    from dasha_amd import adjoint
+    ac = adjoint.small_case(16, 8 if periodic else 12, 16, periodic_y=periodic)
+    x = ac.state()
+    rng = np.random.default_rng(9)
+    d = rng.standard_normal(x.shape)
+    for h in (1e-6, 1e-2):
+        for obj in ("magnetic_energy", "reconnected_flux"):
+            q = adjoint.central_difference(ac.case, x + h * d, x - h * d, h, obj)
+            seed = adjoint.objective_seed(ac.case, x, obj)
+            exact = np.vdot(seed, d)
+            if obj == "reconnected_flux" and h > 1e-4:
+                continue            # large steps cross the zeros of Bz~
+            # x +- h d rounds each term by ~1e-16 |x| / (h |d|)
+            assert abs(q - exact) <= 1e-9 * np.vdot(np.abs(seed), np.abs(d)), (h, obj)
+            assert adjoint.central_difference(ac.case, x, x, h, obj) == 0.0
+    # the same quotient as the difference of two objective evaluations, to
+    # the latter's rounding (two sums of O(1) that cancel to O(1e-3))
+    xp, xm = x + 1e-3 * d, x - 1e-3 * d
+    for obj in ("magnetic_energy", "reconnected_flux"):
+        two = (adjoint.objective_of_state(ac.case, xp, obj) - adjoint.objective_of_state(ac.case, xm, obj)) / 2e-3
+        assert adjoint.central_difference(ac.case, xp, xm, 1e-3, obj) == pytest.approx(two, rel=1e-8)
+
+
+GOLDEN_FD = __import__("pathlib").Path(__file__).parent / "golden" / "c5_fd_oracle.json"
+
+
+@pytest.mark.skipif(not GOLDEN_FD.exists(), reason="c5_fd_oracle.json not generated")
+def test_c5_fd_fixture_is_consistent():
+    """the oracle's central differences at 128x64x128 (make_c5_fd_oracle.py):
+    the directions regenerate from their seed, the dt sequence is the full
+    run's, and the differenced quotient agrees with the two-sum one to the
+    latter's rounding"""
+    import json
+    from dasha_amd import adjoint
+    fx = json.loads(GOLDEN_FD.read_text())
+    ac = adjoint.case_c5()
+    assert fx["h"] == 1e-6 and len(fx["dts"]) == ac.steps
+    dts = np.array(fx["dts"])
+    assert (dts > 0).all() and dts.max() / dts.min() < 2.0
+    for obj, (s1, s2) in fx["direction_checksum"].items():
+        v = adjoint.fd_direction(ac, obj, fx["seed"])
+        assert np.sum(v) == pytest.approx(s1, rel=1e-12) and np.sum(v * v) == pytest.approx(s2, rel=1e-12)
+    for key in ("fd_ay", "fd_eta"):
+        for obj, q in fx[key].items():
+            assert q == pytest.approx(fx[key + "_two_sums"][obj], rel=5e-6)

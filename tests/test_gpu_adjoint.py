@@ -7,10 +7,14 @@ central differences of the HIP forward step.
 * mhd_adjoint_run over a few steps (both objectives, residual captured from
   x0): value, d/dx0 and d/deta == the oracle's reverse sweep;
 * checkpointing: every segment length gives the same gradient;
-* d/d(dAy) and d/deta == central differences of the GPU forward run;
+* d/d(dAy) and d/deta == central differences of the GPU forward run, and,
+  on the full 128x64x128 x 500-step workload, of the CPU oracle (a fixture
+  made by tests/golden/make_c5_fd_oracle.py, step 1e-6);
 * the torch.autograd.Function returns the library's gradients.
 """
 import dataclasses
+import json
+import pathlib
 
 import numpy as np
 import pytest
@@ -180,26 +184,50 @@ def test_torch_function():
 @pytest.mark.parametrize("obj", ["magnetic_energy", "reconnected_flux"])
 def test_c5_central_difference(obj):
     """the full adjoint workload: 128x64x128 stretched, eta 0.005, d_i 0.15,
-    500 steps, sqrt(N) checkpoints; gradients to 1e-6 of central differences"""
+    500 steps, sqrt(N) checkpoints; gradients to 1e-6 of central differences
+    of the GPU forward with step h = 1e-6 (dAy +- h v, max |v| = 1; eta +- h)"""
     ac = adjoint.case_c5()
     print(f"[adjoint] C5 central differences {obj} {ac.case.name} {ac.steps} steps", flush=True)
     dts = adjoint.dt_sequence(ac)
     val, gay, geta, ms = adjoint.objective(ac, dts, obj=obj)
     print(f"forward {ms[0]:.1f} ms reverse {ms[1]:.1f} ms", flush=True)
     assert ms[1] <= 4.0 * ms[0]
-    rng = np.random.default_rng(7)
-    if obj == "magnetic_energy":
-        v = rng.standard_normal(ac.ay0.shape) * 0.01
-    else:
-        v = ac.ay0 * (1.0 + 0.5 * rng.uniform(-1.0, 1.0, ac.ay0.shape))
-    eps = 1e-4
-    fd = (adjoint.forward_value(ac, dts, ac.ay0 + eps * v, obj=obj) -
-          adjoint.forward_value(ac, dts, ac.ay0 - eps * v, obj=obj)) / (2 * eps)
+    h = 1e-6
+    v = adjoint.fd_direction(ac, obj)
+    fd = adjoint.central_difference(ac.case, adjoint.forward_state(ac, dts, ac.ay0 + h * v),
+                                    adjoint.forward_state(ac, dts, ac.ay0 - h * v), h, obj)
     ad = float(np.vdot(gay, v))
     print("ay", fd, ad, abs(fd - ad) / abs(fd), flush=True)
     assert abs(fd - ad) <= 1e-6 * abs(fd)
-    e = 1e-3 * ac.case.eta0
-    fde = (adjoint.forward_value(ac, dts, eta=ac.case.eta0 + e, obj=obj) -
-           adjoint.forward_value(ac, dts, eta=ac.case.eta0 - e, obj=obj)) / (2 * e)
+    e0 = ac.case.eta0
+    fde = adjoint.central_difference(ac.case, adjoint.forward_state(ac, dts, eta=e0 + h),
+                                     adjoint.forward_state(ac, dts, eta=e0 - h), h, obj)
+    print("eta", fde, geta, abs(fde - geta) / abs(geta), flush=True)
+    assert abs(fde - geta) <= 1e-6 * abs(geta)
+
+
+GOLDEN = pathlib.Path(__file__).parent / "golden" / "c5_fd_oracle.json"
+
+
+@pytest.mark.skipif(not GOLDEN.exists(), reason="c5_fd_oracle.json not generated")
+@pytest.mark.parametrize("obj", ["magnetic_energy", "reconnected_flux"])
+def test_c5_matches_oracle_central_differences(obj):
+    """the GPU adjoint on the full workload against central differences of
+    the CPU oracle (tests/golden/make_c5_fd_oracle.py: step 1e-6, the
+    oracle's own frozen dt sequence, which the GPU run reuses)"""
+    fx = json.loads(GOLDEN.read_text())
+    ac = adjoint.case_c5()
+    dts = fx["dts"]
+    assert len(dts) == ac.steps
+    print(f"[adjoint] C5 vs oracle central differences {obj}", flush=True)
+    val, gay, geta, _ = adjoint.objective(ac, dts, obj=obj)
+    print("value", val, fx["value"][obj], flush=True)
+    assert val == pytest.approx(fx["value"][obj], rel=1e-10)
+    v = adjoint.fd_direction(ac, obj, fx["seed"])
+    ad = float(np.vdot(gay, v))
+    fd = fx["fd_ay"][obj]
+    print("ay", fd, ad, abs(fd - ad) / abs(fd), flush=True)
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
+    fde = fx["fd_eta"][obj]
     print("eta", fde, geta, abs(fde - geta) / abs(geta), flush=True)
     assert abs(fde - geta) <= 1e-6 * abs(geta)
