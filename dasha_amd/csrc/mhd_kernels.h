@@ -708,15 +708,22 @@ __device__ inline double rec_field(const Rec &r, int m)
     }
 }
 
-__device__ inline void e_point(const double *__restrict__ X, const double *__restrict__ cur, double eta, long S,
-                               long q, double *__restrict__ E)
+// E at point q from the momentum (x2, x3, x4) the caller already holds
+__device__ inline void e_point_m(const double *__restrict__ X, const double *__restrict__ cur, double eta, long S,
+                                 long q, double x2, double x3, double x4, double *__restrict__ E)
 {
-    double x1 = X[q], x2 = X[q + S], x3 = X[q + 2 * S], x4 = X[q + 3 * S];
+    double x1 = X[q];
     double x5 = X[q + 4 * S], x6 = X[q + 5 * S], x7 = X[q + 6 * S];
     const double y = 1.0 / x1;
     st(E + q, divc(-x3 * x7 + x4 * x6, x1, y) + eta * cur[q]);
     st(E + q + S, divc(-x4 * x5 + x2 * x7, x1, y) + eta * cur[q + S]);
     st(E + q + 2 * S, divc(-x2 * x6 + x3 * x5, x1, y) + eta * cur[q + 2 * S]);
+}
+
+__device__ inline void e_point(const double *__restrict__ X, const double *__restrict__ cur, double eta, long S,
+                               long q, double *__restrict__ E)
+{
+    e_point_m(X, cur, eta, S, q, X[q + S], X[q + 2 * S], X[q + 3 * S], E);
 }
 
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_rhs_a(Geo g, const double *__restrict__ X, const double *__restrict__ pri,
@@ -767,7 +774,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             sp[LX2][y][x] = yx2[0]; sp[LX3][y][x] = yx3[0]; sp[LT22][y][x] = yt22[0]; sp[LT23][y][x] = yt23[0];
             sp[LT24][y][x] = qt24[2]; sp[LT33][y][x] = yt33[0]; sp[LT34][y][x] = qt34[2]; sp[LP][y][x] = qp[2];
         }
-        if (t.own) e_point(X, cur, eta, S, qk, E);
+        // plane k's momentum is in the z queues (loaded two planes ago)
+        if (t.own) e_point_m(X, cur, eta, S, qk, yx2[0], yx3[0], qx4[2], E);
         // halo only (the own value is in place already)
         if (t.tx < 2 || t.tx >= TX - 2) {
             int hx = t.tx < 2 ? -2 : 2;
