@@ -499,6 +499,12 @@ __global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE ==
             __syncthreads();
             #pragma unroll
             for (int p = 1; p <= NP; p++) {
+                // level p is read only on rows p..RB-1-p (the owned rows lie in
+                // NP..RB-1-NP); a row outside skips it, unless image rows
+                // exchange through LDS with a block-wide barrier.  The RK-stage
+                // sweeps keep every row (measured faster: 5.35 against 5.62 ms
+                // at 512x256x512)
+                if (RKS < 0 && !yex && (r < p || r >= RB - p)) continue;
                 const int k = t - p;                        // plane of level p
                 const bool kin = k >= 2 && k <= g.NZ - 3;
                 const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
@@ -593,6 +599,7 @@ __global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE ==
             __syncthreads();
             #pragma unroll
             for (int p = 1; p <= NP; p++) {
+                if (!yex && (r < p || r >= RB - p)) continue;   // as in step
                 const int k = t - p;
                 const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
                 const double zm = q[p - 1][(u + 1) % 3], f0 = q[p - 1][(u + 2) % 3], zp = q[p - 1][u];
