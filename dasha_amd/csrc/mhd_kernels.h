@@ -386,6 +386,9 @@ __global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE ==
         const int j = j0 + r;
         const bool vy = !FACES || g.periodic || j < g.NY;
         const bool own = ownx && vy && j >= nomy && j < nomy + H && j <= g.yhi;
+        // the block's owned rows are r in [olo, ohi) (NP..RB-1-NP, narrower in a
+        // clamped last row tile); level p is read only within NP-p rows of them
+        const int olo = nomy - j0, ohi = min(nomy + H, g.yhi + 1) - j0;
         const bool ld = vx && vy;
         const int jp = vy ? rmap(g, j) : 0;
         const bool yin = !FACES || g.periodic || (j >= 2 && j <= g.NY - 3);
@@ -499,13 +502,12 @@ __global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE ==
             __syncthreads();
             #pragma unroll
             for (int p = 1; p <= NP; p++) {
-                // level p is read only on rows p..RB-1-p (the owned rows lie in
-                // NP..RB-1-NP); a row outside skips it, unless image rows
-                // exchange through LDS with a block-wide barrier.  The RK-stage
-                // sweeps skip only below the last level (fewer spills there:
-                // 5.12 ms against 5.40 with every row, 5.62 skipping both levels,
-                // at 512x256x512)
-                if (!yex && (r < p || r >= RB - p) && (RKS < 0 || p < NP)) continue;
+                // a row farther than NP-p rows from the owned ones skips level p
+                // (nothing reads it), unless image rows exchange through LDS with
+                // a block-wide barrier.  The RK-stage sweeps skip only below the
+                // last level (fewer spills there: 5.12 ms against 5.40 with every
+                // row, 5.62 skipping both levels, at 512x256x512)
+                if (!yex && (r < olo - (NP - p) || r >= ohi + (NP - p)) && (RKS < 0 || p < NP)) continue;
                 const int k = t - p;                        // plane of level p
                 const bool kin = k >= 2 && k <= g.NZ - 3;
                 const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
@@ -600,7 +602,7 @@ __global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE ==
             __syncthreads();
             #pragma unroll
             for (int p = 1; p <= NP; p++) {
-                if (!yex && (r < p || r >= RB - p)) continue;   // as in step
+                if (!yex && (r < olo - (NP - p) || r >= ohi + (NP - p))) continue;   // as in step
                 const int k = t - p;
                 const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
                 const double zm = q[p - 1][(u + 1) % 3], f0 = q[p - 1][(u + 2) % 3], zp = q[p - 1][u];
