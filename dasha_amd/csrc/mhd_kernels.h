@@ -1018,6 +1018,11 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
     const bool pin = ld && yin && i >= 2 && i <= g.NX - 3;
     const bool img = !g.periodic && vy && (j < 2 || j > g.NY - 3);
     const bool xft = i0 < 2 || i0 + TX > g.NX - 2;
+    // with periodic y a row outside the block's owned rows only loads the
+    // plane into LDS for its neighbours' y derivatives: its x/y parts and its
+    // output feed nothing (wave-uniform; non-periodic image rows exchange
+    // through LDS with a block-wide barrier, so there every row works)
+    const bool work = !g.periodic || (j >= nomy && j < min(nomy + H, g.yhi + 1));
     const int rm1 = max(r - 1, 0), rm2 = max(r - 2, 0), rp1 = min(r + 1, RB - 1), rp2 = min(r + 2, RB - 1);
     const int lm1 = max(lane - 1, 0), lm2 = max(lane - 2, 0), lp1 = min(lane + 1, TX - 1), lp2 = min(lane + 2, TX - 1);
     const long S = g.S;
@@ -1070,7 +1075,7 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
         qa[4] = za; qb[4] = zb;
         // J^2 of the output plane t-2 (pressure equation)
         double j2 = 0.0;
-        if (PART == 1 && t - 2 >= kb) {
+        if (PART == 1 && t - 2 >= kb && work) {
             const unsigned e = qc + sz * (unsigned)(t - 2);
             const double c1 = fld(cur, e), c2 = fld(cur + S, e), c3 = fld(cur + 2 * S, e);
             j2 = c1 * c1 + c2 * c2 + c3 * c3;
@@ -1078,7 +1083,7 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
         __syncthreads();
         // x/y parts of plane t
         double xa = 0.0, xb = 0.0, xc = 0.0;
-        if (pin && t >= 2 && t <= g.NZ - 3) {
+        if (work && pin && t >= 2 && t <= g.NZ - 3) {
             if (PART == 0) {
                 xa = -DYS(2);                     // xd5 = -d/dy e3 + ...
                 xb = DXS(2);                      // xd6 = ... + d/dx e3
@@ -1090,7 +1095,7 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
         }
         // plane k = t-2: complete, fill, store
         const int k = t - 2;
-        if (k >= kb) {
+        if (k >= kb && work) {
             double v[NO];
             if (pin) {
                 const double az0 = g.z1a[k], az1 = g.z1b[k], az2 = g.z1c[k], az3 = g.z1d[k];
