@@ -6,12 +6,17 @@ checked against facts fixed independently of its code:
 
 * FD coefficients: closed forms on uniform meshes and the unique
   polynomial-exact (Vandermonde/Fornberg) weights on stretched meshes;
-* smoothing sweep: the uniform-mesh second difference;
+* smoothing sweep: the uniform-mesh second difference; the shapes of the
+  B-component and radial coefficients (value at the profile centre, odd
+  symmetry about it, monotonicity, plateaus, one logistic value), read back
+  from quadratic fields on unit meshes;
 * boundary rules: their defining properties (zero one-sided slope of a
   cubic, 0 / half rules, periodic images, corner precedence);
 * raw right-hand side: symbolic (sympy) derivatives of polynomial fields;
 * composition of smoothing sweeps and pass counts: exact Fourier symbols of
-  a linear resistive/Hall mode far from the origin;
+  a linear resistive/Hall mode far from the origin (x ~ +300 and x ~ -300,
+  where the B-component coefficient is 0 and 1/16), including the final
+  avrg pair of right() and its skip at t = 0;
 * RK4 register scheme: textbook RK4 built from the oracle's own RHS calls;
 * whole step: exact equilibrium invariance, z-mirror symmetry,
   y-translation invariance;
@@ -112,9 +117,88 @@ def test_sweep_is_second_difference_on_uniform_mesh():
     lap = (a[:, 5:-3, 2:-2, 4:-4] + a[:, 3:-5, 2:-2, 4:-4] + a[:, 4:-4, 3:-1, 4:-4] + a[:, 4:-4, 1:-3, 4:-4]
            + a[:, 4:-4, 2:-2, 5:-3] + a[:, 4:-4, 2:-2, 3:-5] - 6 * a[:, 4:-4, 2:-2, 4:-4])
     np.testing.assert_allclose(b[(slice(None),) + s], a[(slice(None),) + s] + lap / 8, rtol=0, atol=1e-12)
-    # the x-profile coefficient (1/32)(1 - tanh((x+3)/8)) on the B components
-    b8 = o.apply(3, rng.standard_normal((8,) + o.shape3), nf=8, coef_kind=1)
-    assert b8.shape[0] == 8
+
+
+def _sweep_coefficient(o, f, kind, curvature):
+    """coefficient c of one smoothing sweep at every interior point, read
+    back from a field whose three second differences sum to `curvature`
+    everywhere: the sweep adds c * curvature (t3dsdsc.f:5636-5666)"""
+    a = np.broadcast_to(f, (1,) + o.shape3).copy()
+    b = o.apply(3, a, nf=1, coef_kind=kind)
+    s = np.s_[0, 4:-4, 2:-2, 4:-4]
+    return (b[s] - a[s]) / curvature
+
+
+def _integer_mesh_case(nx, ny, nz, x_offset=0.0):
+    """uniform unit mesh, integer coordinates: x = x_offset + (-(nx-1)/2 ..
+    (nx-1)/2), y and z centred on 0, non-periodic y (so y = 0 is a row)"""
+    return harris.harris_case("unit", nx, ny, nz, stretched=False, lx=nx - 1.0, lz=nz - 1.0,
+                              ly=ny - 1.0, periodic_y=False, x_offset=x_offset, amp=0.0, noise=0.0)
+
+
+def test_b_component_coefficient_profile():
+    """avrg/avrgtt coefficient of the B components, (1/32)(1 - tanh((x+3)/8))
+    (t3dsdsc.f:5636-5648), pinned by facts of that shape, not by its formula:
+    1/32 at x = -3, odd about x = -3 (c(-3+d) + c(-3-d) = 1/16), strictly
+    decreasing in x, independent of y and z, equal to 1/16 far left and 0 far
+    right, and at x = 5 the logistic value (1/16)/(1 + e^2).  Unit mesh with
+    f = (x+3)^2/2: every x second difference is exactly 1, y and z are 0."""
+    c = _integer_mesh_case(48, 9, 12, x_offset=-3.5)      # x = -27 .. 20
+    o = mk(c)
+    X = np.broadcast_to(c.xx[None, None, :], o.shape3)
+    coef = _sweep_coefficient(o, (X + 3.0) ** 2 / 2.0, 1, 1.0)
+    xs = c.xx[4:-4]
+    # independent of y and z
+    assert np.all(coef == coef[:1, :1, :])
+    cx = coef[0, 0]
+    at = {float(x): float(v) for x, v in zip(xs, cx)}
+    assert at[-3.0] == 1.0 / 32.0
+    # (read-back rounding: f reaches ~110 where c ~ 1/32, ulp 1.4e-14)
+    for d in range(1, 16):
+        assert abs(at[-3.0 + d] + at[-3.0 - d] - 1.0 / 16.0) < 1e-13, d
+    assert np.all(np.diff(cx) < 0)
+    assert abs(at[5.0] - (1.0 / 16.0) / (1.0 + math.e ** 2)) < 1e-13
+    # far from the profile centre: the two plateaus
+    for off, want in ((-300.0, 1.0 / 16.0), (300.0, 0.0)):
+        cf = _integer_mesh_case(20, 9, 12, x_offset=off)
+        of = mk(cf)
+        Xf = np.broadcast_to(cf.xx[None, None, :] - off, of.shape3)
+        got = _sweep_coefficient(of, Xf ** 2 / 2.0, 1, 1.0)
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+
+
+def test_radial_coefficient_profile():
+    """avrg/avrgE/avrgc coefficient 1/32 + (3/32)(1 + tanh((r-3)/3))/2
+    (t3dsdsc.f:5659-5661, 5895-5897, 5981-5983), pinned by facts of that
+    shape: a function of r = |(x,y,z)| alone (equal on every point of equal
+    r), 5/64 at r = 3, odd about r = 3 (c(3+d) + c(3-d) = 5/32), increasing in
+    r, 1/8 far away, and at r = 0 the logistic value 1/32 + (3/32)/(1 + e^2).
+    Unit mesh through the origin with f = r^2/2: the three second
+    differences are exactly 1 each."""
+    c = _integer_mesh_case(17, 13, 17)                    # interior |x|,|z| <= 6, |y| <= 4
+    o = mk(c)
+    Z, Y, X = np.meshgrid(c.zz, c.yy, c.xx, indexing="ij")
+    coef = _sweep_coefficient(o, (X * X + Y * Y + Z * Z) / 2.0, 0, 3.0)
+    s = np.s_[4:-4, 2:-2, 4:-4]
+    r2 = np.rint(X[s] ** 2 + Y[s] ** 2 + Z[s] ** 2).astype(int)
+    by_r2 = {}
+    for k, v in zip(r2.ravel(), coef.ravel()):
+        by_r2.setdefault(int(k), []).append(float(v))
+    for k, vs in by_r2.items():
+        assert max(vs) - min(vs) <= 1e-16, k
+    at = {k: vs[0] for k, vs in by_r2.items()}
+    assert len(by_r2[9]) >= 6                             # (+-3,0,0), (0,+-3,0), (0,0,+-3), ...
+    assert at[9] == 5.0 / 64.0
+    for d in (1, 2, 3):
+        assert abs(at[(3 + d) ** 2] + at[(3 - d) ** 2] - 5.0 / 32.0) < 1e-14, d
+    radii = sorted(at)
+    assert all(at[radii[i]] < at[radii[i + 1]] for i in range(len(radii) - 1))
+    assert abs(at[0] - (1.0 / 32.0 + (3.0 / 32.0) / (1.0 + math.e ** 2))) < 1e-15
+    cf = _integer_mesh_case(20, 9, 12, x_offset=300.0)
+    of = mk(cf)
+    Xf = np.broadcast_to(cf.xx[None, None, :] - 300.0, of.shape3)
+    np.testing.assert_allclose(_sweep_coefficient(of, Xf ** 2 / 2.0, 0, 1.0), 1.0 / 8.0, rtol=0,
+                               atol=1e-13)
 
 
 # ---------------------------------------------------------------- boundaries
@@ -259,20 +343,28 @@ def test_raw_rhs_matches_symbolic_derivatives():
 
 
 # ---------------------------------------------------------------- composition
-def _mode_case(mz=64, ny=8):
+def _mode_case(mz=64, ny=8, x_offset=300.0):
     c = harris.harris_case("mode", 48, ny, mz, stretched=False, lx=20.0, lz=16.0, ly=4.0,
-                           x_offset=300.0, eta0=0.01, di=0.2, amp=0.0, noise=0.0)
+                           x_offset=x_offset, eta0=0.01, di=0.2, amp=0.0, noise=0.0)
     return c
 
 
-def test_fourier_symbols_pin_sweep_counts():
+@pytest.mark.parametrize("x_offset", [300.0, -300.0])
+@pytest.mark.parametrize("t", [0.5, 0.0])
+def test_fourier_symbols_pin_sweep_counts(x_offset, t):
     """far from the origin every radial coefficient is exactly 1/8 and the
-    B-component one exactly 0; a z-mode dBy = eps cos(kz) on a uniform state
-    with background Bz0 must give
-      xdif(6) = -eta sigma^2 F^8 dBy      (avrgc x4, avrgE x4)
-      xdif(5) = -(cht Bz0/rho) sigma^2 F^9 eps cos(kz)   (avrgc x4, avrgh x5)
-    with sigma the 5-point derivative symbol and F = 1 + (2cos kh - 2)/8."""
-    c = _mode_case()
+    B-component one exactly 0 (x ~ +300) or 1/16 (x ~ -300); a z-mode
+    dBy = eps cos(kz) on a uniform state with background Bz0 and no frozen
+    residual must give
+      xdif(3) = -Bz0 sigma F^4 G_r^2 eps sin(kz)              (avrgc x4; avrg x2)
+      xdif(6) = -eta sigma^2 F^8 G_b^2 eps cos(kz)            (avrgc x4, avrgE x4; avrg x2)
+      xdif(5) = -(cht Bz0/rho) sigma^2 F^9 G_b^2 eps cos(kz)  (avrgc x4, avrgh x5; avrg x2)
+    with sigma the 5-point derivative symbol, F = 1 + (2cos kh - 2)/8 the
+    symbol of one pass at coefficient 1/8, G_r = F and G_b the symbol at the
+    B-component coefficient (1 at x ~ +300, 1 + (2cos kh - 2)/16 at x ~ -300).
+    The final avrg pair of right() runs only for t != 0 (t3dsdsc.f:1669): at
+    t = 0 the G factors are 1."""
+    c = _mode_case(x_offset=x_offset)
     shape = (c.mz + 4, c.my, c.mx + 4)
     h = 16.0 / (c.mz - 1)
     kw = 2 * math.pi / 16.0 * 3
@@ -288,18 +380,24 @@ def test_fourier_symbols_pin_sweep_counts():
     c.bxi = np.zeros(shape); c.byi = np.zeros(shape); c.bzi = np.full(shape, bz0)
     c.pri = np.full(shape, p0)
     o = mk(c)
-    o.rhs(0.5)
+    o.rhs(t)
     xd = o.get_xdif()
     sig = (4.0 / 3.0 * math.sin(kw * h) - 1.0 / 6.0 * math.sin(2 * kw * h)) / h
     F = 1 + (2 * math.cos(kw * h) - 2) / 8.0
+    Fb = 1.0 if x_offset > 0 else 1 + (2 * math.cos(kw * h) - 2) / 16.0
+    nfin = 2 if t != 0 else 0
+    Gr, Gb = F ** nfin, Fb ** nfin
     # boundary rules reach ~15 cells inwards: compare the centre only
     s = np.s_[c.mz // 2 - 3:c.mz // 2 + 4, 3:c.my - 3, c.mx // 2 - 3:c.mx // 2 + 4]
     cz = np.cos(kw * Z)[s]
-    np.testing.assert_allclose(xd[5][s], -c.eta0 * sig ** 2 * F ** 8 * eps * cz, rtol=1e-6, atol=1e-22)
-    np.testing.assert_allclose(xd[4][s], -(c.di * bz0 / rho0) * sig ** 2 * F ** 9 * eps * cz, rtol=1e-6,
+    sz = np.sin(kw * Z)[s]
+    np.testing.assert_allclose(xd[2][s], -bz0 * sig * F ** 4 * Gr * eps * sz, rtol=1e-6, atol=1e-22)
+    np.testing.assert_allclose(xd[5][s], -c.eta0 * sig ** 2 * F ** 8 * Gb * eps * cz, rtol=1e-6, atol=1e-22)
+    np.testing.assert_allclose(xd[4][s], -(c.di * bz0 / rho0) * sig ** 2 * F ** 9 * Gb * eps * cz, rtol=1e-6,
                                atol=1e-22)
-    # a wrong sweep count would change F's power: the check is sensitive
-    assert abs(F ** 8 / F ** 9 - 1) > 1e-2
+    # a wrong sweep count or coefficient changes a power of F or Fb by one
+    # factor, which the 1e-6 tolerance resolves
+    assert abs(F - 1) > 1e-2 and abs(Fb - 1) > 5e-3 or x_offset > 0
 
 
 def _rk4_reference(o, x0, dt):
