@@ -1,10 +1,11 @@
 #!/usr/bin/env python3
 """Benchmark of the FP64 RK4 Hall-MHD step on one MI355X.
 
-Workload (default): the tanh-stretched 256x128x256 Harris sheet with Hall
-term (eta 0.003, d_i 0.2), the single-GPU HBM roofline configuration.  A
-"step" is one full RK4 step of the reference's stepon (four right-hand
-sides with every smoothing sweep and boundary fill, the post-step
+Workload (default): the tanh-stretched 512x256x512 Harris sheet with Hall
+term and guide field (beta 0.1, By 0.1, eta 0.002, d_i 0.25), 200 steps: the
+largest single-GPU configuration and the one the 60% HBM-roofline target is
+quoted on.  A "step" is one full RK4 step of the reference's stepon (four
+right-hand sides with every smoothing sweep and boundary fill, the post-step
 perturbation smoothing) plus its setdt, with the state resident in HBM.
 
 Prints ONE JSON line:
@@ -13,7 +14,8 @@ Prints ONE JSON line:
   step_roofline  whole step: value * 6256 B (5104 B without Hall) / 8 TB/s
   divb_drift     max and rms of div B (diverB + bndrydiv) before and after the run
   cpu_baseline   the CPU oracle (C restatement, OpenMP) on a bounded sample
-  C3             the same timed loop on 512x256x512
+                 of the same workload (one step), with the host's CPU model
+  C2             the same timed loop on 256x128x256 (200 steps)
   ensemble       48 uneven Harris runs advanced together vs one after another
   adjoint        discrete adjoint of 500 steps at 128x64x128 (sqrt(N)
                  checkpointing): reverse sweep incl. recomputation / forward
@@ -78,15 +80,28 @@ def traffic_from_profiles(kernel: str, workload: str):
     return None if k is None else k["bytes_per_launch"]
 
 
-def cpu_baseline(case_name: str, steps: int, threads: int):
-    from dasha_amd import harris
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, steps: int, threads: int):
+    """the oracle timed on `steps` setdt + stepon pairs of the same workload
+    (after its residual capture, untimed)"""
     from oracle import oracle
     oracle.build()
     oracle.set_threads(threads)
-    case = harris.CASES[case_name]()
+    t_setup = time.perf_counter()
     o = oracle.Oracle(case)
     o.init_residual()
-    dt = o.compute_dt()
+    print(f"[bench] cpu baseline: {case.name} oracle set up in {time.perf_counter() - t_setup:.1f}s "
+          f"on {threads} threads", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     tt = 0.0
     for _ in range(steps):
@@ -94,10 +109,13 @@ def cpu_baseline(case_name: str, steps: int, threads: int):
         o.step(tt, dt)
         tt += dt
     el = time.perf_counter() - t0
+    o.close()
+    print(f"[bench] cpu baseline: {steps} step(s) in {el:.1f}s", file=sys.stderr, flush=True)
     return {"value": case.cells * steps / el, "unit": "cell-updates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "seconds": el,
             "sample": f"{steps} RK4 step(s) incl. setdt of {case.name} ({case.cells} cells) "
-                      f"by the C/OpenMP oracle, {el:.2f} s"}
+                      f"by the C/OpenMP oracle on {threads} threads, {el:.2f} s"}
 
 
 def time_config(name: str, steps: int, warmup: int):
@@ -198,15 +216,16 @@ def time_adjoint(steps: int = 500):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the configuration's own count, 200 for C3)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3"])
+    ap.add_argument("--config", default="C3", choices=["C1", "C2", "C3"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event run")
     ap.add_argument("--copy-probe", action="store_true", help="also measure a plain device copy")
-    ap.add_argument("--no-c3", action="store_true",
-                    help="skip the 512x256x512 timing added to the 256x128x256 line")
+    ap.add_argument("--no-c2", action="store_true",
+                    help="skip the 256x128x256 timing added to the line")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble timing")
     ap.add_argument("--ensemble-only", action="store_true", help="time the batched ensemble alone")
     ap.add_argument("--ensemble-runs", type=int, default=48)
@@ -226,6 +245,8 @@ def main():
     from dasha_amd.mhd import MHDStep
 
     case = harris.CASES[args.config]()
+    if args.steps is None:
+        args.steps = min(case.steps, 200)
     t_setup = time.perf_counter()
     g = MHDStep(case)
     g.init_residual()
@@ -302,15 +323,16 @@ def main():
         from dasha_amd.mhd import copy_rate
         out["copy_probe_GBps"] = copy_rate() / 1e9
     g.close()
-    if args.config == "C2" and not args.no_c3:
-        out["C3"] = time_config("C3", 10, 2)
-    if args.config == "C2" and not args.no_ensemble:
+    if args.config == "C3" and not args.no_c2:
+        out["C2"] = time_config("C2", 200, 3)
+    if not args.no_ensemble:
         out["ensemble"] = time_ensemble(10, args.ensemble_runs)
-    if args.config == "C2" and not args.no_adjoint:
+    if not args.no_adjoint:
         out["adjoint"] = time_adjoint(args.adjoint_steps)
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_steps, threads)
+        # the host's CPU share for one GPU (OMP_NUM_THREADS on the GPU hosts)
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "16") or 16), len(os.sched_getaffinity(0)))
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps, threads)
     print(json.dumps(out), flush=True)
 
 

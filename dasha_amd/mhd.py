@@ -119,6 +119,9 @@ class MHDStep:
     def __init__(self, case):
         self.L = load_library()
         self.case = case
+        # open ensembles that use this handle; close() waits for them
+        self._groups = 0
+        self._close_pending = False
         cfg = MHDConfig(mx=case.mx, my=case.my, mz=case.mz, periodic_y=int(case.periodic_y),
                         hall=int(case.hall), whistler_dt=int(case.whistler_dt), di=case.di,
                         eta0=case.eta0, fmu0=case.fmu0, frho0=case.frho0, fpr0=case.fpr0,
@@ -141,6 +144,11 @@ class MHDStep:
             raise MHDError(rc, msg)
 
     def close(self):
+        """free the handle; while an open MHDEnsemble still uses it the
+        handle is freed when the last such ensemble closes"""
+        if getattr(self, "_groups", 0) > 0:
+            self._close_pending = True
+            return
         if getattr(self, "h", None):
             self.L.mhd_destroy(self.h)
             self.h = None
@@ -272,6 +280,7 @@ class MHDEnsemble:
     def __init__(self, runs):
         self.L = load_library()
         self.runs = list(runs)
+        self._held = []
         arr = (C.c_void_p * len(self.runs))(*[r.h for r in self.runs])
         e = C.c_void_p()
         rc = self.L.mhd_ensemble_create(arr, len(self.runs), C.byref(e))
@@ -280,6 +289,11 @@ class MHDEnsemble:
             msg = self.L.mhd_ensemble_error(e).decode() if e else ""
             self.close()
             raise MHDError(rc, msg)
+        # the runs' handles must outlive the ensemble (mhd_ensemble_destroy
+        # restores their chunking and synchronises their streams)
+        for r in self.runs:
+            r._groups += 1
+            self._held.append(r)
 
     def advance(self, nsteps: int, times, dt0s):
         """nsteps for every run from times[r] (dt0s[r] arms the collapse
@@ -315,6 +329,12 @@ class MHDEnsemble:
         if getattr(self, "e", None):
             self.L.mhd_ensemble_destroy(self.e)
             self.e = None
+        held, self._held = getattr(self, "_held", []), []
+        for r in held:
+            r._groups -= 1
+            if r._groups == 0 and r._close_pending:
+                r._close_pending = False
+                r.close()
 
     def __del__(self):
         try:

@@ -78,3 +78,75 @@ def test_tuned_ensemble_matches_single_runs():
     for a, b in zip(singles, members):
         assert np.array_equal(a.get_state(), b.get_state())
     ens.close()
+
+
+def test_c4_set_matches_single_runs_and_oracle():
+    """The 48-run C4 set (64x32x64 .. 192x96x192, eta, d_i, amplitude and
+    sheet thickness drawn from seed 4) through the ensemble for 3 steps (a
+    captured step pair and an eager odd step): every run bit-identical to
+    mhd_advance on that run alone (the run's own z chunking there, the
+    ensemble's long chunks here), and the smallest run, the largest run and
+    the run with the smallest d_i against the oracle."""
+    import time
+    from dasha_amd.mhd import MHDEnsemble, MHDStep
+    from oracle.oracle import Oracle
+    from tests.util import field_errors
+    t0 = time.time()
+    cases = harris.case_c4()
+    print(f"[c4] {len(cases)} runs, {sum(c.cells for c in cases) / 1e6:.1f}M cells, "
+          f"built in {time.time() - t0:.1f} s", flush=True)
+    steps = 3
+    alone = []
+    for c in cases:
+        g = MHDStep(c)
+        g.init_residual()
+        dt0 = g.compute_dt()
+        tn, _, _ = g.advance(steps, 0.0, dt0)
+        alone.append((dt0, tn, g.get_state()))
+        g.close()
+    print(f"[c4] runs alone done ({time.time() - t0:.1f} s)", flush=True)
+    members = [MHDStep(c) for c in cases]
+    for g in members:
+        g.init_residual()
+    ens = MHDEnsemble(members)
+    t_ens, ms = ens.advance(steps, np.zeros(len(cases)), [a[0] for a in alone])
+    print(f"[c4] ensemble {steps} steps {ms:.1f} ms ({time.time() - t0:.1f} s)", flush=True)
+    assert ens.status() == [0] * len(cases)
+    for r, (g, (_, tn, x)) in enumerate(zip(members, alone)):
+        assert t_ens[r] == tn, r
+        assert np.array_equal(g.get_state(), x), r
+    pick = {min(range(len(cases)), key=lambda r: cases[r].cells),
+            max(range(len(cases)), key=lambda r: cases[r].cells),
+            min(range(len(cases)), key=lambda r: cases[r].di)}
+    for r in sorted(pick):
+        o = Oracle(cases[r])
+        to, _ = o.run(steps)
+        e = field_errors(alone[r][2], o.get_state())
+        print(f"[c4] run {cases[r].name} d_i {cases[r].di:.4f} eta {cases[r].eta0:.4f}: "
+              f"rel L2 vs oracle {max(e):.2e}", flush=True)
+        assert abs(to - t_ens[r]) <= 1e-13 * to
+        assert max(e) < 1e-12
+        o.close()
+    ens.close()
+    for g in members:
+        g.close()
+
+
+def test_member_close_waits_for_the_ensemble():
+    """closing a run that an open ensemble still uses defers the free until
+    the ensemble closes (the ensemble's destroy touches every run's handle)"""
+    from dasha_amd.mhd import MHDEnsemble, MHDStep
+    cases = _cases()[:2]
+    members = [MHDStep(c) for c in cases]
+    for g in members:
+        g.init_residual()
+    ens = MHDEnsemble(members)
+    members[0].close()
+    assert members[0].h                      # still alive: the ensemble uses it
+    t, _ = ens.advance(2, np.zeros(2), [0.0, 0.0])
+    assert all(v > 0 for v in t)
+    ens.close()
+    assert not members[0].h                  # freed with the ensemble
+    assert members[1].h
+    members[1].close()
+    assert not members[1].h

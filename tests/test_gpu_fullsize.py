@@ -1,24 +1,79 @@
-"""The HIP step at the benchmark sizes, pinned by properties that hold at any
-size (the CPU oracle finishes these sizes too slowly to compare directly):
+"""The HIP step at the benchmark sizes.
 
-* 512x256x512: an unperturbed Harris equilibrium (Hall off) is an exact fixed
-  point: J = curl(B - B_i) = 0, the momentum flux uses p - p_i, the frozen
-  residual is zero, so every sweep sees zero perturbations (the same pin as
-  tests/test_oracle_pins.py, there on the oracle);
-* 256x128x256, Hall on: y-translation invariance -- a state rolled by s
-  periodic rows gives, after three steps, the rolled result to 1e-12 of each
-  field's scale (the domain sits at x ~ 300, where the radial smoothing
-  coefficient is exactly 1/8; the row coordinates (j - 2) dy leave only
-  last-bit differences between the rows' y coefficients);
-* 256x128x256, Hall on: z-mirror symmetry of the symmetric stretched mesh
-  (v polar, B pseudovector) to 1e-11 of each field's scale after three steps.
+* The exact C2 (256x128x256, Hall on) and C3 (512x256x512, Hall on, guide
+  field) workloads against the oracle on the host's cores: the frozen
+  residual, two setdt + stepon pairs (dt bit for bit), the state after each
+  step and the current after the second, relative L2 per field <= 1e-12.
+* Properties that hold at any size:
+  - 512x256x512, Hall on with the guide field: y-translation invariance --
+    a state rolled by s periodic rows gives, after two steps, the rolled
+    result to 1e-12 of each field's scale (the domain sits at x ~ 300, where
+    the radial smoothing coefficient is exactly 1/8; the row coordinates
+    (j - 2) dy leave only last-bit differences between the rows' y
+    coefficients);
+  - 256x128x256, Hall on: the same y-translation invariance after three
+    steps, and z-mirror symmetry of the symmetric stretched mesh (v polar,
+    B pseudovector) to 1e-11 of each field's scale;
+  - 512x256x512, Hall off: an unperturbed Harris equilibrium is an exact
+    fixed point (J = curl(B - B_i) = 0, the momentum flux uses p - p_i, the
+    frozen residual is zero), as the oracle pin shows on small grids.
 """
 import numpy as np
 import pytest
 
 from dasha_amd import harris
+from tests.util import field_errors
 
 pytestmark = pytest.mark.gpu
+
+
+def _vs_oracle(case, steps=2):
+    """GPU and oracle through the same calls: init_residual, then `steps`
+    x (setdt; stepon), compared after every call"""
+    import os
+    import time
+    from oracle.oracle import Oracle, max_threads
+    from dasha_amd.mhd import MHDStep
+    t0 = time.time()
+    g = MHDStep(case)
+    o = Oracle(case)
+    print(f"[fullsize] {case.name}: oracle on {max_threads()} threads "
+          f"(cpu_count {os.cpu_count()}), set up {time.time() - t0:.1f} s", flush=True)
+    try:
+        g.init_residual()
+        o.init_residual()
+        e = field_errors(g.get_residual(), o.get_residual())
+        print(f"[fullsize] {case.name}: xresd rel L2 {max(e):.2e} ({time.time() - t0:.1f} s)", flush=True)
+        assert max(e) < 1e-12
+        t = 0.0
+        for s in range(steps):
+            dg, do = g.compute_dt(), o.compute_dt()
+            print(f"[fullsize] {case.name}: step {s + 1} dt gpu {dg!r} oracle {do!r}", flush=True)
+            assert dg == do
+            g.step(t, dg)
+            o.step(t, do)
+            t += dg
+            xg = g.get_state()
+            xo = o.get_state()
+            e = field_errors(xg, xo)
+            del xg, xo
+            print(f"[fullsize] {case.name}: step {s + 1} rel L2 per field "
+                  f"{['%.1e' % v for v in e]} ({time.time() - t0:.1f} s)", flush=True)
+            assert max(e) < 1e-12
+        e = field_errors(g.get_current(), o.get_current())
+        print(f"[fullsize] {case.name}: current rel L2 {max(e):.2e}", flush=True)
+        assert max(e) < 1e-12
+    finally:
+        g.close()
+        o.close()
+
+
+def test_c2_matches_oracle():
+    _vs_oracle(harris.case_c2())
+
+
+def test_c3_matches_oracle():
+    _vs_oracle(harris.case_c3())
 
 
 def _run(case, steps):
@@ -39,6 +94,39 @@ def test_c3_equilibrium_is_an_exact_fixed_point():
     assert np.array_equal(x, c.x0)
 
 
+def _roll_rows(case, sh):
+    ny = case.ny
+    rolled = np.roll(case.x0[:, :, 2:2 + ny], sh, axis=2)
+    case.x0[:, :, 2:2 + ny] = rolled
+    case.x0[:, :, 0:2] = rolled[:, :, ny - 2:ny]
+    case.x0[:, :, ny + 2:ny + 4] = rolled[:, :, 0:2]
+
+
+def _far_c3():
+    return harris.harris_case("C3_far", 512, 256, 512, stretched=True, lam=0.5, beta_inf=0.1,
+                              guide=0.1, eta0=0.002, di=0.25, amp=0.05, seed=3, x_offset=300.0,
+                              noise=1e-3)
+
+
+def test_c3_y_translation_invariance():
+    """C3's sizes, Hall and guide field (the z-mirror symmetry does not hold
+    with a guide field), two steps"""
+    c1 = _far_c3()
+    print(f"[fullsize] y translation {c1.name} {c1.mx}x{c1.ny}x{c1.mz} hall={c1.hall}", flush=True)
+    r1 = _run(c1, 2)
+    del c1
+    ny, sh = 256, 91
+    c2 = _far_c3()
+    _roll_rows(c2, sh)
+    r2 = _run(c2, 2)
+    del c2
+    assert np.abs(r1[5] - 0.1).max() > 1e-8        # the step has moved By
+    for m in range(8):
+        a = np.roll(r1[m][:, 2:2 + ny], sh, axis=1)
+        b = r2[m][:, 2:2 + ny]
+        assert np.abs(a - b).max() <= 1e-12 * (np.abs(a).max() + 1e-300), m
+
+
 def _far_case(seed):
     return harris.harris_case("C2_far", 256, 128, 256, stretched=True, lam=0.5, beta_inf=0.2,
                               eta0=0.003, di=0.2, amp=0.05, seed=seed, x_offset=300.0, noise=1e-3)
@@ -50,10 +138,7 @@ def test_c2_y_translation_invariance():
     r1 = _run(c1, 3)
     ny, sh = c1.ny, 37
     c2 = _far_case(2)
-    rolled = np.roll(c2.x0[:, :, 2:2 + ny], sh, axis=2)
-    c2.x0[:, :, 2:2 + ny] = rolled
-    c2.x0[:, :, 0:2] = rolled[:, :, ny - 2:ny]
-    c2.x0[:, :, ny + 2:ny + 4] = rolled[:, :, 0:2]
+    _roll_rows(c2, sh)
     r2 = _run(c2, 3)
     a = np.roll(r1[:, :, 2:2 + ny], sh, axis=2)
     b = r2[:, :, 2:2 + ny]
