@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -344,6 +345,7 @@ struct mhd_handle {
     std::vector<double> wx, wy, wz;        // energy weights (host copy)
     double *dwx = nullptr, *dwy = nullptr, *dwz = nullptr;
     std::string err;
+    int dev = -1;                          // HIP device the handle was created on
     bool have_eq = false;
     int kc = 16;                           // z planes per tile chunk
     int fuse_rk = 1;                       // RK update fused into the last avrg pass
@@ -378,6 +380,36 @@ struct mhd_handle {
 
 // launch-time errors of void helpers surface at the next synchronising call
 #define HCHK_V(call) (void)(call)
+
+// ---- allocation-free setdt (mhd_compute_dt_noalloc) ----
+// The block partial minima of setdt go to this fixed device array instead of
+// the handle's reduction buffer.  Its host-side name is not a device address:
+// kernels get the address hipGetSymbolAddress returns for it, looked up once
+// per device when the first handle is created there (mhd_create).
+static constexpr int MHD_SCRATCH_DOUBLES = 4096;
+static constexpr int MHD_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[MHD_SCRATCH_DOUBLES];
+static double *g_scratch_ptr[MHD_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// with device `dev` current: record the device address of `scratch`
+static int scratch_setup(mhd_handle *h, int dev)
+{
+    if (dev < 0 || dev >= MHD_MAX_DEVICES) {
+        h->err = "device index beyond the scratch table";
+        return MHD_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return MHD_OK;
+    void *p = nullptr;
+    HCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+        h->err = "hipGetSymbolAddress returned no address for the scratch array";
+        return MHD_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+    return MHD_OK;
+}
 
 namespace {
 
@@ -810,14 +842,16 @@ static void step_body(mhd_handle *h)
     PROF_END(PT_POST);
 }
 
-static void dt_kernels(mhd_handle *h, int advance_time)
+// part: h->nred partial minima (the handle's reduction buffer by default)
+static void dt_kernels(mhd_handle *h, int advance_time, double *part = nullptr)
 {
     const mhd_config &c = h->cfg;
     double wcoef = (c.whistler_dt && c.hall && c.di != 0.0) ? 3.141592654 * c.di : 0.0;
+    if (!part) part = h->red;
     PROF_BEGIN(PT_DT);
     hipLaunchKernelGGL(k_dt_partial, dim3(h->nred), dim3(256), 0, h->stream, h->g, h->x, c.eta0, c.fmu0, c.cfl,
-                       c.gamma, wcoef, h->red);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, h->stream, h->red, h->nred, h->dtv, advance_time);
+                       c.gamma, wcoef, part);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, h->stream, part, h->nred, h->dtv, advance_time);
     PROF_END(PT_DT);
 }
 
@@ -1066,6 +1100,11 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     long pad = 576;
     if (const char *e = getenv("MHD_PAD")) pad = 64L * std::max(0, atoi(e));
     h->S = (h->N + 63) / 64 * 64 + pad;
+    HCHK(hipGetDevice(&h->dev));
+    {
+        int r = scratch_setup(h, h->dev);
+        if (r) { *out = h; return r; }
+    }
     HCHK(hipStreamCreate(&h->stream));
 
     // ---- 1-D geometry on the host (ext-indexed) ----
@@ -1293,6 +1332,25 @@ int mhd_compute_dt(mhd_handle *h, double *dt)
 {
     if (!h || !dt) return MHD_ERR_ARG;
     dt_kernels(h, 0);
+    double v[8];
+    HCHK(hipMemcpyAsync(v, h->dtv, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HCHK(hipStreamSynchronize(h->stream));
+    HCHK(hipGetLastError());
+    *dt = v[0];
+    return MHD_OK;
+}
+
+int mhd_compute_dt_noalloc(mhd_handle *h, double *dt)
+{
+    if (!h || !dt) return MHD_ERR_ARG;
+    double *part = (h->dev >= 0 && h->dev < MHD_MAX_DEVICES) ? g_scratch_ptr[h->dev] : nullptr;
+    if (!part) { set_err(h, "scratch array not set up on the handle's device"); return MHD_ERR_HIP; }
+    if (h->nred > MHD_SCRATCH_DOUBLES) {
+        set_err(h, "setdt needs " + std::to_string(h->nred) + " partial minima; the scratch array holds " +
+                       std::to_string(MHD_SCRATCH_DOUBLES));
+        return MHD_ERR_ARG;
+    }
+    dt_kernels(h, 0, part);
     double v[8];
     HCHK(hipMemcpyAsync(v, h->dtv, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HCHK(hipStreamSynchronize(h->stream));

@@ -28,7 +28,7 @@ EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state"
            "mhd_set_residual", "mhd_set_profiling", "mhd_kernel_times", "mhd_copy_rate",
            "mhd_last_error", "mhd_destroy", "mhd_ensemble_create", "mhd_ensemble_advance",
            "mhd_ensemble_status", "mhd_ensemble_error", "mhd_ensemble_destroy",
-           "mhd_adjoint_run", "mhd_adjoint_rhs", "mhd_ensemble_tune")
+           "mhd_adjoint_run", "mhd_adjoint_rhs", "mhd_ensemble_tune", "mhd_compute_dt_noalloc")
 
 MHD_ADJ_RESIDUAL = 1
 OBJECTIVES = {"magnetic_energy": 0, "reconnected_flux": 1}
@@ -75,6 +75,7 @@ def load_library(path: str = LIBPATH):
         getattr(L, n).argtypes = [P, D]
     L.mhd_init_residual.argtypes = [P]
     L.mhd_compute_dt.argtypes = [P, D]
+    L.mhd_compute_dt_noalloc.argtypes = [P, D]
     L.mhd_step.argtypes = [P, C.c_double, C.c_double]
     L.mhd_advance.argtypes = [P, I, C.c_double, D, D, D]
     L.mhd_diagnostics.argtypes = [P, C.POINTER(MHDDiag)]
@@ -194,6 +195,13 @@ class MHDStep:
     def compute_dt(self) -> float:
         v = C.c_double()
         self._check(self.L.mhd_compute_dt(self.h, C.byref(v)))
+        return v.value
+
+    def compute_dt_noalloc(self) -> float:
+        """setdt with its partial minima in the library's fixed device array
+        (the call allocates nothing)"""
+        v = C.c_double()
+        self._check(self.L.mhd_compute_dt_noalloc(self.h, C.byref(v)))
         return v.value
 
     def step(self, time: float, dt: float):

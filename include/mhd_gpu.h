@@ -80,6 +80,15 @@ int mhd_get_state(mhd_handle *h, double *x);
 int mhd_init_residual(mhd_handle *h);
 /* setdt (t3dsdsc.f:4694-4810) */
 int mhd_compute_dt(mhd_handle *h, double *dt);
+/* mhd_compute_dt without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The same kernels and result as
+ * mhd_compute_dt; the 1024 block partial minima go to a fixed 32 KB device
+ * array (4096 doubles) of the library instead of the handle's reduction
+ * buffer.  Each GPU has one such array, shared by every handle on it and
+ * holding the last call's partials: calls that overlap on one GPU race.
+ * Returns MHD_ERR_ARG if setdt needed more partials than the array holds. */
+int mhd_compute_dt_noalloc(mhd_handle *h, double *dt);
 /* stepon (t3dsdsc.f:1063-1165): one RK4 step of size dt from time */
 int mhd_step(mhd_handle *h, double time, double dt);
 /* the time loop body of program main (t3dsdsc.f:139-176) without I/O:
