@@ -1,0 +1,142 @@
+"""Magnetosphere-mode inputs: the reference's own default case.
+
+The reference (t3dsdsc.f) runs a dipole magnetosphere driven from the x-left
+face.  This module builds that case in the reference's array conventions
+(see ``harris.py``):
+
+* the uniform mesh of ``gridpntnew2`` (t3dsdsc.f:4385-4396):
+  ``xx(jx) = xmin + (jx-1) * (xmax - xmin) / (mx - 1)``, non-periodic y;
+* the initial state of ``initia`` (t3dsdsc.f:771-814): density and pressure
+  shells around r = 1.04, zero velocity, and either the point dipole
+  ``14 (-3xz, -3yz, x^2+y^2-2z^2) / r^5`` (zero for r < 0.1) or the field
+  read from ``inp/binx``, ``biny``, ``binz`` scaled by -25/3 (``mxdat``); the
+  equilibrium arrays are the initial state;
+* the inflow face data ``bxinfc, byinfc, bzinfc`` of ``fcinp/bin*`` scaled by
+  -50 (``mxdatfc``, t3dsdsc.f:728-768), zero otherwise.
+
+Single-precision literals of those routines enter as float32 values, as a
+Fortran compiler stores them (``F32``).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from dasha_amd.harris import MHDCase, pad2
+
+
+def F32(v: float) -> float:
+    """a default-real Fortran literal as the compiler stores it"""
+    return float(np.float32(v))
+
+
+def uniform_axis(n: int, lo: float, hi: float):
+    """gridpntnew2's uniform branch: lo + (j-1) * (hi - lo) / (n - 1), the
+    spacing array holding that constant step"""
+    d = (hi - lo) / float(n - 1)
+    pts = lo + np.arange(n, dtype=np.float64) * d
+    return pts, np.full(n, d)
+
+
+@dataclasses.dataclass
+class MagnetosphereCase(MHDCase):
+    """An MHDCase with the magnetosphere-mode switches of the C ABI."""
+    magnetosphere: bool = True
+    inflow: bool = True
+    radial_hall: bool = True
+    partial_e: bool = False
+    rcr: float = 1.0
+    inflow_face: tuple | None = None       # (bxinfc, byinfc, bzinfc), each (mz+4, my)
+
+
+def dipole_state(xx, yy, zz, bfield=None):
+    """initia (t3dsdsc.f:771-814) on the padded grid: returns (x0, rr0)
+    with x0 of shape (8, mz+4, my, mx+4); bfield = (bx, by, bz) replaces the
+    point dipole (the mxdat branch, already scaled)"""
+    X = xx[None, None, :]
+    Y = yy[None, :, None]
+    Z = zz[:, None, None]
+    shape = (zz.size, yy.size, xx.size)
+    r0 = np.broadcast_to(np.sqrt(X ** 2 + Y ** 2 + Z ** 2), shape)
+    r1 = np.broadcast_to(np.sqrt((X - 0.0) ** 2 + Y ** 2 + Z ** 2), shape)
+    wid = float(np.float32(np.float32(2.0) * np.float32(0.15)) * np.float32(0.15))   # 2.*0.15*0.15
+    shell = np.exp(-((r0 - F32(1.04)) * (r0 - F32(1.04))) / wid)
+    x0 = np.zeros((8,) + shape)
+    x0[0] = 1.0e-3 * shell + F32(0.1) * 1.0e-3
+    if bfield is None:
+        r5 = r0 * ((r0 * r0) * (r0 * r0))
+        far = r1 >= F32(0.1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            bx = -42.0 * X * Z / r5
+            by = -42.0 * Y * Z / r5
+            bz = 14.0 * (X * X + Y * Y - 2.0 * (Z * Z)) / r5
+        x0[4] = np.where(far, bx, 0.0)
+        x0[5] = np.where(far, by, 0.0)
+        x0[6] = np.where(far, bz, 0.0)
+    else:
+        for m in range(3):
+            x0[4 + m] = bfield[m]
+    pcoef = float(np.float32(np.float32(0.1) * np.float32(0.2)))                      # 0.1*0.2
+    x0[7] = F32(0.2) * 1.0e-3 * shell + pcoef * 1.0e-3
+    return x0, np.ascontiguousarray(r0)
+
+
+def magnetosphere_case(name: str, mx: int, my: int, mz: int, *, xmin: float = F32(-2.1),
+                       xmax: float = F32(5.65), ymin: float = -2.5, ymax: float = 2.5,
+                       zmin: float = -2.5, zmax: float = 2.5, eta0: float = 2.0e-4,
+                       di: float = F32(0.07), hall: bool = True, radial_hall: bool = True,
+                       partial_e: bool = False, inflow: bool = True, rcr: float = 1.0,
+                       cfl: float = 2.0, whistler_dt: bool = False, steps: int = 100,
+                       bfield=None, face=None) -> MagnetosphereCase:
+    """The dipole magnetosphere on a uniform mx x my x mz mesh (defaults:
+    mhd03.for -- box [-2.1, 5.65] x [-2.5, 2.5]^2, eta 2e-4, chall 0.07,
+    rcr 1, cfl 2).  The density, pressure and field arrays also serve as
+    the equilibrium.  ``face`` = the three inflow face arrays (mz+4, my)."""
+    px, sx = uniform_axis(mx, xmin, xmax)
+    py, sy = uniform_axis(my, ymin, ymax)
+    pz, sz = uniform_axis(mz, zmin, zmax)
+    xx, zz, dx, dz = pad2(px), pad2(pz), pad2(sx), pad2(sz)
+    x0, _ = dipole_state(xx, py, zz, bfield)
+    # ghost columns (never read by the step) stay zero, as in the reference
+    x0[:, :2] = 0; x0[:, -2:] = 0; x0[:, :, :, :2] = 0; x0[:, :, :, -2:] = 0
+    eq = [x0[0].copy(), x0[4].copy(), x0[5].copy(), x0[6].copy(), x0[7].copy()]
+    if face is None:
+        face = tuple(np.zeros((mz + 4, my)) for _ in range(3))
+    return MagnetosphereCase(
+        name=name, mx=mx, my=my, mz=mz, periodic_y=False, hall=hall, di=di, eta0=eta0,
+        gamma=F32(5.0 / 3.0), cfl=cfl, whistler_dt=whistler_dt, steps=steps, xx=xx, yy=py,
+        zz=zz, dx=dx, dy=sy, dz=dz, rhoi=eq[0], bxi=eq[1], byi=eq[2], bzi=eq[3], pri=eq[4],
+        x0=x0, magnetosphere=True, inflow=inflow, radial_hall=radial_hall, partial_e=partial_e,
+        rcr=rcr, inflow_face=tuple(np.ascontiguousarray(f, dtype=np.float64) for f in face))
+
+
+def synthetic_face(case_or_sizes, amp: float = 1.0):
+    """smooth inflow face data (mz+4, my) for tests: three different
+    products of sines in y and z"""
+    if isinstance(case_or_sizes, tuple):
+        my, mz = case_or_sizes
+    else:
+        my, mz = case_or_sizes.my, case_or_sizes.mz
+    j = np.arange(my)[None, :] / max(my - 1, 1)
+    k = np.arange(mz + 4)[:, None] / (mz + 3)
+    return (amp * np.sin(np.pi * j) * np.sin(np.pi * k),
+            amp * 0.5 * np.sin(2 * np.pi * j) * np.sin(np.pi * k),
+            amp * 0.7 * np.sin(np.pi * j) * np.cos(np.pi * k))
+
+
+def case_default(**kw) -> MagnetosphereCase:
+    """the reference's default grid (mhd01.for:3-4): 201 x 51 x 105"""
+    return magnetosphere_case("M_201x51x105", 201, 51, 105, **kw)
+
+
+def case_small(mx: int = 33, my: int = 13, mz: int = 25, **kw) -> MagnetosphereCase:
+    """a small magnetosphere for parity tests: box [-2.1, 2.5] x [-1.5, 1.5]^2,
+    coarse enough that the inner sphere covers a few dozen points and the
+    driven face sits two radii upstream"""
+    kw.setdefault("xmax", 2.5)
+    kw.setdefault("ymin", -1.5)
+    kw.setdefault("ymax", 1.5)
+    kw.setdefault("zmin", -1.5)
+    kw.setdefault("zmax", 1.5)
+    return magnetosphere_case(f"Msmall_{mx}x{my}x{mz}", mx, my, mz, **kw)

@@ -30,6 +30,19 @@
  *   - optional whistler speed pi*cht*|B|/(rho*|dx|) added to each directional
  *     signal speed in setdt (t3dsdsc.f:4719-4721 has none).
  *
+ * Magnetosphere mode (the reference's own default case, selected by the
+ * config fields after `cfl`): rr0 = rr1 = sqrt(x^2+y^2+z^2) (t3dsdsc.f:
+ * 773-774); the inner-sphere mask rr1 <= rcr at the start of every boundary
+ * routine; setdt over rr1 >= 1 only; optionally the driven inflow at x-left
+ * (t3dsdsc.f:1711-1772, with rhoi(4,jy,jz) where the reference reads
+ * rhoi(4,jz,jz), an index that leaves the array on a single domain whenever
+ * mz+4 > my), the radial Hall coefficient (1398-1399) and E with B - B_i plus
+ * the curl(v x B0) dipole terms (totalE = .false., 1363-1372, 1460-1548).
+ * Single-precision literals of those routines enter as (double)(float) of
+ * the literal, as a Fortran compiler stores them.  Integer powers r**5,
+ * r**7 expand by binary exponentiation (x7 = (x*(x*x))*((x*x)*(x*x))), the
+ * expansion of constant integer exponents in LLVM and GCC.
+ *
  * Parity pins: see tests/test_oracle_*.py (closed forms, polynomial
  * exactness, invariants, textbook RK4 composition, brute force).
  */
@@ -50,6 +63,11 @@ typedef struct {
     int filters;             /* 1 normal; 0 disables every smoothing pass (operator tests) */
     int bcs;                 /* 1 normal; 0 disables every boundary routine (operator tests) */
     double di, eta0, fmu0, frho0, fpr0, gamma, cfl;
+    int magnetosphere;       /* rr0 = rr1 = |(x,y,z)|: mask rr1 <= rcr, setdt over rr1 >= 1 */
+    int inflow;              /* driven inflow at x-left (t3dsdsc.f:1711-1772) */
+    int radial_hall;         /* cht(rr0) of t3dsdsc.f:1398-1399 (chall = di) */
+    int partial_e;           /* totalE = .false.: E from B - B_i + dipole terms */
+    double rcr;              /* inner-sphere radius (mhd03.for rcr) */
 } orc_config;
 
 typedef struct {
@@ -73,6 +91,10 @@ typedef struct {
     double *fs, *gs, *hs, *w, *u, *v;
     double etan, fmu, frho, fpr;             /* foreta/formu give constants */
     double time;
+    double *rr;                              /* rr0 = rr1 (magnetosphere mode) */
+    double *bxinfc, *byinfc, *bzinfc;        /* inflow face data (my, mzpr), t3dsdsc.f:728-768 */
+    double st_t, st_ddt;                     /* arguments t, ddt, km of the running right() */
+    int st_km;
 } orc;
 
 #define I3(jx, jy, jz) ((size_t)((jz) - 1) * (size_t)o->ny * (size_t)o->nx + \
@@ -191,6 +213,67 @@ static void build_coeffs(orc *o)
 }
 
 /* ------------------------------------------------------------------ */
+/* Magnetosphere-mode helpers                                          */
+/* ------------------------------------------------------------------ */
+/* a single-precision Fortran literal as the compiler stores it */
+#define F32(v) ((double)(float)(v))
+/* face data bxinfc(jy, jz) (my, mzpr) */
+#define FC(a, jy, jz) (a)[(size_t)((jz) - 1) * (size_t)o->ny + (size_t)((jy) - 1)]
+
+static inline double pow5(double x) { double x2 = x * x; return x * (x2 * x2); }
+static inline double pow7(double x) { double x2 = x * x; return (x * x2) * (x2 * x2); }
+
+/* the mask at the start of bndry, bndrytt, bndryc, bndryh, bndryE
+ * (t3dsdsc.f:1692-1707, 2003-2017, 2549-2559, 2709-2719, 2881-2891) */
+static void mask_sphere(orc *o, double *a, int ms, int me)
+{
+    int m, jx, jy, jz;
+    if (!o->c.magnetosphere) return;
+    for (m = ms; m <= me; m++)
+#pragma omp parallel for private(jy, jx)
+        for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+            for (jy = 1; jy <= o->ny; jy++)
+                for (jx = o->x_f - 2; jx <= o->x_l + 2; jx++)
+                    if (F(o->rr, jx, jy, jz) <= o->c.rcr) FM(a, jx, jy, jz, m) = 0.;
+}
+
+/* driven inflow of bndry at x-left, t3dsdsc.f:1711-1772: the state at
+ * jx = 3, 4 is set to the driven values of time t and xdif there to
+ * (x - xfold) / (ddt + 1e-9) (km = 1) or (x - xm) / (ddt + 1e-9) (km = 2);
+ * the density base is rhoi(jx, jy, jz) (the reference reads rhoi(jx, jz, jz)) */
+static void inflow_left(orc *o)
+{
+    const double t = o->st_t, ddt = o->st_ddt;
+    const double *ref = o->st_km == 1 ? o->xfold : o->xm;
+    const double tt = tanh((t - 9.) / 1.) - tanh((t - 39.) / 1.5);
+    const double sq = sqrt(t * t * t * 7.25 * 7.25 * 7.25);
+    const double de = exp(t * 7.25 * F32(0.0069)) + 1.;
+    int jy, jz;
+#pragma omp parallel for private(jy)
+    for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
+        for (jy = 1; jy <= o->ny; jy++) {
+            const double z = V1(o->zz, jz), y = V1(o->yy, jy);
+            const double zd = tanh((z + F32(0.6)) / F32(0.15)) - tanh((z - F32(0.6)) / F32(0.15));
+            const double yd = tanh((y + F32(0.4)) / F32(0.15)) - tanh((y - F32(0.4)) / F32(0.15));
+            const double zd2 = tanh((z + F32(0.2)) / F32(0.15)) - tanh((z - F32(0.2)) / F32(0.15));
+            const double yd2 = tanh((y + F32(0.3)) / F32(0.15)) - tanh((y - F32(0.3)) / F32(0.15));
+            for (int jx = 4; jx >= 3; jx--) {
+                double *X = o->x;
+                FM(X, jx, jy, jz, 1) = F(o->rhoi, jx, jy, jz) + 1. * 0.5 * tt * 0.5 * zd * 0.5 * yd;
+                FM(X, jx, jy, jz, 2) = F32(0.8) * FM(X, jx, jy, jz, 1) * 0.5 * tt * 0.5 * zd2 * 0.5 * yd2;
+                FM(X, jx, jy, jz, 3) = 0.;
+                FM(X, jx, jy, jz, 4) = 0.;
+                FM(X, jx, jy, jz, 5) = F(o->bxi, jx, jy, jz) + FC(o->bxinfc, jy, jz) * F32(0.0017) * sq / de;
+                FM(X, jx, jy, jz, 6) = F(o->byi, jx, jy, jz) + FC(o->byinfc, jy, jz) * F32(0.0017) * sq / de;
+                FM(X, jx, jy, jz, 7) = F(o->bzi, jx, jy, jz) + FC(o->bzinfc, jy, jz) * F32(0.0017) * sq / de;
+                FM(X, jx, jy, jz, 8) = F(o->pri, jx, jy, jz) + F32(0.2) * 0.5 * tt * 0.5 * zd * 0.5 * yd;
+                for (int m = 1; m <= 8; m++)
+                    FM(o->xdif, jx, jy, jz, m) = (FM(X, jx, jy, jz, m) - FM(ref, jx, jy, jz, m)) / (ddt + 1.e-9);
+            }
+        }
+}
+
+/* ------------------------------------------------------------------ */
 /* Boundary routines                                                   */
 /* ------------------------------------------------------------------ */
 /* periodic y, two ghost rows, whole extended x/z range (replaces the
@@ -216,7 +299,7 @@ static void bndry_aux(orc *o, double *a)
 {
     int m, jx, jy, jz, my = o->ny, xl = o->xl, zl = o->zl;
     if (!o->c.bcs) return;
-    /* mask t3dsdsc.f:2549-2557: rr1 > rcr everywhere in the Harris setup */
+    mask_sphere(o, a, 1, 3);                            /* 2549-2557 */
     /* x: t3dsdsc.f:2558-2577 */
     for (m = 1; m <= 3; m++)
 #pragma omp parallel for private(jy)
@@ -269,14 +352,17 @@ static void bndry(orc *o)
     int m, jx, jy, jz, my = o->ny, xl = o->xl, zl = o->zl;
     double *a = o->xdif;
     if (!o->c.bcs) return;
-    /* mask 1692-1707 never active; message4 1708 moves nothing */
+    mask_sphere(o, a, 1, 8);                            /* 1692-1707; message4 1708 moves nothing */
+    if (o->c.inflow) inflow_left(o);                    /* 1711-1772 */
     for (m = 1; m <= 8; m++)
 #pragma omp parallel for private(jy)
         for (jz = o->z_f - 2; jz <= o->z_l + 2; jz++)
             for (jy = 1; jy <= my; jy++) {
-                /* x-left: x-right rule 1780-1781 mirrored */
-                FM(a, 3, jy, jz, m) = 0;
-                FM(a, 4, jy, jz, m) = 0.5 * FM(a, 5, jy, jz, m);
+                if (!o->c.inflow) {
+                    /* x-left: x-right rule 1780-1781 mirrored */
+                    FM(a, 3, jy, jz, m) = 0;
+                    FM(a, 4, jy, jz, m) = 0.5 * FM(a, 5, jy, jz, m);
+                }
                 /* x-right 1775-1783 */
                 FM(a, xl, jy, jz, m) = 0;
                 FM(a, xl - 1, jy, jz, m) = 0.5 * FM(a, xl - 2, jy, jz, m);
@@ -319,6 +405,7 @@ static void bndrytt(orc *o, int ms, int me)
     int m, jx, jy, jz, my = o->ny, xl = o->xl, zl = o->zl;
     double *a = o->xm;
     if (!o->c.bcs) return;
+    mask_sphere(o, a, 1, 8);                            /* 2003-2017 */
     /* x 2019-2046 */
     for (m = ms; m <= me; m++)
 #pragma omp parallel for private(jy)
@@ -545,18 +632,31 @@ static void right(orc *o, double t, int init)
                 FM(XD, jx, jy, jz, 2) = FM(XD, jx, jy, jz, 2) - FM(o->xresd, jx, jy, jz, 1) + c2 * x7 - c3 * x6;
                 FM(XD, jx, jy, jz, 3) = FM(XD, jx, jy, jz, 3) - FM(o->xresd, jx, jy, jz, 2) + c3 * x5 - c1 * x7;
                 FM(XD, jx, jy, jz, 4) = FM(XD, jx, jy, jz, 4) - FM(o->xresd, jx, jy, jz, 3) + c1 * x6 - c2 * x5;
-                FM(E, jx, jy, jz, 1) = (-x3 * x7 + x4 * x6) / x1 + eta * c1;
-                FM(E, jx, jy, jz, 2) = (-x4 * x5 + x2 * x7) / x1 + eta * c2;
-                FM(E, jx, jy, jz, 3) = (-x2 * x6 + x3 * x5) / x1 + eta * c3;
+                if (!o->c.partial_e) {
+                    FM(E, jx, jy, jz, 1) = (-x3 * x7 + x4 * x6) / x1 + eta * c1;
+                    FM(E, jx, jy, jz, 2) = (-x4 * x5 + x2 * x7) / x1 + eta * c2;
+                    FM(E, jx, jy, jz, 3) = (-x2 * x6 + x3 * x5) / x1 + eta * c3;
+                } else {
+                    /* totalE = .false. (1370-1386): the field perturbation only */
+                    double b1 = F(o->bxi, jx, jy, jz), b2 = F(o->byi, jx, jy, jz), b3 = F(o->bzi, jx, jy, jz);
+                    FM(E, jx, jy, jz, 1) = (-x3 * (x7 - b3) + x4 * (x6 - b2)) / x1 + eta * c1;
+                    FM(E, jx, jy, jz, 2) = (-x4 * (x5 - b1) + x2 * (x7 - b3)) / x1 + eta * c2;
+                    FM(E, jx, jy, jz, 3) = (-x2 * (x6 - b2) + x3 * (x5 - b1)) / x1 + eta * c3;
+                }
             }
     avrg_aux(o, E, COEF_RADIAL, 4);                                 /* 1389 */
 
     if (o->c.hall) {                                                /* 1391-1433 */
-        double cht = o->c.di;                                       /* uniform d_i */
 #pragma omp parallel for private(jy, jx)
         for (jz = o->z_f; jz <= o->z_l; jz++)
             for (jy = 3; jy <= o->ny - 2; jy++)
                 for (jx = o->x_f; jx <= o->x_l; jx++) {
+                    /* uniform d_i, or the radial profile of 1398-1399 with chall = d_i */
+                    double cht = o->c.di;
+                    if (o->c.radial_hall) {
+                        double r0 = F(o->rr, jx, jy, jz);
+                        cht = 0.5 * o->c.di * (tanh((r0 - F32(1.1)) / 8.) - tanh((r0 - 2.) / 10.));
+                    }
                     double x1 = FM(X, jx, jy, jz, 1), x5 = FM(X, jx, jy, jz, 5), x6 = FM(X, jx, jy, jz, 6);
                     double x7 = FM(X, jx, jy, jz, 7);
                     double c1 = FM(C, jx, jy, jz, 1), c2 = FM(C, jx, jy, jz, 2), c3 = FM(C, jx, jy, jz, 3);
@@ -583,6 +683,31 @@ static void right(orc *o, double t, int init)
                 FM(XD, jx, jy, jz, 5) = -DYM(E, 3) + DZM(E, 2);
                 FM(XD, jx, jy, jz, 6) = -DZM(E, 1) + DXM(E, 3);
                 FM(XD, jx, jy, jz, 7) = -DXM(E, 2) + DYM(E, 1);
+                if (o->c.partial_e) {
+                    /* totalE = .false. (1460-1548): + curl(v x B0) with the
+                     * analytic derivatives of the unit dipole */
+                    double xg = V1(o->xx, jx), yg = V1(o->yy, jy), zg = V1(o->zz, jz);
+                    double r0 = F(o->rr, jx, jy, jz), r2 = r0 * r0, r7 = pow7(r0);
+                    double b1 = F(o->bxi, jx, jy, jz), b2 = F(o->byi, jx, jy, jz), b3 = F(o->bzi, jx, jy, jz);
+                    double u = F(o->fs, jx, jy, jz), v = F(o->gs, jx, jy, jz), w = F(o->hs, jx, jy, jz);
+                    double dv = DX(o->fs) + DY(o->gs) + DZ(o->hs);
+                    double q2 = xg * xg + yg * yg - 2. * (zg * zg);
+                    double a1 = -3. * zg * (r2 - 5. * (xg * xg)) / r7;
+                    double a2 = 3. * xg * zg * 5. * yg / r7;
+                    double a3 = -3. * xg * (r2 - 5. * (zg * zg)) / r7;
+                    double e1 = 3. * yg * zg * 5. * xg / r7;
+                    double e2 = -3. * zg * (r2 - 5. * (yg * yg)) / r7;
+                    double e3 = -3. * yg * (r2 - 5. * (zg * zg)) / r7;
+                    double g1 = (2. * xg * r2 - 5. * xg * q2) / r7;
+                    double g2 = (2. * yg * r2 - 5. * yg * q2) / r7;
+                    double g3 = (-4. * zg * r2 - 5. * zg * q2) / r7;
+                    FM(XD, jx, jy, jz, 5) = FM(XD, jx, jy, jz, 5) +
+                        (-b1 * dv + (b1 * DX(o->fs) + b2 * DY(o->fs) + b3 * DZ(o->fs)) - (u * a1 + v * a2 + w * a3));
+                    FM(XD, jx, jy, jz, 6) = FM(XD, jx, jy, jz, 6) +
+                        (-b2 * dv + (b1 * DX(o->gs) + b2 * DY(o->gs) + b3 * DZ(o->gs)) - (u * e1 + v * e2 + w * e3));
+                    FM(XD, jx, jy, jz, 7) = FM(XD, jx, jy, jz, 7) +
+                        (-b3 * dv + (b1 * DX(o->hs) + b2 * DY(o->hs) + b3 * DZ(o->hs)) - (u * g1 + v * g2 + w * g3));
+                }
                 FM(XD, jx, jy, jz, 8) = -DXM(X, 8) * F(o->fs, jx, jy, jz) - DYM(X, 8) * F(o->gs, jx, jy, jz) -
                                         DZM(X, 8) * F(o->hs, jx, jy, jz) +
                                         (-DX(o->fs) - DY(o->gs) - DZ(o->hs)) * gam * FM(X, jx, jy, jz, 8) +
@@ -638,18 +763,22 @@ static void stepon(orc *o, double time, double dt)
     size_t n3 = o->n3;
     double *X = o->x, *XM = o->xm, *XF = o->xfold, *XD = o->xdif;
     for (m = 0; m < 8; m++) EXT_LOOP(XF[m * n3 + q] = X[m * n3 + q];);       /* 1079-1084 */
+    o->st_t = time + dt / 2; o->st_ddt = dt / 2.; o->st_km = 1;
     right(o, time + dt / 2, 0);                                             /* 1086-1089 */
     for (m = 0; m < 8; m++) EXT_LOOP(                                        /* 1090-1097 */
         XM[m * n3 + q] = XF[m * n3 + q] + XD[m * n3 + q] * dt / 6.;
         X[m * n3 + q] = XF[m * n3 + q] + XD[m * n3 + q] * dt / 2.;);
+    o->st_t = time + dt / 2; o->st_ddt = dt / 2.; o->st_km = 1;
     right(o, time + dt / 2, 0);                                             /* 1102-1104 */
     for (m = 0; m < 8; m++) EXT_LOOP(                                        /* 1105-1111 */
         XM[m * n3 + q] = XM[m * n3 + q] + XD[m * n3 + q] * dt / 3.;
         X[m * n3 + q] = XF[m * n3 + q] + XD[m * n3 + q] * dt / 2.;);
+    o->st_t = time + dt; o->st_ddt = dt; o->st_km = 1;
     right(o, time + dt, 0);                                                 /* 1113-1115 */
     for (m = 0; m < 8; m++) EXT_LOOP(                                        /* 1116-1122 */
         XM[m * n3 + q] = XM[m * n3 + q] + XD[m * n3 + q] * dt / 3.;
         X[m * n3 + q] = XF[m * n3 + q] + XD[m * n3 + q] * dt;);
+    o->st_t = time + dt; o->st_ddt = dt / 6.; o->st_km = 2;
     right(o, time + dt, 0);                                                 /* 1124-1126 */
     for (m = 0; m < 8; m++) EXT_LOOP(                                        /* 1127-1132 */
         X[m * n3 + q] = XM[m * n3 + q] + XD[m * n3 + q] * dt / 6.;);
@@ -709,6 +838,15 @@ orc *orc_create(const orc_config *cfg, const double *xx, const double *yy, const
     o->fs = zalloc(o->n3); o->gs = zalloc(o->n3); o->hs = zalloc(o->n3);
     o->w = zalloc(o->n3); o->u = zalloc(o->n3); o->v = zalloc(o->n3);
     build_coeffs(o);
+    o->rr = zalloc(o->n3);
+    o->bxinfc = zalloc((size_t)o->ny * o->nz); o->byinfc = zalloc((size_t)o->ny * o->nz);
+    o->bzinfc = zalloc((size_t)o->ny * o->nz);
+    for (int jz = 1; jz <= o->nz; jz++)                  /* t3dsdsc.f:771-775 */
+        for (int jy = 1; jy <= o->ny; jy++)
+            for (int jx = 1; jx <= o->nx; jx++)
+                F(o->rr, jx, jy, jz) = sqrt((V1(o->xx, jx) - 0.0) * (V1(o->xx, jx) - 0.0) +
+                                            V1(o->yy, jy) * V1(o->yy, jy) + V1(o->zz, jz) * V1(o->zz, jz));
+    o->st_t = 0.0; o->st_ddt = 1.e-9; o->st_km = 1;
     /* foreta (6489-6511) and formu (6515-6532): constants */
     o->etan = cfg->eta0; o->fmu = cfg->fmu0; o->frho = cfg->frho0; o->fpr = cfg->fpr0;
     return o;
@@ -724,7 +862,7 @@ void orc_destroy(orc *o)
                     &o->byp, &o->cyp, &o->aym, &o->bym, &o->cym, &o->azp, &o->bzp, &o->czp, &o->azm,
                     &o->bzm, &o->czm, &o->x, &o->xm, &o->xfold, &o->xdif, &o->rhoi, &o->bxi, &o->byi,
                     &o->bzi, &o->pri, &o->xresd, &o->cur, &o->efld, &o->xdh, &o->fs, &o->gs, &o->hs,
-                    &o->w, &o->u, &o->v};
+                    &o->w, &o->u, &o->v, &o->rr, &o->bxinfc, &o->byinfc, &o->bzinfc};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(*p[i]);
     free(o);
 }
@@ -785,12 +923,25 @@ void orc_init_residual(orc *o)
     int m;
     memset(o->xresd, 0, sizeof(double) * 3 * n3);
     for (m = 0; m < 8; m++) EXT_LOOP(o->xfold[m * n3 + q] = o->x[m * n3 + q];);
+    o->st_t = 0.0; o->st_ddt = 1.e-9; o->st_km = 1;               /* right(0d0, 1.d-9, 1) */
     right(o, 0.0, 1);
     for (m = 0; m < 3; m++) EXT_LOOP(o->xresd[m * n3 + q] = o->xdif[(m + 1) * n3 + q];);
 }
 
 /* one call of right(t) for the tests (t = 0 skips the final avrg as in 1669) */
-void orc_rhs(orc *o, double t) { right(o, t, 0); }
+void orc_rhs(orc *o, double t) { o->st_t = t; right(o, t, 0); }
+/* right(t, ddt, km) with the inflow's stage arguments */
+void orc_rhs_stage(orc *o, double t, double ddt, int km)
+{
+    o->st_t = t; o->st_ddt = ddt; o->st_km = km;
+    right(o, t, 0);
+}
+/* inflow face data bxinfc, byinfc, bzinfc, each (my, mz+4) Fortran order */
+void orc_set_inflow_face(orc *o, const double *bx, const double *by, const double *bz)
+{
+    size_t b = sizeof(double) * (size_t)o->ny * o->nz;
+    memcpy(o->bxinfc, bx, b); memcpy(o->byinfc, by, b); memcpy(o->bzinfc, bz, b);
+}
 void orc_current(orc *o) { current(o); }
 void orc_current_raw(orc *o) { current_raw(o); }
 void orc_step(orc *o, double time, double dt) { stepon(o, time, dt); }
@@ -818,6 +969,7 @@ double orc_compute_dt(const orc *o)
     for (jz = o->zf; jz <= o->z_l; jz++)
         for (int jy = 1; jy <= o->ny - 1; jy++)
             for (int jx = o->xf; jx <= o->x_l; jx++) {
+                if (o->c.magnetosphere && !(F(o->rr, jx, jy, jz) >= 1.)) continue;   /* 4712 */
                 double x1 = FM(o->x, jx, jy, jz, 1);
                 double vx = FM(o->x, jx, jy, jz, 2) / x1;
                 double vy = FM(o->x, jx, jy, jz, 3) / x1;

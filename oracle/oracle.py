@@ -22,7 +22,9 @@ class OrcConfig(C.Structure):
                 ("hall", C.c_int), ("whistler_dt", C.c_int), ("filters", C.c_int),
                 ("bcs", C.c_int), ("di", C.c_double), ("eta0", C.c_double),
                 ("fmu0", C.c_double), ("frho0", C.c_double), ("fpr0", C.c_double),
-                ("gamma", C.c_double), ("cfl", C.c_double)]
+                ("gamma", C.c_double), ("cfl", C.c_double),
+                ("magnetosphere", C.c_int), ("inflow", C.c_int), ("radial_hall", C.c_int),
+                ("partial_e", C.c_int), ("rcr", C.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -53,6 +55,8 @@ def lib():
         L.orc_set_equilibrium.argtypes = [P, D, D, D, D, D]
         L.orc_init_residual.argtypes = [P]
         L.orc_rhs.argtypes = [P, C.c_double]
+        L.orc_rhs_stage.argtypes = [P, C.c_double, C.c_double, C.c_int]
+        L.orc_set_inflow_face.argtypes = [P, D, D, D]
         L.orc_current.argtypes = [P]
         L.orc_current_raw.argtypes = [P]
         L.orc_step.argtypes = [P, C.c_double, C.c_double]
@@ -94,7 +98,11 @@ class Oracle:
                         hall=int(case.hall), whistler_dt=int(case.whistler_dt),
                         filters=int(filters), bcs=int(bcs), di=case.di, eta0=case.eta0,
                         fmu0=case.fmu0, frho0=case.frho0, fpr0=case.fpr0, gamma=case.gamma,
-                        cfl=case.cfl)
+                        cfl=case.cfl, magnetosphere=int(getattr(case, "magnetosphere", False)),
+                        inflow=int(getattr(case, "inflow", False)),
+                        radial_hall=int(getattr(case, "radial_hall", False)),
+                        partial_e=int(getattr(case, "partial_e", False)),
+                        rcr=float(getattr(case, "rcr", 0.0)))
         self._keep = [np.ascontiguousarray(a, dtype=np.float64)
                       for a in (case.xx, case.yy, case.zz, case.dx, case.dy, case.dz)]
         self.h = lib().orc_create(C.byref(cfg), *[_p(a) for a in self._keep])
@@ -102,6 +110,9 @@ class Oracle:
         eq = [np.ascontiguousarray(a, dtype=np.float64)
               for a in (case.rhoi, case.bxi, case.byi, case.bzi, case.pri)]
         lib().orc_set_equilibrium(self.h, *[_p(a) for a in eq])
+        face = getattr(case, "inflow_face", None)
+        if face is not None:
+            self.set_inflow_face(*face)
         self.set_state(case.x0)
 
     def close(self):
@@ -158,6 +169,17 @@ class Oracle:
 
     def rhs(self, t: float):
         lib().orc_rhs(self.h, t)
+
+    def rhs_stage(self, t: float, ddt: float, km: int):
+        """right(t, ddt, km): the stage arguments the driven inflow reads"""
+        lib().orc_rhs_stage(self.h, t, ddt, int(km))
+
+    def set_inflow_face(self, bx, by, bz):
+        """inflow face data bxinfc, byinfc, bzinfc: each (mz+4, my) C order"""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (bx, by, bz)]
+        for a in arrs:
+            assert a.shape == (self.case.mz + 4, self.case.my)
+        lib().orc_set_inflow_face(self.h, *[_p(a) for a in arrs])
 
     def current(self):
         lib().orc_current(self.h)
