@@ -23,6 +23,7 @@
 #include "mhd_gpu.h"
 #include "mhd_kernels.h"
 #include "mhd_adjoint.h"
+#include "mhd_magneto.h"
 
 using namespace mhd;
 
@@ -90,8 +91,9 @@ __global__ void k_copy(long n, const double *__restrict__ a, double *__restrict_
 // jx = 3..mx (k = 0..NZ-3, j = 0..NY-2, i = 0..NX-3); with periodic y the
 // image rows repeat unique rows, so the unique rows give the same minimum
 // ---------------------------------------------------------------------------
+// rr (magnetosphere mode, else null): only points with rr1 >= 1 count (t3dsdsc.f:4712)
 __global__ void k_dt_partial(Geo g, const double *__restrict__ X, double eta, double fmu, double cfl,
-                             double gam, double wcoef, double *__restrict__ part)
+                             double gam, double wcoef, const double *__restrict__ rr, double *__restrict__ part)
 {
     __shared__ double sm[256];
     int j0 = g.periodic ? 2 : 0, j1 = g.periodic ? g.NY - 3 : g.NY - 2;
@@ -102,6 +104,7 @@ __global__ void k_dt_partial(Geo g, const double *__restrict__ X, double eta, do
         int i = (int)(t % ni); long r = t / ni;
         int j = j0 + (int)(r % nj); int k = (int)(r / nj);
         long q = IDX(g, i, j, k), S = g.S;
+        if (rr && !(rr[q] >= 1.)) continue;
         // quotients with a shared or precomputed rounded reciprocal (divc:
         // equal to the IEEE quotients); (eta + fmu) / dd / 2 is a per-index
         // table computed the same way on the host
@@ -346,6 +349,11 @@ struct mhd_handle {
     double *dwx = nullptr, *dwy = nullptr, *dwz = nullptr;
     std::string err;
     int dev = -1;                          // HIP device the handle was created on
+    // magnetosphere mode: |r| per point, the Hall coefficient per point (radial
+    // profile), v = m / rho, the inflow profiles and face data
+    bool mag = false;
+    double *rr = nullptr, *chtab = nullptr, *vel = nullptr, *mgeo = nullptr;
+    const double *zd = nullptr, *yd = nullptr, *zd2 = nullptr, *yd2 = nullptr, *face = nullptr;
     bool have_eq = false;
     int kc = 16;                           // z planes per tile chunk
     int fuse_rk = 1;                       // RK update fused into the last avrg pass
@@ -849,10 +857,144 @@ static void dt_kernels(mhd_handle *h, int advance_time, double *part = nullptr)
     double wcoef = (c.whistler_dt && c.hall && c.di != 0.0) ? 3.141592654 * c.di : 0.0;
     if (!part) part = h->red;
     PROF_BEGIN(PT_DT);
+    const double *rr = h->mag && c.magnetosphere ? h->rr : nullptr;
     hipLaunchKernelGGL(k_dt_partial, dim3(h->nred), dim3(256), 0, h->stream, h->g, h->x, c.eta0, c.fmu0, c.cfl,
-                       c.gamma, wcoef, part);
+                       c.gamma, wcoef, rr, part);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, h->stream, part, h->nred, h->dtv, advance_time);
     PROF_END(PT_DT);
+}
+
+// ---- magnetosphere mode (kernels in mhd_magneto.h) ----
+static Mag mag_args(const mhd_handle *h)
+{
+    Mag a;
+    a.rr = h->rr;
+    a.cht = h->chtab;
+    a.rcr = h->cfg.rcr;
+    a.mask = h->cfg.magnetosphere != 0;
+    a.inflow = h->cfg.inflow != 0;
+    a.partial = h->cfg.partial_e != 0;
+    a.zd = h->zd; a.yd = h->yd; a.zd2 = h->zd2; a.yd2 = h->yd2;
+    a.fb = h->face;
+    return a;
+}
+
+// time factors of the drive at stage time t (t3dsdsc.f:1714-1768)
+static Stage mag_stage(const mhd_handle *h, double t, double ddt, int km)
+{
+    Stage s;
+    s.tt = tanh((t - 9.) / 1.) - tanh((t - 39.) / 1.5);
+    s.sq = sqrt(t * t * t * 7.25 * 7.25 * 7.25);
+    s.de = exp(t * 7.25 * (double)0.0069f) + 1.;
+    s.ddt = ddt;
+    s.ref = km == 1 ? h->xfold : h->xm;
+    s.x = h->x;
+    return s;
+}
+
+static dim3 vol_grid(const mhd_handle *h) { return dim3((unsigned)blocks((long)h->NX * h->NY * h->NZ, 256)); }
+
+// a boundary routine: mask (whole volume), then x, y, z rules
+template <int TYPE>
+static void mag_fill(mhd_handle *h, double *A, int nf, const Stage &s)
+{
+    const Mag a = mag_args(h);
+    const Geo &g = h->g;
+    if (a.mask) hipLaunchKernelGGL(km_mask, vol_grid(h), dim3(256), 0, h->stream, g, a, A, nf);
+    hipLaunchKernelGGL((km_fill_x<TYPE>), dim3(blocks((long)g.NY * g.NZ, 256)), dim3(256), 0, h->stream, g, a, s,
+                       h->eq, A, nf);
+    hipLaunchKernelGGL((km_fill_y<TYPE>), dim3(blocks((long)g.NX * g.NZ, 256)), dim3(256), 0, h->stream, g, A, nf);
+    hipLaunchKernelGGL((km_fill_z<TYPE>), dim3(blocks((long)g.NX * g.NY, 256)), dim3(256), 0, h->stream, g, A, nf);
+}
+
+// kk sweeps of nf fields, each followed by the boundary routine TYPE;
+// a and b alternate, the result is returned
+template <int KIND, int TYPE>
+static double *mag_smooth(mhd_handle *h, double *a, double *b, int nf, int kk, const Stage &s)
+{
+    for (int p = 0; p < kk; p++) {
+        hipLaunchKernelGGL((km_sweep<KIND>), vol_grid(h), dim3(256), 0, h->stream, h->g, a, b, nf);
+        mag_fill<TYPE>(h, b, nf, s);
+        std::swap(a, b);
+    }
+    return a;
+}
+
+// current (t3dsdsc.f:2452-2500) of state X into h->cur
+static void mag_current(mhd_handle *h, const double *X)
+{
+    const Stage none = mag_stage(h, 0.0, 1.0, 1);
+    PROF_BEGIN(PT_CURL);
+    hipLaunchKernelGGL(km_curl, vol_grid(h), dim3(256), 0, h->stream, h->g, X, h->eq, h->cA);
+    mag_fill<BC_AUX>(h, h->cA, 3, none);
+    h->cur = mag_smooth<KIND_RADIAL, BC_AUX>(h, h->cA, h->cB, 3, 4, none);
+    PROF_END(PT_CURL);
+}
+
+// right(t, ddt, km) (t3dsdsc.f:1297-1673) of the state h->x; result in h->xd
+static void mag_rhs(mhd_handle *h, double t, double ddt, int km, bool smooth_xdif, bool init)
+{
+    const mhd_config &c = h->cfg;
+    const double eta = init ? 0.0 : c.eta0;
+    const Mag a = mag_args(h);
+    const Stage st = mag_stage(h, t, ddt, km);
+    const double *X = h->x;
+    mag_current(h, X);
+    PROF_BEGIN(PT_RHSA);
+    hipLaunchKernelGGL(km_rhs_a, vol_grid(h), dim3(256), 0, h->stream, h->g, a, X, h->eq, h->cur, h->xresd, eta,
+                       h->xdA, h->eA, h->vel);
+    double *E = mag_smooth<KIND_RADIAL, BC_AUX>(h, h->eA, h->eB, 3, 4, st);
+    PROF_END(PT_RHSA);
+    if (c.hall) {
+        PROF_BEGIN(PT_EIG1);
+        hipLaunchKernelGGL(km_hall, vol_grid(h), dim3(256), 0, h->stream, h->g, a, X, h->cur, c.di, h->hA);
+        mag_fill<BC_AUX>(h, h->hA, 3, st);
+        const double *H = mag_smooth<KIND_EIGHTH, BC_AUX>(h, h->hA, h->hB, 3, 5, st);
+        hipLaunchKernelGGL(km_add3, dim3(blocks(h->S, 256)), dim3(256), 0, h->stream, h->S, h->S, E, H);
+        PROF_END(PT_EIG1);
+    }
+    PROF_BEGIN(PT_RHSB);
+    hipLaunchKernelGGL(km_rhs_b, vol_grid(h), dim3(256), 0, h->stream, h->g, a, X, h->eq, h->cur, E, h->vel, eta,
+                       c.gamma, h->xdA);
+    mag_fill<BC_STATE>(h, h->xdA, 8, st);
+    PROF_END(PT_RHSB);
+    double *r = h->xdA;
+    if (smooth_xdif) {
+        PROF_BEGIN(PT_ST1);
+        r = mag_smooth<KIND_STATE, BC_STATE>(h, h->xdA, h->xdB, 8, 2, st);
+        PROF_END(PT_ST1);
+    }
+    h->xd = r;
+}
+
+// stepon (t3dsdsc.f:1063-1171) with dt in h->dtv[0]; the state is copied to
+// xfold (the driven columns of x are overwritten inside right())
+static void mag_step(mhd_handle *h, double time, double dt)
+{
+    HCHK_V(hipMemcpyAsync(h->xfold, h->x, 8 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    const double ts[4] = {time + dt / 2, time + dt / 2, time + dt, time + dt};
+    const double dd[4] = {dt / 2., dt / 2., dt, dt / 6.};
+    const int kms[4] = {1, 1, 1, 2};
+    for (int s = 0; s < 4; s++) {
+        mag_rhs(h, ts[s], dd[s], kms[s], true, false);
+        rk(h, s);
+    }
+    // perturbation smoothing (1135-1165): xm = x - eq; avrgtt x2; x = xm + eq
+    const Stage none = mag_stage(h, 0.0, 1.0, 1);
+    PROF_BEGIN(PT_POST);
+    hipLaunchKernelGGL(km_eq, dim3(blocks(h->S, 256)), dim3(256), 0, h->stream, h->S, h->S, h->x, h->eq, -1, h->xm);
+    double *r = mag_smooth<KIND_STATE, BC_TT>(h, h->xm, h->xdB, 8, 2, none);
+    hipLaunchKernelGGL(km_eq, dim3(blocks(h->S, 256)), dim3(256), 0, h->stream, h->S, h->S, r, h->eq, +1, h->x);
+    PROF_END(PT_POST);
+    h->xd = nullptr;
+    mag_current(h, h->x);
+}
+
+// current of either path
+static void any_current(mhd_handle *h, const double *X)
+{
+    if (h->mag) mag_current(h, X);
+    else current(h, X);
 }
 
 // ---- host <-> device layout conversion ----
@@ -1090,6 +1232,9 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
         bool xbad = cfg->mx > TX && (cfg->mx % TX == 1 || cfg->mx % TX == 2);
         bool ybad = !cfg->periodic_y && nyt > TY && (nyt % TY == 1 || nyt % TY == 2);
         if (vd && (xbad || ybad)) return MHD_ERR_ARG;
+        // magnetosphere mode: non-periodic y, no viscosity or diffusion
+        bool mag = cfg->magnetosphere || cfg->inflow || cfg->radial_hall || cfg->partial_e;
+        if (mag && (cfg->periodic_y || vd)) return MHD_ERR_ARG;
     }
     mhd_handle *h = new mhd_handle();
     h->cfg = *cfg;
@@ -1213,6 +1358,44 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     HCHK(hipMalloc(&h->red, std::max<long>(4096, nrows) * sizeof(double)));
     HCHK(hipMalloc(&h->dtv, 8 * sizeof(double)));
     HCHK(hipMemset(h->dtv, 0, 8 * sizeof(double)));
+    h->mag = cfg->magnetosphere || cfg->inflow || cfg->radial_hall || cfg->partial_e;
+    if (h->mag) {
+        // |r| (t3dsdsc.f:773-774) and the radial Hall coefficient (1398-1399)
+        // per point, the inflow profiles per plane / row (1714-1768), the
+        // face data (zero until mhd_set_inflow_face), v = m / rho
+        std::vector<double> rr(h->S, 0.0), ch;
+        for (int k = 0; k < NZ; k++)
+            for (int j = 0; j < NY; j++)
+                for (int i = 0; i < NX; i++)
+                    rr[IDX(g, i, j, k)] = sqrt((X[i] - 0.0) * (X[i] - 0.0) + Y[j] * Y[j] + Z[k] * Z[k]);
+        if (hipMalloc(&h->rr, fb) != hipSuccess || hipMalloc(&h->vel, 3 * fb) != hipSuccess) {
+            set_err(h, "device allocation failed"); *out = h; return MHD_ERR_NOMEM;
+        }
+        HCHK(hipMemcpy(h->rr, rr.data(), fb, hipMemcpyHostToDevice));
+        HCHK(hipMemset(h->vel, 0, 3 * fb));
+        if (cfg->radial_hall) {
+            ch.assign(h->S, 0.0);
+            for (size_t q = 0; q < ch.size(); q++)
+                ch[q] = 0.5 * cfg->di * (tanh((rr[q] - (double)1.1f) / 8.) - tanh((rr[q] - 2.) / 10.));
+            if (hipMalloc(&h->chtab, fb) != hipSuccess) { set_err(h, "device allocation failed"); *out = h; return MHD_ERR_NOMEM; }
+            HCHK(hipMemcpy(h->chtab, ch.data(), fb, hipMemcpyHostToDevice));
+        }
+        std::vector<double> mg(2 * (size_t)NZ + 2 * (size_t)NY + 3 * (size_t)NZ * NY, 0.0);
+        const double c06 = (double)0.6f, c04 = (double)0.4f, c02 = (double)0.2f, c03 = (double)0.3f,
+                     c15 = (double)0.15f;
+        for (int k = 0; k < NZ; k++) {
+            mg[k] = tanh((Z[k] + c06) / c15) - tanh((Z[k] - c06) / c15);
+            mg[NZ + NY + k] = tanh((Z[k] + c02) / c15) - tanh((Z[k] - c02) / c15);
+        }
+        for (int j = 0; j < NY; j++) {
+            mg[NZ + j] = tanh((Y[j] + c04) / c15) - tanh((Y[j] - c04) / c15);
+            mg[2 * NZ + NY + j] = tanh((Y[j] + c03) / c15) - tanh((Y[j] - c03) / c15);
+        }
+        HCHK(hipMalloc(&h->mgeo, mg.size() * sizeof(double)));
+        HCHK(hipMemcpy(h->mgeo, mg.data(), mg.size() * sizeof(double), hipMemcpyHostToDevice));
+        h->zd = h->mgeo; h->yd = h->mgeo + NZ; h->zd2 = h->mgeo + NZ + NY; h->yd2 = h->mgeo + 2 * NZ + NY;
+        h->face = h->mgeo + 2 * NZ + 2 * NY;
+    }
     h->xd = h->xdA;
     h->cur = h->cA;
     // z planes per chunk of the tile kernels: 16 on volumes of 2^25 points
@@ -1293,7 +1476,13 @@ int mhd_init_residual(mhd_handle *h)
     if (!h) return MHD_ERR_ARG;
     if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
     HCHK(hipMemsetAsync(h->xresd, 0, 3 * h->S * sizeof(double), h->stream));
-    rhs(h, h->x, false, true, -1);
+    if (h->mag) {
+        // xfold = x, right(0d0, 1.d-9, 1) (t3dsdsc.f:821-829)
+        HCHK(hipMemcpyAsync(h->xfold, h->x, 8 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        mag_rhs(h, 0.0, 1.e-9, 1, false, true);
+    } else {
+        rhs(h, h->x, false, true, -1);
+    }
     HCHK(hipMemcpyAsync(h->xresd, h->xd + h->S, 3 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     HCHK(hipStreamSynchronize(h->stream));
     HCHK(hipGetLastError());
@@ -1302,10 +1491,31 @@ int mhd_init_residual(mhd_handle *h)
 
 int mhd_rhs(mhd_handle *h, double t)
 {
-    if (!h) return MHD_ERR_ARG;
-    rhs(h, h->x, t != 0.0, false, -1);
+    return mhd_rhs_stage(h, t, 1.e-9, 1);
+}
+
+int mhd_rhs_stage(mhd_handle *h, double t, double ddt, int km)
+{
+    if (!h || (km != 1 && km != 2)) return MHD_ERR_ARG;
+    if (h->mag) mag_rhs(h, t, ddt, km, t != 0.0, false);
+    else rhs(h, h->x, t != 0.0, false, -1);
     HCHK(hipStreamSynchronize(h->stream));
     HCHK(hipGetLastError());
+    return MHD_OK;
+}
+
+int mhd_set_inflow_face(mhd_handle *h, const double *bx, const double *by, const double *bz)
+{
+    if (!h || !bx || !by || !bz) return MHD_ERR_ARG;
+    if (!h->mag) { set_err(h, "inflow face data need the magnetosphere mode"); return MHD_ERR_ARG; }
+    // host (my, mz+4) Fortran order = C [jz-1][jy-1]; device [k * NY + j], jz = k + 3
+    const int NY = h->NY, NZ = h->NZ;
+    std::vector<double> f(3 * (size_t)NZ * NY);
+    const double *src[3] = {bx, by, bz};
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < NZ; k++)
+            for (int j = 0; j < NY; j++) f[((size_t)c * NZ + k) * NY + j] = src[c][(size_t)(k + 2) * NY + j];
+    HCHK(hipMemcpy(const_cast<double *>(h->face), f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice));
     return MHD_OK;
 }
 
@@ -1366,17 +1576,56 @@ int mhd_step(mhd_handle *h, double time, double dt)
     if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
     if (!(dt > 0.0)) { set_err(h, "dt must be positive"); return MHD_ERR_ARG; }
     hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, h->stream, h->dtv, dt);
-    step_body(h);
+    if (h->mag) mag_step(h, time, dt);
+    else step_body(h);
     HCHK(hipStreamSynchronize(h->stream));
     HCHK(hipGetLastError());
     if (h->prof) return prof_collect(h);
     return MHD_OK;
 }
 
+// magnetosphere mode: the drive's time factors are evaluated on the host, so
+// the loop reads each step's dt; it stops before the step whose setdt
+// collapses (t3dsdsc.f:142-154)
+static int mag_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_last, double *elapsed_ms)
+{
+    hipEvent_t e0, e1;
+    HCHK(hipEventCreate(&e0));
+    HCHK(hipEventCreate(&e1));
+    HCHK(hipEventRecord(e0, h->stream));
+    double t = *time, dl = 0.0;
+    int rc = MHD_OK;
+    for (int s = 0; s < nsteps; s++) {
+        dt_kernels(h, 0);
+        double v[8];
+        HCHK(hipMemcpyAsync(v, h->dtv, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HCHK(hipStreamSynchronize(h->stream));
+        const double dt = v[0];
+        if (dt0 > 0.0 && dt < 0.1 * dt0) { rc = MHD_ERR_DT_COLLAPSE; dl = dt; break; }
+        mag_step(h, t, dt);
+        t += dt;
+        dl = dt;
+    }
+    HCHK(hipEventRecord(e1, h->stream));
+    HCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HCHK(hipGetLastError());
+    if (h->prof) { int r = prof_collect(h); if (r) return r; }
+    *time = t;
+    if (dt_last) *dt_last = dl;
+    if (elapsed_ms) *elapsed_ms = ms;
+    if (rc) set_err(h, "dt collapsed below 0.1*dt0");
+    return rc;
+}
+
 int mhd_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_last, double *elapsed_ms)
 {
     if (!h || nsteps < 0 || !time) return MHD_ERR_ARG;
     if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
+    if (h->mag) return mag_advance(h, nsteps, dt0, time, dt_last, elapsed_ms);
     double init[8] = {0, *time, dt0, 0, 0, 0, 0, 0};
     HCHK(hipMemcpyAsync(h->dtv, init, 8 * sizeof(double), hipMemcpyHostToDevice, h->stream));
     hipEvent_t e0, e1;
@@ -1407,7 +1656,7 @@ int mhd_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_
 int mhd_get_current(mhd_handle *h, double *cur)
 {
     if (!h || !cur) return MHD_ERR_ARG;
-    current(h, h->x);
+    any_current(h, h->x);
     return download_fields(h, cur, h->cur, 3);
 }
 
@@ -1416,6 +1665,7 @@ int mhd_adjoint_run(mhd_handle *h, int nsteps, const double *dts, int objective,
 {
     if (!h || nsteps < 1 || !dts || objective < 0 || objective > 1) return MHD_ERR_ARG;
     if (!h->have_eq) { set_err(h, "equilibrium not set"); return MHD_ERR_ARG; }
+    if (h->mag) { set_err(h, "the adjoint covers the Harris configuration, not the magnetosphere mode"); return MHD_ERR_ARG; }
     const mhd_config &c = h->cfg;
     for (int n = 0; n < nsteps; n++)
         if (!(dts[n] > 0.0)) { set_err(h, "adjoint: dt must be positive"); return MHD_ERR_ARG; }
@@ -1509,6 +1759,7 @@ int mhd_adjoint_rhs(mhd_handle *h, const double *xdif_bar, double *x_bar, double
                     int init, int smooth)
 {
     if (!h || !xdif_bar || !x_bar) return MHD_ERR_ARG;
+    if (h->mag) { set_err(h, "the adjoint covers the Harris configuration, not the magnetosphere mode"); return MHD_ERR_ARG; }
     AdjBufs A;
     int r = adj_alloc(h, A, 0, 0);
     if (!r) r = upload_fields(h, A.ab, xdif_bar, 8);
@@ -1538,7 +1789,7 @@ int mhd_diagnostics(mhd_handle *h, mhd_diag *d)
 {
     if (!h || !d) return MHD_ERR_ARG;
     // diagn: current then extrema on jy = my/2+1 (j = my/2)
-    current(h, h->x);
+    any_current(h, h->x);
     int nb = 64;
     hipLaunchKernelGGL(k_plane_extrema, dim3(nb, 3), dim3(256), 0, h->stream, h->g, h->cur, h->cfg.my / 2, h->red);
     std::vector<double> ex(3 * nb * 2);
@@ -1779,7 +2030,7 @@ int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out)
     if (!runs || n <= 0 || !out) return MHD_ERR_ARG;
     *out = nullptr;
     for (int r = 0; r < n; r++)
-        if (!runs[r]) return MHD_ERR_ARG;
+        if (!runs[r] || runs[r]->mag) return MHD_ERR_ARG;   // Harris runs only
     for (int r = 0; r < n; r++)
         for (int q = 0; q < r; q++)
             if (runs[q] == runs[r]) return MHD_ERR_ARG;   // a run appears once
@@ -1944,7 +2195,7 @@ void mhd_destroy(mhd_handle *h)
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
     double *ps[] = {h->geo, h->x, h->xfold, h->xm, h->xdA, h->xdB, h->eq, h->xresd, h->cA, h->cB,
-                    h->eA, h->eB, h->hA, h->hB, h->red, h->dtv, h->crad};
+                    h->eA, h->eB, h->hA, h->hB, h->red, h->dtv, h->crad, h->rr, h->chtab, h->vel, h->mgeo};
     for (auto p : ps)
         if (p) hipFree(p);
     for (auto &zc : h->zt)

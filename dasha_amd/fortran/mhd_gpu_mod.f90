@@ -10,6 +10,9 @@ module mhd_gpu
      integer(c_int) :: mx, my, mz
      integer(c_int) :: periodic_y, hall, whistler_dt
      real(c_double) :: di, eta0, fmu0, frho0, fpr0, gamma, cfl
+     ! magnetosphere mode (t3dsdsc.f:1398-1399, 1460-1548, 1692-1772, 4712)
+     integer(c_int) :: magnetosphere = 0, inflow = 0, radial_hall = 0, partial_e = 0
+     real(c_double) :: rcr = 0.d0
   end type mhd_config
 
   type, bind(C) :: mhd_diag
@@ -53,6 +56,18 @@ module mhd_gpu
        import :: c_int, c_ptr
        type(c_ptr), value :: h
      end function mhd_init_residual
+
+     integer(c_int) function mhd_set_inflow_face(h, bxinfc, byinfc, bzinfc) bind(C, name='mhd_set_inflow_face')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: bxinfc(*), byinfc(*), bzinfc(*)
+     end function mhd_set_inflow_face
+
+     integer(c_int) function mhd_compute_dt_noalloc(h, dt) bind(C, name='mhd_compute_dt_noalloc')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: dt
+     end function mhd_compute_dt_noalloc
 
      integer(c_int) function mhd_compute_dt(h, dt) bind(C, name='mhd_compute_dt')
        import :: c_int, c_double, c_ptr

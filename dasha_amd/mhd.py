@@ -28,7 +28,8 @@ EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state"
            "mhd_set_residual", "mhd_set_profiling", "mhd_kernel_times", "mhd_copy_rate",
            "mhd_last_error", "mhd_destroy", "mhd_ensemble_create", "mhd_ensemble_advance",
            "mhd_ensemble_status", "mhd_ensemble_error", "mhd_ensemble_destroy",
-           "mhd_adjoint_run", "mhd_adjoint_rhs", "mhd_ensemble_tune", "mhd_compute_dt_noalloc")
+           "mhd_adjoint_run", "mhd_adjoint_rhs", "mhd_ensemble_tune", "mhd_compute_dt_noalloc",
+           "mhd_set_inflow_face", "mhd_rhs_stage")
 
 MHD_ADJ_RESIDUAL = 1
 OBJECTIVES = {"magnetic_energy": 0, "reconnected_flux": 1}
@@ -38,7 +39,9 @@ class MHDConfig(C.Structure):
     _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("periodic_y", C.c_int),
                 ("hall", C.c_int), ("whistler_dt", C.c_int), ("di", C.c_double),
                 ("eta0", C.c_double), ("fmu0", C.c_double), ("frho0", C.c_double),
-                ("fpr0", C.c_double), ("gamma", C.c_double), ("cfl", C.c_double)]
+                ("fpr0", C.c_double), ("gamma", C.c_double), ("cfl", C.c_double),
+                ("magnetosphere", C.c_int), ("inflow", C.c_int), ("radial_hall", C.c_int),
+                ("partial_e", C.c_int), ("rcr", C.c_double)]
 
 
 class MHDDiag(C.Structure):
@@ -80,6 +83,8 @@ def load_library(path: str = LIBPATH):
     L.mhd_advance.argtypes = [P, I, C.c_double, D, D, D]
     L.mhd_diagnostics.argtypes = [P, C.POINTER(MHDDiag)]
     L.mhd_rhs.argtypes = [P, C.c_double]
+    L.mhd_rhs_stage.argtypes = [P, C.c_double, C.c_double, I]
+    L.mhd_set_inflow_face.argtypes = [P, D, D, D]
     L.mhd_set_profiling.argtypes = [P, I]
     L.mhd_kernel_times.argtypes = [P, I, C.POINTER(C.c_char_p), D, C.POINTER(I)]
     L.mhd_copy_rate.argtypes = [C.c_long, I, D]
@@ -126,7 +131,12 @@ class MHDStep:
         cfg = MHDConfig(mx=case.mx, my=case.my, mz=case.mz, periodic_y=int(case.periodic_y),
                         hall=int(case.hall), whistler_dt=int(case.whistler_dt), di=case.di,
                         eta0=case.eta0, fmu0=case.fmu0, frho0=case.frho0, fpr0=case.fpr0,
-                        gamma=case.gamma, cfl=case.cfl)
+                        gamma=case.gamma, cfl=case.cfl,
+                        magnetosphere=int(getattr(case, "magnetosphere", False)),
+                        inflow=int(getattr(case, "inflow", False)),
+                        radial_hall=int(getattr(case, "radial_hall", False)),
+                        partial_e=int(getattr(case, "partial_e", False)),
+                        rcr=float(getattr(case, "rcr", 0.0)))
         geo = [np.ascontiguousarray(a, dtype=np.float64)
                for a in (case.xx, case.yy, case.zz, case.dx, case.dy, case.dz)]
         h = C.c_void_p()
@@ -137,6 +147,9 @@ class MHDStep:
         eq = [np.ascontiguousarray(a, dtype=np.float64)
               for a in (case.rhoi, case.bxi, case.byi, case.bzi, case.pri)]
         self._check(self.L.mhd_set_equilibrium(self.h, *[_p(a) for a in eq]))
+        face = getattr(case, "inflow_face", None)
+        if face is not None:
+            self.set_inflow_face(*face)
         self.set_state(case.x0)
 
     def _check(self, rc):
@@ -209,6 +222,17 @@ class MHDStep:
 
     def rhs(self, t: float):
         self._check(self.L.mhd_rhs(self.h, t))
+
+    def rhs_stage(self, t: float, ddt: float, km: int):
+        """right(t, ddt, km): the stage arguments the driven inflow reads"""
+        self._check(self.L.mhd_rhs_stage(self.h, t, ddt, int(km)))
+
+    def set_inflow_face(self, bx, by, bz):
+        """inflow face data bxinfc, byinfc, bzinfc: each (mz+4, my) C order"""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (bx, by, bz)]
+        for a in arrs:
+            assert a.shape == (self.case.mz + 4, self.case.my)
+        self._check(self.L.mhd_set_inflow_face(self.h, *[_p(a) for a in arrs]))
 
     def advance(self, nsteps: int, time: float = 0.0, dt0: float = 0.0):
         """nsteps of setdt+stepon with dt on the device; returns

@@ -44,6 +44,19 @@ typedef struct mhd_config {
     double fmu0, frho0, fpr0; /* mhd03.for:62-64 (formu, t3dsdsc.f:6524-6528) */
     double gamma;          /* mhd03.for:60                                  */
     double cfl;            /* mhd03.for:71                                  */
+    /* magnetosphere mode (the reference's own default case); any of the four
+     * switches selects it; it needs non-periodic y and no viscosity or
+     * diffusion.  Zero: the Harris configuration. */
+    int magnetosphere;     /* rr0 = rr1 = |(x,y,z)|: inner-sphere mask rr1 <= rcr at
+                            * the start of every bndry* (t3dsdsc.f:1692-1707, 2003-2017,
+                            * 2549-2559, 2709-2719, 2881-2891), setdt over rr1 >= 1 (4712) */
+    int inflow;            /* driven x-left inflow (t3dsdsc.f:1711-1772) instead of the
+                            * mirrored x-right rule; face data via mhd_set_inflow_face */
+    int radial_hall;       /* cht = 0.5 di (tanh((rr0-1.1)/8) - tanh((rr0-2)/10)), di = chall
+                            * (t3dsdsc.f:1398-1399); 0: cht = di */
+    int partial_e;         /* totalE = .false.: E from B - B_i plus the curl(v x B0) dipole
+                            * terms (t3dsdsc.f:1370-1386, 1460-1548); 0: totalE */
+    double rcr;            /* mhd03.for rcr: mask radius */
 } mhd_config;
 
 typedef struct mhd_diag {
@@ -91,6 +104,10 @@ int mhd_compute_dt(mhd_handle *h, double *dt);
 int mhd_compute_dt_noalloc(mhd_handle *h, double *dt);
 /* stepon (t3dsdsc.f:1063-1165): one RK4 step of size dt from time */
 int mhd_step(mhd_handle *h, double time, double dt);
+/* the inflow face data of magnetosphere mode, bxinfc/byinfc/bzinfc(my, mz+4)
+ * (common /var17/, read from fcinp/bin* and scaled by -50 at t3dsdsc.f:728-768);
+ * zero until set */
+int mhd_set_inflow_face(mhd_handle *h, const double *bxinfc, const double *byinfc, const double *bzinfc);
 /* the time loop body of program main (t3dsdsc.f:139-176) without I/O:
  * nsteps x (setdt; stepon; time += dt) with dt kept on the device.
  * dt0 > 0 arms the collapse guard dt < 0.1*dt0 (t3dsdsc.f:142): a device
@@ -116,6 +133,10 @@ int mhd_get_divb(mhd_handle *h, double *divb);
  * last sweep updates the state in place and leaves none: MHD_ERR_ARG);
  * xresd is the frozen momentum residual. */
 int mhd_rhs(mhd_handle *h, double t);
+/* right(t, ddt, km) with the stage arguments the driven inflow reads
+ * (ddt the stage's time step, km = 1: the quotient subtracts xfold, 2: xm);
+ * mhd_rhs(h, t) is mhd_rhs_stage(h, t, 1e-9, 1) */
+int mhd_rhs_stage(mhd_handle *h, double t, double ddt, int km);
 int mhd_get_xdif(mhd_handle *h, double *xdif);
 int mhd_get_residual(mhd_handle *h, double *xresd);
 int mhd_set_residual(mhd_handle *h, const double *xresd);

@@ -929,7 +929,7 @@ void orc_init_residual(orc *o)
 }
 
 /* one call of right(t) for the tests (t = 0 skips the final avrg as in 1669) */
-void orc_rhs(orc *o, double t) { o->st_t = t; right(o, t, 0); }
+void orc_rhs(orc *o, double t) { o->st_t = t; o->st_ddt = 1.e-9; o->st_km = 1; right(o, t, 0); }
 /* right(t, ddt, km) with the inflow's stage arguments */
 void orc_rhs_stage(orc *o, double t, double ddt, int km)
 {

@@ -62,7 +62,8 @@ int main(void) {
   F(mhd_config, mx); F(mhd_config, my); F(mhd_config, mz); F(mhd_config, periodic_y);
   F(mhd_config, hall); F(mhd_config, whistler_dt); F(mhd_config, di); F(mhd_config, eta0);
   F(mhd_config, fmu0); F(mhd_config, frho0); F(mhd_config, fpr0); F(mhd_config, gamma);
-  F(mhd_config, cfl);
+  F(mhd_config, cfl); F(mhd_config, magnetosphere); F(mhd_config, inflow);
+  F(mhd_config, radial_hall); F(mhd_config, partial_e); F(mhd_config, rcr);
   F(mhd_diag, cxmax); F(mhd_diag, cxmin); F(mhd_diag, cymax); F(mhd_diag, cymin);
   F(mhd_diag, czmax); F(mhd_diag, czmin); F(mhd_diag, e_kin); F(mhd_diag, e_mag);
   F(mhd_diag, e_th); F(mhd_diag, e_mag_pert); F(mhd_diag, divb_max); F(mhd_diag, divb_rms);
@@ -101,6 +102,8 @@ program layout
   print '(a,i0)', 'diag ', c_sizeof(d)
   print '(a,i0)', 'di ', transfer(c_loc(c%di), 0_c_intptr_t) - transfer(c_loc(c), 0_c_intptr_t)
   print '(a,i0)', 'cfl ', transfer(c_loc(c%cfl), 0_c_intptr_t) - transfer(c_loc(c), 0_c_intptr_t)
+  print '(a,i0)', 'partial_e ', transfer(c_loc(c%partial_e), 0_c_intptr_t) - transfer(c_loc(c), 0_c_intptr_t)
+  print '(a,i0)', 'rcr ', transfer(c_loc(c%rcr), 0_c_intptr_t) - transfer(c_loc(c), 0_c_intptr_t)
   print '(a,i0)', 'divb_rms ', transfer(c_loc(d%divb_rms), 0_c_intptr_t) - &
        transfer(c_loc(d), 0_c_intptr_t)
 end program layout
@@ -126,6 +129,8 @@ def test_struct_layout_fortran_module(tmp_path):
     assert int(out["diag"]) == C.sizeof(mhd.MHDDiag)
     assert int(out["di"]) == mhd.MHDConfig.di.offset
     assert int(out["cfl"]) == mhd.MHDConfig.cfl.offset
+    assert int(out["partial_e"]) == mhd.MHDConfig.partial_e.offset
+    assert int(out["rcr"]) == mhd.MHDConfig.rcr.offset
     assert int(out["divb_rms"]) == mhd.MHDDiag.divb_rms.offset
 
 
