@@ -16,6 +16,7 @@ Prints ONE JSON line:
   cpu_baseline   the CPU oracle (C restatement, OpenMP) on a bounded sample
                  of the same workload (one step), with the host's CPU model
   C2             the same timed loop on 256x128x256 (200 steps)
+  magnetosphere  the reference's own case (201x51x105 dipole, mask, inflow)
   ensemble       48 uneven Harris runs advanced together vs one after another
   adjoint        discrete adjoint of 500 steps at 128x64x128 (sqrt(N)
                  checkpointing): reverse sweep incl. recomputation / forward
@@ -138,6 +139,29 @@ def time_config(name: str, steps: int, warmup: int):
             "step_roofline_frac": value * b_alg / HBM_PEAK}
 
 
+def time_magnetosphere(steps: int = 50, warmup: int = 3):
+    """the reference's own case on its default grid (201 x 51 x 105, dipole,
+    inner-sphere mask, driven x-left inflow, mhd03.for's constants, Hall
+    off): the reference-structured kernels of magnetosphere mode"""
+    from dasha_amd import magneto
+    from dasha_amd.mhd import MHDStep
+    case = magneto.case_default(face=magneto.synthetic_face((51, 105)))
+    g = MHDStep(case)
+    g.init_residual()
+    dt0 = g.compute_dt()
+    tm, _, _ = g.advance(warmup, 0.0, dt0)
+    tm, _, ms = g.advance(steps, tm, dt0)
+    g.close()
+    value = case.cells * steps / (ms / 1e3)
+    b_alg = B_ALG_HALL if case.hall else B_ALG_NOHALL
+    print(f"[bench] {case.name}: {ms / steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": f"{case.name} dipole magnetosphere, driven inflow, inner-sphere mask "
+                        f"(RK4 MHD step, stepon+setdt)",
+            "grid": [case.mx, case.ny, case.mz], "steps": steps, "warmup": warmup, "value": value,
+            "unit": "cell-updates/s", "ms_per_step": ms / steps,
+            "step_roofline_frac": value * b_alg / HBM_PEAK}
+
+
 def time_ensemble(steps: int, nruns: int = 48):
     """the batched ensemble (48 Harris runs of uneven size and parameters):
     aggregate cell-updates/s of the runs advanced together against the
@@ -227,6 +251,7 @@ def main():
     ap.add_argument("--no-c2", action="store_true",
                     help="skip the 256x128x256 timing added to the line")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble timing")
+    ap.add_argument("--no-magnetosphere", action="store_true", help="skip the magnetosphere-mode timing")
     ap.add_argument("--ensemble-only", action="store_true", help="time the batched ensemble alone")
     ap.add_argument("--ensemble-runs", type=int, default=48)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete adjoint timing")
@@ -325,6 +350,8 @@ def main():
     g.close()
     if args.config == "C3" and not args.no_c2:
         out["C2"] = time_config("C2", 200, 3)
+    if not args.no_magnetosphere:
+        out["magnetosphere"] = time_magnetosphere()
     if not args.no_ensemble:
         out["ensemble"] = time_ensemble(10, args.ensemble_runs)
     if not args.no_adjoint:

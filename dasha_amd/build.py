@@ -48,13 +48,14 @@ def build_fortran(verbose: bool = False) -> str | None:
     fc = FLANG if os.path.exists(FLANG) else shutil.which("flang")
     fdir = os.path.join(HERE, "fortran")
     mod = os.path.join(fdir, "mhd_gpu_mod.f90")
+    io = os.path.join(fdir, "mhd_io.f90")
     drv = os.path.join(fdir, "mhd_driver.f90")
-    if not (os.path.exists(mod) and os.path.exists(drv)):
+    if not (os.path.exists(mod) and os.path.exists(io) and os.path.exists(drv)):
         return None
     bdir = os.path.join(fdir, "build")
     os.makedirs(bdir, exist_ok=True)
     exe = os.path.join(fdir, "mhd_driver")
-    cmd = [fc, "-O2", "-J", bdir, mod, drv, "-o", exe, f"-L{HERE}", "-lmhd_gpu",
+    cmd = [fc, "-O2", "-J", bdir, mod, io, drv, "-o", exe, f"-L{HERE}", "-lmhd_gpu",
            f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -10,6 +10,12 @@
 !   grid.bin     xx, yy, zz, dx, dy, dz
 !   equil.bin    rhoi, bxi, byi, bzi, pri
 !   mhd<nst><000> restart record with the initial state (readin format)
+! Magnetosphere mode (magneto = .true.; the reference's own case): with
+! initia = .true. the program builds the initial state itself as initia does
+! (t3dsdsc.f:771-814): the point dipole, or with mxdat the field of
+! inp/binx000, biny000, binz000 (mhd_io), and writes it as mhdinit in the
+! restart format; equil.bin and the restart file are not read then.  With
+! mxdatfc the inflow face data come from fcinp/binx000 .. binz000.
 ! Outputs: fort.20 (dt, time, nstep per step, t3dsdsc.f:180-181),
 ! data/stepnm (nstep, time, dt, t3dsdsc.f:157-169), current.dat (diagn,
 ! t3dsdsc.f:1228-1230), restart files mhd<nst><000> (recrd), the plot files
@@ -26,12 +32,15 @@
 program mhd_driver
   use iso_c_binding
   use mhd_gpu
+  use mhd_io
   implicit none
   integer :: mx, my, mz, nstop, nst, ncase, dnstep, nend, nint, nadj, adjobj
   logical :: periody, hall, whistler
-  real(c_double) :: di, eta0, fmu0, frho0, fpr0, gamma, cfl, dtime
+  logical :: magneto, initia, mxdat, mxdatfc, inflow, radial_hall, partial_e
+  real(c_double) :: di, eta0, fmu0, frho0, fpr0, gamma, cfl, dtime, rcr
   namelist /run/ mx, my, mz, periody, hall, whistler, di, eta0, fmu0, frho0, fpr0, gamma, cfl, &
-       nstop, nst, ncase, dtime, dnstep, nend, nint, nadj, adjobj
+       nstop, nst, ncase, dtime, dnstep, nend, nint, nadj, adjobj, &
+       magneto, initia, mxdat, mxdatfc, inflow, radial_hall, partial_e, rcr
   type(mhd_config) :: cfg
   type(mhd_diag) :: dg
   type(c_ptr) :: h
@@ -48,6 +57,8 @@ program mhd_driver
   di = 0; eta0 = 0; fmu0 = 0; frho0 = 0; fpr0 = 0; gamma = 5.d0/3.d0; cfl = 0.4d0
   nstop = 10; nst = 1; ncase = 1; dtime = 1.d0; dnstep = 10; nend = 5011; nint = 1
   nadj = 0; adjobj = 0
+  magneto = .false.; initia = .false.; mxdat = .false.; mxdatfc = .false.
+  inflow = .false.; radial_hall = .false.; partial_e = .false.; rcr = 1.d0
   nstp = 0
   open(newunit=u, file='mhd.nml', status='old', action='read', iostat=ios)
   if (ios /= 0) call fail('cannot open mhd.nml', 1)
@@ -65,27 +76,37 @@ program mhd_driver
   read(u) dy
   read(u) dz
   close(u)
-  open(newunit=u, file='equil.bin', form='unformatted', status='old', action='read')
-  read(u) rhoi
-  read(u) bxi
-  read(u) byi
-  read(u) bzi
-  read(u) pri
-  close(u)
-  ! readin (t3dsdsc.f:4822-4828)
-  open(newunit=u, file='mhd'//cn(nst)//cn(0), form='unformatted', status='old', action='read')
-  read(u) ncase_in, nstep, time, nst_in
-  read(u) x
-  close(u)
+  if (initia) then
+     call build_initia()
+  else
+     open(newunit=u, file='equil.bin', form='unformatted', status='old', action='read')
+     read(u) rhoi
+     read(u) bxi
+     read(u) byi
+     read(u) bzi
+     read(u) pri
+     close(u)
+     ! readin (t3dsdsc.f:4822-4828)
+     open(newunit=u, file='mhd'//cn(nst)//cn(0), form='unformatted', status='old', action='read')
+     read(u) ncase_in, nstep, time, nst_in
+     read(u) x
+     close(u)
+  end if
 
   cfg%mx = mx; cfg%my = my; cfg%mz = mz
   cfg%periodic_y = merge(1, 0, periody); cfg%hall = merge(1, 0, hall)
   cfg%whistler_dt = merge(1, 0, whistler)
   cfg%di = di; cfg%eta0 = eta0; cfg%fmu0 = fmu0; cfg%frho0 = frho0; cfg%fpr0 = fpr0
   cfg%gamma = gamma; cfg%cfl = cfl
+  if (magneto) then
+     cfg%magnetosphere = 1; cfg%rcr = rcr
+     cfg%inflow = merge(1, 0, inflow); cfg%radial_hall = merge(1, 0, radial_hall)
+     cfg%partial_e = merge(1, 0, partial_e)
+  end if
   rc = mhd_create(cfg, xx, yy, zz, dx, dy, dz, h)
   if (rc /= MHD_OK) call fail('mhd_create failed', 1)
   call check(mhd_set_equilibrium(h, rhoi, bxi, byi, bzi, pri))
+  if (magneto .and. mxdatfc) call set_face()
   call check(mhd_set_state(h, x))
   ! initia's frozen residual (t3dsdsc.f:816-834)
   call check(mhd_init_residual(h))
@@ -196,15 +217,7 @@ contains
           bl(jx,jz,:) = f
        end do
     end do
-    open(newunit=v, file='data/m2d'//cn(nst)//'.dat', status='replace', form='formatted')
-    write(v,*) 2, mx, mz
-    write(v,*) time
-    write(v,'(5(1x,e14.5E3))') xx(3:mx+2), zz(3:mz+2)
-    do i = 1, 11
-       write(v,*) i
-       write(v,'(5(1x,e14.5E3))') ((bl(jx,jz,i), jx=1,mx), jz=1,mz)
-    end do
-    close(v)
+    call write_m2d('data/m2d'//cn(nst)//'.dat', time, mx, mz, xx(3:mx+2), zz(3:mz+2), bl)
   end subroutine recrd2
 
   subroutine recrd3()
@@ -223,15 +236,7 @@ contains
           end do
        end do
     end do
-    open(newunit=v, file='data/m3d'//cn(nst), status='replace', form='formatted')
-    write(v,*) 3, mx, my, mz
-    write(v,*) time
-    write(v,'(11(1x,e10.4))') xx(3:mx+2), yy, zz(3:mz+2)
-    do i = 1, 11
-       write(v,*) i
-       write(v,'(11(1x,e10.4))') (((bl(jx,jy,jz,i), jx=1,mx), jy=1,my), jz=1,mz)
-    end do
-    close(v)
+    call write_m3d('data/m3d'//cn(nst), time, mx, my, mz, xx(3:mx+2), yy, zz(3:mz+2), bl)
   end subroutine recrd3
 
   subroutine adjoint_run()
@@ -256,6 +261,42 @@ contains
     write(v) gx
     close(v)
   end subroutine adjoint_run
+
+  subroutine build_initia()
+    ! initia (t3dsdsc.f:612-814) of the magnetosphere; the state is also
+    ! written as mhdinit (restart format) for comparisons
+    real(c_double), allocatable :: bin(:,:,:,:)
+    integer :: v, ios, c
+    character(len=1), parameter :: comp(3) = ['x', 'y', 'z']
+    if (.not. magneto) call fail('initia = .true. builds the magnetosphere case only', 1)
+    allocate(bin(mx+4,my,mz+4,3))
+    bin = 0.d0
+    if (mxdat) then
+       do c = 1, 3
+          call read_inp_field('inp/bin'//comp(c)//cn(0), mx, my, mz, bin(:,:,:,c), ios)
+          if (ios /= 0) call fail('cannot read inp/bin'//comp(c)//cn(0), 1)
+       end do
+    end if
+    call magnetosphere_initia(mx, my, mz, xx, yy, zz, mxdat, bin, x, rhoi, bxi, byi, bzi, pri)
+    ncase_in = ncase; nstep = 0; time = 0.d0; nst_in = nst
+    open(newunit=v, file='mhdinit', form='unformatted', status='replace', action='write')
+    write(v) ncase, nstep, time, nst
+    write(v) x
+    close(v)
+  end subroutine build_initia
+
+  subroutine set_face()
+    ! fcinp/bin* (t3dsdsc.f:728-768) -> bxinfc, byinfc, bzinfc
+    real(c_double), allocatable :: f(:,:,:)
+    integer :: ios, c
+    character(len=1), parameter :: comp(3) = ['x', 'y', 'z']
+    allocate(f(my, mz+4, 3))
+    do c = 1, 3
+       call read_fcinp_field('fcinp/bin'//comp(c)//cn(0), my, mz, f(:,:,c), ios)
+       if (ios /= 0) call fail('cannot read fcinp/bin'//comp(c)//cn(0), 1)
+    end do
+    call check(mhd_set_inflow_face(h, f(:,:,1), f(:,:,2), f(:,:,3)))
+  end subroutine set_face
 
   subroutine check(r)
     integer(c_int), intent(in) :: r

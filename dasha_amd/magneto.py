@@ -85,14 +85,15 @@ def dipole_state(xx, yy, zz, bfield=None):
 def magnetosphere_case(name: str, mx: int, my: int, mz: int, *, xmin: float = F32(-2.1),
                        xmax: float = F32(5.65), ymin: float = -2.5, ymax: float = 2.5,
                        zmin: float = -2.5, zmax: float = 2.5, eta0: float = 2.0e-4,
-                       di: float = F32(0.07), hall: bool = True, radial_hall: bool = True,
+                       di: float = F32(0.07), hall: bool = False, radial_hall: bool = True,
                        partial_e: bool = False, inflow: bool = True, rcr: float = 1.0,
                        cfl: float = 2.0, whistler_dt: bool = False, steps: int = 100,
                        bfield=None, face=None) -> MagnetosphereCase:
     """The dipole magnetosphere on a uniform mx x my x mz mesh (defaults:
-    mhd03.for -- box [-2.1, 5.65] x [-2.5, 2.5]^2, eta 2e-4, chall 0.07,
-    rcr 1, cfl 2).  The density, pressure and field arrays also serve as
-    the equilibrium.  ``face`` = the three inflow face arrays (mz+4, my)."""
+    mhd03.for -- box [-2.1, 5.65] x [-2.5, 2.5]^2, eta 2e-4, Hall off with
+    chall 0.07 and the radial profile when on, totalE, rcr 1, cfl 2).  The
+    density, pressure and field arrays also serve as the equilibrium.
+    ``face`` = the three inflow face arrays (mz+4, my)."""
     px, sx = uniform_axis(mx, xmin, xmax)
     py, sy = uniform_axis(my, ymin, ymax)
     pz, sz = uniform_axis(mz, zmin, zmax)

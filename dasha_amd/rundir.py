@@ -9,7 +9,12 @@
 * ``equil.bin``: one record each of rhoi, bxi, byi, bzi, pri (initia,
   t3dsdsc.f:806-814);
 * ``mhd.nml``: namelist /run/ with the parameters the reference takes from
-  mhd01.for / mhd03.for and the main-loop constants (t3dsdsc.f:88-92).
+  mhd01.for / mhd03.for and the main-loop constants (t3dsdsc.f:88-92);
+* ``inp/binx000`` .. ``binz000`` (initia's mxdat branch, t3dsdsc.f:674-726):
+  list-directed values, z fastest, then x, then y, over the physical points
+  (rank 000 of a single domain); ``fcinp/binx000`` .. ``binz000`` (mxdatfc,
+  t3dsdsc.f:728-768): z fastest, then y, the inflow face data before the
+  reference's -50 scaling.
 
 Record markers are 4-byte little-endian lengths (the flang and gfortran
 default).
@@ -57,10 +62,49 @@ def read_restart(path: str, shape):
     return dict(ncase=ncase, nstep=nstep, time=time, nst=nst), x
 
 
+def write_inp_field(path: str, a: np.ndarray, mx: int, my: int, mz: int):
+    """one inp/bin* file from a padded (mz+4, my, mx+4) array: for each y row,
+    for each x column, the z column's physical values"""
+    with open(path, "w") as f:
+        for jy in range(my):
+            for jx in range(mx):
+                col = a[2:mz + 2, jy, 2 + jx]
+                f.write(" ".join("%.17g" % v for v in col) + "\n")
+
+
+def read_inp_field(path: str, mx: int, my: int, mz: int) -> np.ndarray:
+    """the padded (mz+4, my, mx+4) array of one inp/bin* file (zero ghosts)"""
+    vals = np.array(open(path).read().split(), dtype=np.float64)[:mx * my * mz]
+    out = np.zeros((mz + 4, my, mx + 4))
+    out[2:mz + 2, :, 2:mx + 2] = vals.reshape(my, mx, mz).transpose(2, 0, 1)
+    return out
+
+
+def write_fcinp_field(path: str, f: np.ndarray, my: int, mz: int):
+    """one fcinp/bin* file from unscaled face data (mz+4, my): for each y
+    row, the physical z values"""
+    with open(path, "w") as fh:
+        for jy in range(my):
+            fh.write(" ".join("%.17g" % v for v in f[2:mz + 2, jy]) + "\n")
+
+
+def read_fcinp_field(path: str, my: int, mz: int) -> np.ndarray:
+    """bxinfc(my, mz+4) as a C (mz+4, my) array: the file's values times -50
+    (t3dsdsc.f:754-765), zero on the ghost planes"""
+    vals = np.array(open(path).read().split(), dtype=np.float64)[:my * mz]
+    out = np.zeros((mz + 4, my))
+    out[2:mz + 2, :] = (vals.reshape(my, mz) * -50.0).T
+    return out
+
+
 def write_case(rundir: str, case, *, nstop: int, nst: int = 1, ncase: int = 1, dtime: float = 1.0,
-               dnstep: int = 10, nadj: int = 0, adjobj: int = 0):
+               dnstep: int = 10, nadj: int = 0, adjobj: int = 0, initia: bool = False,
+               mxdat: bool = False, mxdatfc: bool = False):
     """Write grid.bin, equil.bin, mhd.nml and the initial restart file
-    (nadj > 0: the driver differentiates nadj steps instead of running)."""
+    (nadj > 0: the driver differentiates nadj steps instead of running).
+    For a magnetosphere case the namelist carries its switches; initia: the
+    driver builds the initial state itself (mxdat: from inp/bin*), mxdatfc:
+    it reads the inflow face data from fcinp/bin*."""
     os.makedirs(rundir, exist_ok=True)
     with open(os.path.join(rundir, "grid.bin"), "wb") as f:
         for a in (case.xx, case.yy, case.zz, case.dx, case.dy, case.dz):
@@ -77,7 +121,12 @@ def write_case(rundir: str, case, *, nstop: int, nst: int = 1, ncase: int = 1, d
                 f"fpr0={case.fpr0!r},\n")
         f.write(f" gamma={case.gamma!r}, cfl={case.cfl!r},\n")
         f.write(f" nstop={nstop}, nst={nst}, ncase={ncase}, dtime={dtime!r}, dnstep={dnstep},\n")
-        f.write(f" nadj={nadj}, adjobj={adjobj}\n/\n")
+        f.write(f" nadj={nadj}, adjobj={adjobj}")
+        if getattr(case, "magnetosphere", False):
+            f.write(f",\n magneto=.true., initia={b(initia)}, mxdat={b(mxdat)}, mxdatfc={b(mxdatfc)},\n")
+            f.write(f" inflow={b(case.inflow)}, radial_hall={b(case.radial_hall)}, "
+                    f"partial_e={b(case.partial_e)}, rcr={case.rcr!r}")
+        f.write("\n/\n")
     write_restart(os.path.join(rundir, "mhd" + cn(nst) + cn(0)), case.x0, ncase=ncase, nstep=0,
                   time=0.0, nst=nst)
 

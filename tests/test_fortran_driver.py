@@ -159,3 +159,40 @@ def test_driver_adjoint_matches_python(tmp_path):
         g.close()
     assert val == v2 and geta == ge2
     assert np.array_equal(gx, gx2)
+
+
+@pytest.mark.gpu
+def test_driver_magnetosphere_initia(tmp_path):
+    """magnetosphere mode: the driver builds initia's state from inp/bin*
+    (mxdat) and reads the inflow face data from fcinp/bin* (mxdatfc); 4 steps
+    equal the Python binding run from the driver's own initial state (mhdinit)
+    with the face data read back in Python, bit for bit"""
+    import dataclasses
+    from dasha_amd import magneto
+    from dasha_amd.mhd import MHDStep
+    drv = _driver()
+    face = magneto.synthetic_face((13, 25))
+    case = magneto.case_small(face=face, di=0.2, hall=True)
+    s = float(np.float32(-25.0) / np.float32(3.0))
+    os.makedirs(tmp_path / "inp")
+    os.makedirs(tmp_path / "fcinp")
+    for m, c in enumerate("xyz"):
+        rundir.write_inp_field(str(tmp_path / "inp" / f"bin{c}000"), case.x0[4 + m] / s, case.mx, case.my,
+                               case.mz)
+        rundir.write_fcinp_field(str(tmp_path / "fcinp" / f"bin{c}000"), face[m] / -50.0, case.my, case.mz)
+    nstop = 3
+    rundir.write_case(str(tmp_path), case, nstop=nstop, dtime=1e9, initia=True, mxdat=True, mxdatfc=True)
+    print(f"[driver] magnetosphere initia, case={case.name}", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "110", drv], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    _, x0 = rundir.read_restart(str(tmp_path / "mhdinit"), case.x0.shape)
+    np.testing.assert_allclose(x0[4:7], case.x0[4:7], rtol=1e-13, atol=1e-300)
+    hdr, xf = rundir.read_restart(str(tmp_path / "mhdfinal"), case.x0.shape)
+    assert hdr["nstep"] == nstop + 1
+    fc = tuple(rundir.read_fcinp_field(str(tmp_path / "fcinp" / f"bin{c}000"), case.my, case.mz) for c in "xyz")
+    pc = dataclasses.replace(case, x0=x0, rhoi=x0[0].copy(), bxi=x0[4].copy(), byi=x0[5].copy(),
+                             bzi=x0[6].copy(), pri=x0[7].copy(), inflow_face=fc)
+    g = MHDStep(pc)
+    t, _ = g.run(nstop + 1)
+    assert np.array_equal(xf, g.get_state())
+    assert hdr["time"] == t

@@ -3,7 +3,8 @@ against the oracle, through the C ABI: the frozen residual, one right() with
 the driven inflow at both stage kinds, the current, setdt over r >= 1, three
 steps from t = 0 and from the middle of the drive pulse (t = 20), the
 diagnostics, and the device-dt loop against the host loop.  Three switch
-sets: the reference's defaults (mask, inflow, radial Hall, totalE), the
+sets: the reference's defaults with the Hall term on (mask, inflow, radial
+Hall, totalE), the
 totalE = .false. dipole terms without Hall, and a run without the inflow.
 The third one also takes setdt's whistler term.  Tolerances as in
 test_gpu_parity: 1e-12 relative L2 per field for one call, 1e-11 after
@@ -19,12 +20,12 @@ pytestmark = pytest.mark.gpu
 TIGHT = 1e-12
 
 CASES = {
-    "defaults": lambda: magneto.case_small(face=magneto.synthetic_face((13, 25)), di=0.2),
+    "defaults": lambda: magneto.case_small(face=magneto.synthetic_face((13, 25)), di=0.2, hall=True),
     "partial_e": lambda: magneto.case_small(face=magneto.synthetic_face((13, 25)), hall=False,
                                             partial_e=True),
     # a uniform Hall coefficient needs the whistler speed in setdt to stay stable
     "no_inflow": lambda: magneto.case_small(29, 15, 21, inflow=False, radial_hall=False, di=0.15,
-                                            whistler_dt=True),
+                                            hall=True, whistler_dt=True),
 }
 
 
