@@ -142,7 +142,11 @@ __global__ void k_dt_partial(Geo g, const double *__restrict__ X, double eta, do
 
 // final MIN; keeps the running time and the collapse flag on the device:
 // dtv[0] = dt, dtv[1] = time (advanced by the previous dt), dtv[2] = dt0,
-// dtv[3] = collapse flag, dtv[4] = previous dt
+// dtv[3] = collapse flag, dtv[4] = previous dt, dtv[5] = the collapsing dt.
+// A collapse (dt < 0.1 dt0, t3dsdsc.f:142-154) freezes dt and time: the
+// step after it runs with the last good dt and k_restore undoes it, so the
+// state and time stay those before the collapsing step, as the reference
+// stops there.
 __global__ void k_dt_final(const double *__restrict__ part, int n, double *dtv, int advance_time)
 {
     __shared__ double sm[256];
@@ -156,12 +160,32 @@ __global__ void k_dt_final(const double *__restrict__ part, int n, double *dtv, 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        if (dtv[3] != 0.0) return;
         double dt = sm[0];
         if (advance_time) dtv[1] = dtv[1] + dtv[4];
+        if (dtv[2] > 0.0 && dt < 0.1 * dtv[2]) {
+            dtv[3] = 1.0;
+            dtv[5] = dt;
+            return;
+        }
         dtv[0] = dt;
         dtv[4] = dt;
-        if (dtv[2] > 0.0 && dt < 0.1 * dtv[2]) dtv[3] = 1.0;
     }
+}
+
+// after a step of a guarded loop: once the collapse flag is up, the step
+// that followed the collapsing setdt is undone (x = xfold, the state before it)
+__global__ void k_restore(const double *__restrict__ dtv, const double *__restrict__ XF, double *__restrict__ X, long n)
+{
+    if (dtv[3] == 0.0) return;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) X[q] = XF[q];
+}
+
+// disarm the collapse guard (a stand-alone setdt reports every dt)
+__global__ void k_disarm(double *dtv)
+{
+    dtv[2] = 0.0;
+    dtv[3] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -864,6 +888,12 @@ static void dt_kernels(mhd_handle *h, int advance_time, double *part = nullptr)
     PROF_END(PT_DT);
 }
 
+// the guarded loop's undo of a step after a collapse (see k_dt_final)
+static void restore_after_collapse(mhd_handle *h)
+{
+    hipLaunchKernelGGL(k_restore, dim3(1024), dim3(256), 0, h->stream, h->dtv, h->xfold, h->x, 8 * h->S);
+}
+
 // ---- magnetosphere mode (kernels in mhd_magneto.h) ----
 static Mag mag_args(const mhd_handle *h)
 {
@@ -1541,6 +1571,7 @@ int mhd_set_residual(mhd_handle *h, const double *r)
 int mhd_compute_dt(mhd_handle *h, double *dt)
 {
     if (!h || !dt) return MHD_ERR_ARG;
+    hipLaunchKernelGGL(k_disarm, dim3(1), dim3(1), 0, h->stream, h->dtv);
     dt_kernels(h, 0);
     double v[8];
     HCHK(hipMemcpyAsync(v, h->dtv, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1560,6 +1591,7 @@ int mhd_compute_dt_noalloc(mhd_handle *h, double *dt)
                        std::to_string(MHD_SCRATCH_DOUBLES));
         return MHD_ERR_ARG;
     }
+    hipLaunchKernelGGL(k_disarm, dim3(1), dim3(1), 0, h->stream, h->dtv);
     dt_kernels(h, 0, part);
     double v[8];
     HCHK(hipMemcpyAsync(v, h->dtv, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1590,6 +1622,7 @@ int mhd_step(mhd_handle *h, double time, double dt)
 static int mag_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_last, double *elapsed_ms)
 {
     hipEvent_t e0, e1;
+    hipLaunchKernelGGL(k_disarm, dim3(1), dim3(1), 0, h->stream, h->dtv);
     HCHK(hipEventCreate(&e0));
     HCHK(hipEventCreate(&e1));
     HCHK(hipEventRecord(e0, h->stream));
@@ -1635,6 +1668,7 @@ int mhd_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_
     for (int s = 0; s < nsteps; s++) {
         dt_kernels(h, s > 0);
         step_body(h);
+        restore_after_collapse(h);
     }
     HCHK(hipEventRecord(e1, h->stream));
     HCHK(hipEventSynchronize(e1));
@@ -1646,10 +1680,11 @@ int mhd_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_
     HCHK(hipMemcpy(v, h->dtv, 8 * sizeof(double), hipMemcpyDeviceToHost));
     HCHK(hipGetLastError());
     if (h->prof) { int r = prof_collect(h); if (r) return r; }
-    *time = v[1] + (nsteps > 0 ? v[4] : 0.0);
-    if (dt_last) *dt_last = v[4];
+    const bool collapsed = v[3] != 0.0;
+    *time = v[1] + (nsteps > 0 && !collapsed ? v[4] : 0.0);
+    if (dt_last) *dt_last = collapsed ? v[5] : v[4];
     if (elapsed_ms) *elapsed_ms = ms;
-    if (v[3] != 0.0) { set_err(h, "dt collapsed below 0.1*dt0"); return MHD_ERR_DT_COLLAPSE; }
+    if (collapsed) { set_err(h, "dt collapsed below 0.1*dt0"); return MHD_ERR_DT_COLLAPSE; }
     return MHD_OK;
 }
 
@@ -1958,8 +1993,10 @@ static int capture_pair(mhd_handle *h, int opening, hipGraphExec_t *out)
     HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     dt_kernels(h, opening ? 0 : 1);
     step_body(h);
+    restore_after_collapse(h);
     dt_kernels(h, 1);
     step_body(h);
+    restore_after_collapse(h);
     HCHK(hipStreamEndCapture(h->stream, &gr));
     const hipError_t e = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gr);
@@ -2093,6 +2130,7 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
         for (int r = 0; r < n; r++) {
             dt_kernels(e->runs[r], nsteps > 1 ? 1 : 0);
             step_body(e->runs[r]);
+            restore_after_collapse(e->runs[r]);
         }
     for (int r = 0; r < n; r++) {
         if ((st = hipEventRecord(e->done[r], e->runs[r]->stream)) != hipSuccess) return hip_fail(st, "hipEventRecord");
@@ -2109,7 +2147,7 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
         mhd_handle *h = e->runs[r];
         double v[8];
         if ((st = hipMemcpy(v, h->dtv, sizeof(v), hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(st, "dt download");
-        time[r] = v[1] + (nsteps > 0 ? v[4] : 0.0);
+        time[r] = v[1] + (nsteps > 0 && v[3] == 0.0 ? v[4] : 0.0);
         if (v[3] != 0.0) {
             set_err(h, "dt collapsed below 0.1*dt0");
             e->status[r] = MHD_ERR_DT_COLLAPSE;

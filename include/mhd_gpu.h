@@ -110,12 +110,13 @@ int mhd_step(mhd_handle *h, double time, double dt);
 int mhd_set_inflow_face(mhd_handle *h, const double *bxinfc, const double *byinfc, const double *bzinfc);
 /* the time loop body of program main (t3dsdsc.f:139-176) without I/O:
  * nsteps x (setdt; stepon; time += dt) with dt kept on the device.
- * dt0 > 0 arms the collapse guard dt < 0.1*dt0 (t3dsdsc.f:142): a device
- * flag, reported as MHD_ERR_DT_COLLAPSE when the call returns (the steps
- * queued after the collapsing one have run too; a caller that must stop
- * at that step, as the reference does, drives setdt/stepon per step).
- * On return *time is advanced, *dt_last holds the last dt and
- * *elapsed_ms the HIP-event time of the loop on the handle's stream. */
+ * dt0 > 0 arms the collapse guard dt < 0.1*dt0 (t3dsdsc.f:142): the loop
+ * stops at the collapsing setdt as the reference does (a device flag; the
+ * steps queued after it leave the state and time alone) and the call
+ * returns MHD_ERR_DT_COLLAPSE with the state and *time of before that step
+ * and *dt_last = the collapsing dt.  Otherwise *time is advanced and
+ * *dt_last holds the last dt.  *elapsed_ms is the HIP-event time of the
+ * loop on the handle's stream. */
 int mhd_advance(mhd_handle *h, int nsteps, double dt0, double *time, double *dt_last,
                 double *elapsed_ms);
 /* current (t3dsdsc.f:2452-2500) of the current state, cur(mx+4,my,mz+4,3) */
@@ -160,8 +161,8 @@ void mhd_destroy(mhd_handle *h);
  * must outlive the ensemble.  mhd_ensemble_advance advances every run
  * nsteps x (setdt; stepon; time += dt) with its own device dt, all runs
  * concurrently: each run's step pair is a captured HIP graph replayed on
- * the run's stream.  dt0[r] arms run r's collapse guard; time[r] is
- * advanced in place; *elapsed_ms is the wall time of the whole batch
+ * the run's stream.  dt0[r] arms run r's collapse guard (a collapsing run
+ * stops there, as in mhd_advance); time[r] is advanced in place; *elapsed_ms is the wall time of the whole batch
  * (HIP events).  Returns the first failing run's status (its text via
  * mhd_ensemble_error); mhd_ensemble_status gives every run's.  While
  * grouped, a run's kernels march longer z chunks (the runs fill the device

@@ -99,3 +99,57 @@ def test_bad_arguments():
     with pytest.raises(MHDError):
         h.adjoint_run([0.005, -1.0], 0)             # dt must be positive
     h.close()
+
+
+def _collapse_plan(c, n=8):
+    """the dt sequence of n steps and a dt0 whose guard 0.1 dt0 falls between
+    the smallest dt before step k and dt_k, k the first step whose dt is below
+    every earlier one (the guard then trips exactly there).  No frozen
+    residual: the perturbation's force accelerates the flow and dt falls."""
+    from dasha_amd.mhd import MHDStep
+    g = MHDStep(c)
+    t, dts = g.run(n, init=False)
+    g.close()
+    for k in range(1, n):
+        if dts[k] < min(dts[:k]):
+            return dts, k, 10.0 * 0.5 * (dts[k] + min(dts[:k]))
+    pytest.skip("dt does not fall in this run")
+
+
+def test_guarded_loop_stops_at_the_collapsing_step():
+    """mhd_advance and the ensemble stop at the setdt that collapses (the
+    reference stops before that stepon): state and time are those after
+    k steps, dt_last is the collapsing dt"""
+    from dasha_amd.mhd import MHDEnsemble, MHDError, MHDStep
+    c = harris.case_small(24, 8, 24, seed=35, amp=1.0)
+    dts, k, dt0 = _collapse_plan(c)
+    print(f"[guards] collapse at step {k} of {dts}", flush=True)
+    ref = MHDStep(c)
+    tk, _, _ = ref.advance(k, 0.0)
+    xk = ref.get_state()
+    g = MHDStep(c)
+    with pytest.raises(MHDError) as e:
+        g.advance(len(dts), 0.0, dt0)
+    assert e.value.code == -4
+    assert np.array_equal(g.get_state(), xk)
+    # the ensemble: one run collapses at step k, the other runs through
+    a, b = MHDStep(c), MHDStep(c)
+    ens = MHDEnsemble([a, b])
+    with pytest.raises(MHDError):
+        ens.advance(len(dts), [0.0, 0.0], [dt0, 0.0])
+    assert ens.status() == [-4, 0]
+    assert np.array_equal(a.get_state(), xk)
+    full = MHDStep(c)
+    full.advance(len(dts), 0.0)
+    assert np.array_equal(b.get_state(), full.get_state())
+    ens.close()
+    for h in (ref, g, a, b, full):
+        h.close()
+    # the time the loop reports on collapse is the time after k steps
+    h = MHDStep(c)
+    L = h.L
+    import ctypes as C
+    t, dl, ms = C.c_double(0.0), C.c_double(), C.c_double()
+    rc = L.mhd_advance(h.h, len(dts), dt0, C.byref(t), C.byref(dl), C.byref(ms))
+    assert rc == -4 and t.value == tk and dl.value == dts[k]
+    h.close()
