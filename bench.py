@@ -184,10 +184,9 @@ def time_ensemble(steps: int, nruns: int = 48):
         times[r], _, ms = g.advance(steps, times[r], d)
         seq_ms += ms
     ens = MHDEnsemble(runs)
-    # the autotuner: 4 trial steps of the whole batch per z-chunk policy
-    policy, trial_ms = ens.tune(4, times, dt0)
-    print(f"[bench] ensemble: tuned policy {policy}, trial ms {[round(m, 1) for m in trial_ms]}",
-          file=sys.stderr, flush=True)
+    # the autotuner: 4 trial steps of the whole batch per tile shape
+    shape, trials = ens.tune_shapes(4, times, dt0)
+    print(f"[bench] ensemble: tuned shape {shape}", file=sys.stderr, flush=True)
     tt, _ = ens.advance(2, times, dt0)           # captures the step graphs
     tt, ens_ms = ens.advance(steps, tt, dt0)
     ens.close()
@@ -199,8 +198,7 @@ def time_ensemble(steps: int, nruns: int = 48):
             "runs": nruns, "cells": cells, "steps": steps, "unit": "cell-updates/s",
             "value": cells * steps / (ens_ms / 1e3), "sequential_value": cells * steps / (seq_ms / 1e3),
             "speedup": seq_ms / ens_ms, "ms": ens_ms, "sequential_ms": seq_ms,
-            "tuned_policy": ["long chunks", "each run's own", "16/16", "8/8"][policy],
-            "tuner_trial_ms": trial_ms}
+            "tuned_shape": shape, "tuner_trials": trials}
 
 
 def time_adjoint(steps: int = 500):

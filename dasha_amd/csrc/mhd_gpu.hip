@@ -1951,26 +1951,39 @@ struct RunGraphs {
     hipGraphExec_t next = nullptr;
 };
 
+// The whole batch's step pair as one graph: every run's pair on its own
+// stream, forked from and joined back to the lead stream during capture.
+struct BatchGraphs {
+    std::vector<double *> xs;              // the runs' x buffers when captured
+    hipGraphExec_t first = nullptr;
+    hipGraphExec_t next = nullptr;
+};
+
 struct mhd_ensemble {
     std::vector<mhd_handle *> runs;
     std::vector<std::pair<int, int>> chunks;   // each run's own (kc, zkc), restored at destroy
+    std::vector<int> face12s;                  // and its face-window rows choice
     int policy = 0;                            // z-chunk policy in force (ENS_POLICIES)
+    bool batch = false;                        // replay one graph of the whole batch
     std::vector<RunGraphs> graphs;         // two per run
+    BatchGraphs bg[2];
     std::vector<int> status;
     std::vector<hipEvent_t> done;          // end of each run's work
     hipStream_t lead = nullptr;            // timing stream
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, fork = nullptr;
     std::string err;
 };
 
-// z chunk lengths of a handle's tile and smoothing kernels (the smoothing
-// block tables are rebuilt).  Results do not depend on them.
-static int set_chunks(mhd_handle *h, int kc, int zkc)
+// tile shape of a handle: z chunk lengths of its tile and smoothing
+// kernels and the rows of its face windows (the smoothing block tables are
+// rebuilt).  Results do not depend on them.
+static int set_shape(mhd_handle *h, int kc, int zkc, int face12)
 {
-    if (h->kc == kc && h->zkc == zkc) return 0;
+    if (h->kc == kc && h->zkc == zkc && h->face12 == face12) return 0;
     HCHK(hipStreamSynchronize(h->stream));
     h->kc = kc;
     h->zkc = zkc;
+    h->face12 = face12;
     for (auto &zc : h->zt)
         for (auto &t : zc) {
             if (t.d) (void)hipFree(t.d);
@@ -1986,6 +1999,8 @@ static int set_chunks(mhd_handle *h, int kc, int zkc)
     }
     return zs_tables(h, 3, 3, h->zt3);
 }
+
+static int set_chunks(mhd_handle *h, int kc, int zkc) { return set_shape(h, kc, zkc, h->face12); }
 
 static int capture_pair(mhd_handle *h, int opening, hipGraphExec_t *out)
 {
@@ -2012,6 +2027,65 @@ static void drop_graphs(RunGraphs &g)
     if (g.first) (void)hipGraphExecDestroy(g.first);
     if (g.next) (void)hipGraphExecDestroy(g.next);
     g = RunGraphs{};
+}
+
+static void drop_batch(BatchGraphs &g)
+{
+    if (g.first) (void)hipGraphExecDestroy(g.first);
+    if (g.next) (void)hipGraphExecDestroy(g.next);
+    g = BatchGraphs{};
+}
+
+// one step pair of every run, captured from the lead stream
+static int capture_batch(mhd_ensemble *e, int opening, hipGraphExec_t *out)
+{
+    hipGraph_t gr = nullptr;
+    auto fail = [&](hipError_t st, const char *what) {
+        e->err = std::string(what) + ": " + hipGetErrorString(st);
+        return MHD_ERR_HIP;
+    };
+    hipError_t st = hipStreamBeginCapture(e->lead, hipStreamCaptureModeThreadLocal);
+    if (st != hipSuccess) return fail(st, "hipStreamBeginCapture");
+    st = hipEventRecord(e->fork, e->lead);
+    for (size_t r = 0; r < e->runs.size() && st == hipSuccess; r++) {
+        mhd_handle *h = e->runs[r];
+        st = hipStreamWaitEvent(h->stream, e->fork, 0);
+        if (st != hipSuccess) break;
+        dt_kernels(h, opening ? 0 : 1);
+        step_body(h);
+        restore_after_collapse(h);
+        dt_kernels(h, 1);
+        step_body(h);
+        restore_after_collapse(h);
+        st = hipEventRecord(e->done[r], h->stream);
+        if (st == hipSuccess) st = hipStreamWaitEvent(e->lead, e->done[r], 0);
+    }
+    const hipError_t se = hipStreamEndCapture(e->lead, &gr);
+    if (st != hipSuccess) { if (gr) (void)hipGraphDestroy(gr); return fail(st, "batch capture"); }
+    if (se != hipSuccess) return fail(se, "hipStreamEndCapture");
+    st = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (st != hipSuccess) return fail(st, "hipGraphInstantiate");
+    return MHD_OK;
+}
+
+// the batch graphs for the runs' current pointer states (captured when missing)
+static int batch_graphs(mhd_ensemble *e, BatchGraphs **out)
+{
+    std::vector<double *> xs;
+    for (auto *h : e->runs) xs.push_back(h->x);
+    BatchGraphs *slot = nullptr;
+    for (auto &g : e->bg) {
+        if (g.first && g.xs == xs) { *out = &g; return MHD_OK; }
+        if (!slot && !g.first) slot = &g;
+    }
+    if (!slot) { slot = &e->bg[0]; drop_batch(*slot); }
+    int rc = capture_batch(e, 1, &slot->first);
+    if (!rc) rc = capture_batch(e, 0, &slot->next);
+    if (rc) { drop_batch(*slot); return rc; }
+    slot->xs = xs;
+    *out = slot;
+    return MHD_OK;
 }
 
 // the graphs of run r for its current pointer state (captured when missing)
@@ -2051,6 +2125,7 @@ static std::pair<int, int> policy_chunks(const mhd_ensemble *e, int r, int p)
 static int apply_policy(mhd_ensemble *e, int p)
 {
     for (auto &g : e->graphs) drop_graphs(g);   // captured with the old launch arguments
+    for (auto &g : e->bg) drop_batch(g);
     for (size_t r = 0; r < e->runs.size(); r++) {
         const auto c = policy_chunks(e, (int)r, p);
         const int rc = set_chunks(e->runs[r], c.first, c.second);
@@ -2080,12 +2155,16 @@ int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out)
     hipError_t st = hipStreamCreate(&e->lead);
     if (st == hipSuccess) st = hipEventCreate(&e->e0);
     if (st == hipSuccess) st = hipEventCreate(&e->e1);
+    if (st == hipSuccess) st = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
     for (int r = 0; r < n && st == hipSuccess; r++) st = hipEventCreateWithFlags(&e->done[r], hipEventDisableTiming);
     if (st != hipSuccess) { e->err = std::string("ensemble set-up: ") + hipGetErrorString(st); return MHD_ERR_HIP; }
     // runs that share the device do not need short chunks to fill it: the
     // long ones (less z halo per plane) measured faster for the C4 batch;
     // mhd_ensemble_tune picks by measurement
-    for (int r = 0; r < n; r++) e->chunks.emplace_back(e->runs[r]->kc, e->runs[r]->zkc);
+    for (int r = 0; r < n; r++) {
+        e->chunks.emplace_back(e->runs[r]->kc, e->runs[r]->zkc);
+        e->face12s.push_back(e->runs[r]->face12);
+    }
     return apply_policy(e, 0);
 }
 
@@ -2101,10 +2180,15 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
             e->err = "run " + std::to_string(r) + (h->prof ? ": profiling is on" : ": equilibrium not set");
             return MHD_ERR_ARG;
         }
-        if (nsteps >= 2) {
+        if (nsteps >= 2 && !e->batch) {
             const int rc = run_graphs(e, r, &gs[r]);
             if (rc) { e->err = "run " + std::to_string(r) + ": " + h->err; return rc; }
         }
+    }
+    BatchGraphs *bgs = nullptr;
+    if (nsteps >= 2 && e->batch) {
+        const int rc = batch_graphs(e, &bgs);
+        if (rc) return rc;
     }
     auto hip_fail = [&](hipError_t st, const char *what) {
         e->err = std::string(what) + ": " + hipGetErrorString(st);
@@ -2121,11 +2205,22 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
     if ((st = hipEventRecord(e->e0, e->lead)) != hipSuccess) return hip_fail(st, "hipEventRecord");
     for (int r = 0; r < n; r++)
         if ((st = hipStreamWaitEvent(e->runs[r]->stream, e->e0, 0)) != hipSuccess) return hip_fail(st, "hipStreamWaitEvent");
-    // step pairs round-robin over the runs, then an odd last step eagerly
-    for (int p = 0; p < nsteps / 2; p++)
-        for (int r = 0; r < n; r++)
-            if ((st = hipGraphLaunch(p == 0 ? gs[r]->first : gs[r]->next, e->runs[r]->stream)) != hipSuccess)
+    // step pairs round-robin over the runs (or the batch graph on the lead
+    // stream), then an odd last step eagerly
+    if (bgs) {
+        for (int p = 0; p < nsteps / 2; p++)
+            if ((st = hipGraphLaunch(p == 0 ? bgs->first : bgs->next, e->lead)) != hipSuccess)
                 return hip_fail(st, "hipGraphLaunch");
+        if ((st = hipEventRecord(e->fork, e->lead)) != hipSuccess) return hip_fail(st, "hipEventRecord");
+        for (int r = 0; r < n; r++)
+            if ((st = hipStreamWaitEvent(e->runs[r]->stream, e->fork, 0)) != hipSuccess)
+                return hip_fail(st, "hipStreamWaitEvent");
+    } else {
+        for (int p = 0; p < nsteps / 2; p++)
+            for (int r = 0; r < n; r++)
+                if ((st = hipGraphLaunch(p == 0 ? gs[r]->first : gs[r]->next, e->runs[r]->stream)) != hipSuccess)
+                    return hip_fail(st, "hipGraphLaunch");
+    }
     if (nsteps % 2)
         for (int r = 0; r < n; r++) {
             dt_kernels(e->runs[r], nsteps > 1 ? 1 : 0);
@@ -2204,6 +2299,111 @@ int mhd_ensemble_tune(mhd_ensemble *e, int trial, const double *dt0, const doubl
     return apply_policy(e, pick);
 }
 
+int mhd_ensemble_set_shape(mhd_ensemble *e, const mhd_tile_shape *sh)
+{
+    if (!e || !sh || sh->kc < 1 || sh->zkc < 4 || sh->zkc > ZKC_MAX || (sh->face_rows != 12 && sh->face_rows != 16))
+        return MHD_ERR_ARG;
+    for (auto &g : e->graphs) drop_graphs(g);
+    for (auto &g : e->bg) drop_batch(g);
+    for (size_t r = 0; r < e->runs.size(); r++) {
+        const int rc = set_shape(e->runs[r], sh->kc, sh->zkc, sh->face_rows == 12 ? 1 : 0);
+        if (rc) { e->err = "run " + std::to_string(r) + ": " + e->runs[r]->err; return rc; }
+    }
+    e->batch = sh->batch_graph != 0;
+    return MHD_OK;
+}
+
+int mhd_ensemble_tune_shapes(mhd_ensemble *e, int trial, const double *dt0, const double *time, int max_trials,
+                             mhd_tile_shape *trials, int *ntrials, mhd_tile_shape *best)
+{
+    if (!e || trial < 1 || !dt0 || !time || max_trials < 0 || (max_trials > 0 && !trials)) return MHD_ERR_ARG;
+    const int n = (int)e->runs.size();
+    std::vector<double *> keep(n, nullptr);
+    auto release = [&]() {
+        for (auto p : keep)
+            if (p) (void)hipFree(p);
+    };
+    for (int r = 0; r < n; r++) {
+        mhd_handle *h = e->runs[r];
+        if (hipMalloc(&keep[r], 8 * h->S * sizeof(double)) != hipSuccess ||
+            hipMemcpyAsync(keep[r], h->x, 8 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            release();
+            e->err = "tune: saving the states failed";
+            return MHD_ERR_NOMEM;
+        }
+    }
+    auto apply = [&](const mhd_tile_shape &sh) {
+        for (auto &g : e->graphs) drop_graphs(g);
+        for (auto &g : e->bg) drop_batch(g);
+        for (int r = 0; r < n; r++) {
+            const int rc = set_shape(e->runs[r], sh.kc, sh.zkc, sh.face_rows == 12 ? 1 : 0);
+            if (rc) { e->err = "run " + std::to_string(r) + ": " + e->runs[r]->err; return rc; }
+        }
+        e->batch = sh.batch_graph != 0;
+        return (int)MHD_OK;
+    };
+    int count = 0, rc = MHD_OK;
+    // one trial: two untimed steps (they capture the graphs), then `trial`
+    // timed steps of the whole batch; the states are restored afterwards
+    auto measure = [&](mhd_tile_shape &sh) {
+        int r0 = apply(sh);
+        std::vector<double> t(time, time + n);
+        double el = 0;
+        if (!r0) r0 = mhd_ensemble_advance(e, 2, dt0, t.data(), nullptr);
+        if (!r0) r0 = mhd_ensemble_advance(e, trial, dt0, t.data(), &el);
+        sh.ms = (r0 == MHD_OK) ? el : 1e30;
+        for (int r = 0; r < n; r++) {
+            mhd_handle *h = e->runs[r];
+            if (hipMemcpyAsync(h->x, keep[r], 8 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess) {
+                e->err = "tune: restoring the states failed";
+                return (int)MHD_ERR_HIP;
+            }
+        }
+        if (count < max_trials) trials[count] = sh;
+        count++;
+        return (r0 == MHD_OK || r0 == MHD_ERR_DT_COLLAPSE) ? (int)MHD_OK : r0;
+    };
+    // coordinate search from the long-chunk shape: the smoothing chunk, the
+    // tile-kernel chunk, the face-window rows, then the graph form
+    mhd_tile_shape cur{32, 32, 12, 0, 0.0};
+    rc = measure(cur);
+    const int zkcs[] = {16, 24, 48, 64};
+    const int kcs[] = {8, 16, 64};
+    for (int v : zkcs) {
+        if (rc) break;
+        mhd_tile_shape s = cur;
+        s.zkc = v;
+        rc = measure(s);
+        if (!rc && s.ms < cur.ms) cur = s;
+    }
+    for (int v : kcs) {
+        if (rc) break;
+        mhd_tile_shape s = cur;
+        s.kc = v;
+        rc = measure(s);
+        if (!rc && s.ms < cur.ms) cur = s;
+    }
+    if (!rc) {
+        mhd_tile_shape s = cur;
+        s.face_rows = 16;
+        rc = measure(s);
+        if (!rc && s.ms < cur.ms) cur = s;
+    }
+    if (!rc) {
+        mhd_tile_shape s = cur;
+        s.batch_graph = 1;
+        rc = measure(s);
+        if (!rc && s.ms < cur.ms) cur = s;
+    }
+    release();
+    if (ntrials) *ntrials = count;
+    if (rc) return rc;
+    if (best) *best = cur;
+    return apply(cur);
+}
+
 int mhd_ensemble_status(const mhd_ensemble *e, int *status)
 {
     if (!e || !status) return MHD_ERR_ARG;
@@ -2219,7 +2419,10 @@ void mhd_ensemble_destroy(mhd_ensemble *e)
     if (e->lead) (void)hipStreamSynchronize(e->lead);
     for (auto *h : e->runs) (void)hipStreamSynchronize(h->stream);
     for (auto &g : e->graphs) drop_graphs(g);
-    for (size_t r = 0; r < e->chunks.size(); r++) (void)set_chunks(e->runs[r], e->chunks[r].first, e->chunks[r].second);
+    for (auto &g : e->bg) drop_batch(g);
+    for (size_t r = 0; r < e->chunks.size(); r++)
+        (void)set_shape(e->runs[r], e->chunks[r].first, e->chunks[r].second, e->face12s[r]);
+    if (e->fork) (void)hipEventDestroy(e->fork);
     for (auto ev : e->done)
         if (ev) (void)hipEventDestroy(ev);
     if (e->e0) (void)hipEventDestroy(e->e0);

@@ -29,7 +29,7 @@ EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state"
            "mhd_last_error", "mhd_destroy", "mhd_ensemble_create", "mhd_ensemble_advance",
            "mhd_ensemble_status", "mhd_ensemble_error", "mhd_ensemble_destroy",
            "mhd_adjoint_run", "mhd_adjoint_rhs", "mhd_ensemble_tune", "mhd_compute_dt_noalloc",
-           "mhd_set_inflow_face", "mhd_rhs_stage")
+           "mhd_set_inflow_face", "mhd_rhs_stage", "mhd_ensemble_tune_shapes", "mhd_ensemble_set_shape")
 
 MHD_ADJ_RESIDUAL = 1
 OBJECTIVES = {"magnetic_energy": 0, "reconnected_flux": 1}
@@ -48,6 +48,14 @@ class MHDDiag(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("cxmax", "cxmin", "cymax", "cymin", "czmax", "czmin",
                                            "e_kin", "e_mag", "e_th", "e_mag_pert", "divb_max",
                                            "divb_rms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class MHDTileShape(C.Structure):
+    _fields_ = [("kc", C.c_int), ("zkc", C.c_int), ("face_rows", C.c_int), ("batch_graph", C.c_int),
+                ("ms", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -100,6 +108,9 @@ def load_library(path: str = LIBPATH):
     L.mhd_adjoint_run.argtypes = [P, I, D, I, I, I, D, D, D, D]
     L.mhd_ensemble_tune.argtypes = [P, I, D, D, C.POINTER(I), D]
     L.mhd_adjoint_rhs.argtypes = [P, D, D, D, D, I, I]
+    L.mhd_ensemble_tune_shapes.argtypes = [P, I, D, D, I, C.POINTER(MHDTileShape), C.POINTER(I),
+                                           C.POINTER(MHDTileShape)]
+    L.mhd_ensemble_set_shape.argtypes = [P, C.POINTER(MHDTileShape)]
     _lib = L
     return L
 
@@ -351,6 +362,28 @@ class MHDEnsemble:
         if rc != MHD_OK:
             raise MHDError(rc, self.L.mhd_ensemble_error(self.e).decode())
         return best.value, [float(m) for m in ms]
+
+    def tune_shapes(self, trial: int, times, dt0s, max_trials: int = 32):
+        """coordinate search over the runs' tile shapes (smoothing and tile
+        z chunks, face-window rows, replay form); returns (best shape,
+        every shape tried), each a dict with its trial time 'ms'.  The best
+        stays in force; states and times are unchanged."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        d = np.ascontiguousarray(dt0s, dtype=np.float64)
+        tr = (MHDTileShape * max_trials)()
+        nt = C.c_int()
+        best = MHDTileShape()
+        rc = self.L.mhd_ensemble_tune_shapes(self.e, int(trial), _p(d), _p(t), max_trials, tr, C.byref(nt),
+                                             C.byref(best))
+        if rc != MHD_OK:
+            raise MHDError(rc, self.L.mhd_ensemble_error(self.e).decode())
+        return best.as_dict(), [tr[i].as_dict() for i in range(min(nt.value, max_trials))]
+
+    def set_shape(self, kc: int, zkc: int, face_rows: int = 12, batch_graph: bool = False):
+        sh = MHDTileShape(kc=kc, zkc=zkc, face_rows=face_rows, batch_graph=int(batch_graph), ms=0.0)
+        rc = self.L.mhd_ensemble_set_shape(self.e, C.byref(sh))
+        if rc != MHD_OK:
+            raise MHDError(rc, self.L.mhd_ensemble_error(self.e).decode() if self.e else "")
 
     def status(self):
         st = (C.c_int * len(self.runs))()

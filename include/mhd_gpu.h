@@ -184,6 +184,27 @@ void mhd_ensemble_destroy(mhd_ensemble *e);
  * runs' states and the caller's times are unchanged on return. */
 int mhd_ensemble_tune(mhd_ensemble *e, int trial, const double *dt0, const double *time, int *best,
                       double *ms);
+/* Tile-shape autotuner of the ensemble (no counterpart in the reference):
+ * a coordinate search over the shape of every run's launches -- the z chunk
+ * of the smoothing kernels (zkc 16..64 planes), of the tile kernels (kc
+ * 8..64), the rows of the face windows (12 or 16) and the replay form (each
+ * run's own graph on its stream, or one graph of the whole batch) -- timing
+ * `trial` batch steps per shape (after two untimed steps that capture the
+ * graphs) and restoring the states after each.  The fastest shape stays in
+ * force.  trials[max_trials] (may be NULL when max_trials is 0) receives
+ * every shape tried with its time, *ntrials their count, *best the choice. */
+typedef struct mhd_tile_shape {
+    int kc;           /* z planes per chunk of the tile kernels (curl, RHS)    */
+    int zkc;          /* interior z planes per chunk of the smoothing kernels  */
+    int face_rows;    /* rows of a face window's block (12 or 16)              */
+    int batch_graph;  /* 1: one graph of the whole batch per step pair         */
+    double ms;        /* the trial's time                                      */
+} mhd_tile_shape;
+int mhd_ensemble_tune_shapes(mhd_ensemble *e, int trial, const double *dt0, const double *time, int max_trials,
+                             mhd_tile_shape *trials, int *ntrials, mhd_tile_shape *best);
+/* put a shape in force for every run (the ms field is ignored); results do
+ * not depend on it */
+int mhd_ensemble_set_shape(mhd_ensemble *e, const mhd_tile_shape *shape);
 
 /* Discrete adjoint of the step (no counterpart in the reference, which has
  * no derivatives; the reverse mode of stepon/right, t3dsdsc.f:1063-1673).

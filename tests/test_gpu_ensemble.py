@@ -150,3 +150,47 @@ def test_member_close_waits_for_the_ensemble():
     assert members[1].h
     members[1].close()
     assert not members[1].h
+
+
+@pytest.mark.parametrize("shape", [dict(kc=8, zkc=16, face_rows=16, batch_graph=True),
+                                   dict(kc=64, zkc=48, face_rows=12, batch_graph=False),
+                                   dict(kc=16, zkc=24, face_rows=12, batch_graph=True)])
+def test_shapes_and_batch_graph_match_single_runs(shape):
+    """any tile shape and the whole-batch graph give the runs alone, bit for
+    bit (pairs from both pointer states and an odd step)"""
+    from dasha_amd.mhd import MHDEnsemble, MHDStep
+    cases = _cases()
+    print(f"[ensemble] shape {shape}", flush=True)
+    singles = [MHDStep(c) for c in cases]
+    members = [MHDStep(c) for c in cases]
+    for g in singles + members:
+        g.init_residual()
+    dt0 = [g.compute_dt() for g in singles]
+    ens = MHDEnsemble(members)
+    ens.set_shape(**shape)
+    t_ens = np.zeros(len(cases))
+    for nsteps in (4, 3, 2):
+        t_ref = [g.advance(nsteps, float(t), d)[0] for g, d, t in zip(singles, dt0, t_ens)]
+        t_ens, _ = ens.advance(nsteps, t_ens, dt0)
+        assert list(t_ens) == t_ref
+        for a, b in zip(singles, members):
+            assert np.array_equal(a.get_state(), b.get_state())
+    ens.close()
+
+
+def test_shape_tuner_keeps_states():
+    from dasha_amd.mhd import MHDEnsemble, MHDStep
+    cases = _cases()
+    members = [MHDStep(c) for c in cases]
+    for g in members:
+        g.init_residual()
+    dt0 = [g.compute_dt() for g in members]
+    x_before = [g.get_state() for g in members]
+    ens = MHDEnsemble(members)
+    best, trials = ens.tune_shapes(2, np.zeros(len(cases)), dt0)
+    print(f"[ensemble] tuned shape {best}; {len(trials)} trials", flush=True)
+    assert len(trials) == 10 and all(t["ms"] > 0 for t in trials)
+    assert best["ms"] == min(t["ms"] for t in trials)
+    for g, x in zip(members, x_before):
+        assert np.array_equal(g.get_state(), x)
+    ens.close()
