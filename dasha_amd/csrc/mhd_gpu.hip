@@ -12,12 +12,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mhd_gpu.h"
@@ -91,16 +94,18 @@ __global__ void k_copy(long n, const double *__restrict__ a, double *__restrict_
 // jx = 3..mx (k = 0..NZ-3, j = 0..NY-2, i = 0..NX-3); with periodic y the
 // image rows repeat unique rows, so the unique rows give the same minimum
 // ---------------------------------------------------------------------------
-// rr (magnetosphere mode, else null): only points with rr1 >= 1 count (t3dsdsc.f:4712)
-__global__ void k_dt_partial(Geo g, const double *__restrict__ X, double eta, double fmu, double cfl,
-                             double gam, double wcoef, const double *__restrict__ rr, double *__restrict__ part)
+// rr (magnetosphere mode, else null): only points with rr1 >= 1 count (t3dsdsc.f:4712);
+// block b of nb writes part[b]
+__device__ __forceinline__ void dt_partial_body(const Geo &g, const double *__restrict__ X, double eta, double fmu,
+                                                double cfl, double gam, double wcoef, const double *__restrict__ rr,
+                                                double *__restrict__ part, unsigned b, unsigned nb)
 {
     __shared__ double sm[256];
     int j0 = g.periodic ? 2 : 0, j1 = g.periodic ? g.NY - 3 : g.NY - 2;
     long ni = g.NX - 2, nj = j1 - j0 + 1, nk = g.NZ - 2;
     long n = ni * nj * nk;
     double best = 100.;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    for (long t = (long)b * blockDim.x + threadIdx.x; t < n; t += (long)nb * blockDim.x) {
         int i = (int)(t % ni); long r = t / ni;
         int j = j0 + (int)(r % nj); int k = (int)(r / nj);
         long q = IDX(g, i, j, k), S = g.S;
@@ -137,7 +142,13 @@ __global__ void k_dt_partial(Geo g, const double *__restrict__ X, double eta, do
         if ((int)threadIdx.x < s && sm[threadIdx.x + s] < sm[threadIdx.x]) sm[threadIdx.x] = sm[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+    if (threadIdx.x == 0) part[b] = sm[0];
+}
+
+__global__ void k_dt_partial(Geo g, const double *__restrict__ X, double eta, double fmu, double cfl, double gam,
+                             double wcoef, const double *__restrict__ rr, double *__restrict__ part)
+{
+    dt_partial_body(g, X, eta, fmu, cfl, gam, wcoef, rr, part, blockIdx.x, gridDim.x);
 }
 
 // final MIN; keeps the running time and the collapse flag on the device:
@@ -147,7 +158,7 @@ __global__ void k_dt_partial(Geo g, const double *__restrict__ X, double eta, do
 // step after it runs with the last good dt and k_restore undoes it, so the
 // state and time stay those before the collapsing step, as the reference
 // stops there.
-__global__ void k_dt_final(const double *__restrict__ part, int n, double *dtv, int advance_time)
+__device__ __forceinline__ void dt_final_body(const double *__restrict__ part, int n, double *dtv, int advance_time)
 {
     __shared__ double sm[256];
     double best = 100.;
@@ -173,12 +184,60 @@ __global__ void k_dt_final(const double *__restrict__ part, int n, double *dtv, 
     }
 }
 
+__global__ void k_dt_final(const double *__restrict__ part, int n, double *dtv, int advance_time)
+{
+    dt_final_body(part, n, dtv, advance_time);
+}
+
 // after a step of a guarded loop: once the collapse flag is up, the step
 // that followed the collapsing setdt is undone (x = xfold, the state before it)
-__global__ void k_restore(const double *__restrict__ dtv, const double *__restrict__ XF, double *__restrict__ X, long n)
+__device__ __forceinline__ void restore_body(const double *__restrict__ dtv, const double *__restrict__ XF,
+                                             double *__restrict__ X, long n, unsigned b, unsigned nb)
 {
     if (dtv[3] == 0.0) return;
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) X[q] = XF[q];
+    for (long q = (long)b * blockDim.x + threadIdx.x; q < n; q += (long)nb * blockDim.x) X[q] = XF[q];
+}
+
+__global__ void k_restore(const double *__restrict__ dtv, const double *__restrict__ XF, double *__restrict__ X, long n)
+{
+    restore_body(dtv, XF, X, n, blockIdx.x, gridDim.x);
+}
+
+// batched forms of setdt and the collapse undo (one launch over every run of
+// an ensemble; {run, block} entries, arguments per run)
+struct BDtArgs {
+    Geo g;
+    const double *X, *rr;
+    double eta, fmu, cfl, gam, wcoef;
+    double *part, *dtv;
+    int nb, advance_time;
+};
+
+struct BRestoreArgs {
+    const double *dtv, *XF;
+    double *X;
+    long n;
+    int nb;
+};
+
+__global__ void kb_dt_partial(const BDtArgs *__restrict__ args, const BEntry *__restrict__ tab)
+{
+    const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
+    const BDtArgs &a = run_args(args, run);
+    dt_partial_body(a.g, a.X, a.eta, a.fmu, a.cfl, a.gam, a.wcoef, a.rr, a.part, tab[blockIdx.x].w, a.nb);
+}
+
+__global__ void kb_dt_final(const BDtArgs *__restrict__ args, const BEntry *__restrict__ tab)
+{
+    const BDtArgs &a = run_args(args, tab[blockIdx.x].run);
+    dt_final_body(a.part, a.nb, a.dtv, a.advance_time);
+}
+
+__global__ void kb_restore(const BRestoreArgs *__restrict__ args, const BEntry *__restrict__ tab)
+{
+    const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
+    const BRestoreArgs &a = run_args(args, run);
+    restore_body(a.dtv, a.XF, a.X, a.n, tab[blockIdx.x].w, a.nb);
 }
 
 // disarm the collapse guard (a stand-alone setdt reports every dt)
@@ -345,10 +404,22 @@ struct Prof {
     int calls;
 };
 
-// block table of a temporal-blocked launch (device words, one per block)
+// block table of a temporal-blocked launch (device words in dispatch
+// order, one per block; the same words in enumeration order on the host)
 struct ZTab {
     unsigned *d = nullptr;
     unsigned n = 0;
+    std::vector<unsigned> nat;
+};
+
+// One launch of a recorded step sequence, for the batched kernels of an
+// ensemble: the launcher of the batched kernel, the run's argument block
+// and the run's block words (enumeration order).
+using BLaunch = void (*)(const void *args, const BEntry *tab, unsigned nblk, hipStream_t s);
+struct RecLaunch {
+    BLaunch fn;
+    std::string arg;
+    std::vector<unsigned> words;
 };
 
 struct mhd_handle {
@@ -393,6 +464,11 @@ struct mhd_handle {
     // right() copies its state and current into slot rec_n and advances it
     double *rec_x = nullptr, *rec_c = nullptr;
     int rec_n = 0;
+    // when set, the launches that have a batched form are appended here
+    // instead of issued (ensemble batching); rec_err names a launch shape
+    // the batched forms cannot take
+    std::vector<RecLaunch> *rec = nullptr;
+    std::string rec_err;
     // profiling
     bool prof = false;
     std::vector<Prof> pt;
@@ -651,14 +727,128 @@ static int zs_tables(mhd_handle *h, int nf, int np, ZTab *out)
         }
         HCHK(hipMalloc(&out[c].d, T * sizeof(unsigned)));
         HCHK(hipMemcpy(out[c].d, w.data(), T * sizeof(unsigned), hipMemcpyHostToDevice));
+        out[c].nat = std::move(list[c]);
     }
     return MHD_OK;
 }
 
+// ---- recording for the batched ensemble kernels ----
+template <class A>
+static void rec_add(mhd_handle *h, BLaunch fn, const A &a, std::vector<unsigned> words)
+{
+    static_assert(std::is_trivially_copyable<A>::value, "argument blocks travel as bytes");
+    h->rec->push_back(RecLaunch{fn, std::string((const char *)&a, sizeof(A)), std::move(words)});
+}
+
+// argument blocks start zeroed, padding included (they are compared as bytes)
+template <class A>
+static A zeroed()
+{
+    A a;
+    std::memset((void *)&a, 0, sizeof(A));
+    return a;
+}
+
+// words 0..n-1 (one-dimensional launches)
+static std::vector<unsigned> lin_words(unsigned n)
+{
+    std::vector<unsigned> w(n);
+    for (unsigned i = 0; i < n; i++) w[i] = i;
+    return w;
+}
+
+// words of a gx x gy x gz grid, x fastest: bx | by << ZT_BY | bz << ZT_CH
+static std::vector<unsigned> grid_words(mhd_handle *h, dim3 gr)
+{
+    if (gr.x >= (1u << ZT_BY) || gr.y >= (1u << (ZT_CH - ZT_BY)) || gr.z >= (1u << (32 - ZT_CH))) {
+        h->rec_err = "grid too large for the batched kernels' block words";
+        return {};
+    }
+    std::vector<unsigned> w;
+    w.reserve((size_t)gr.x * gr.y * gr.z);
+    for (unsigned z = 0; z < gr.z; z++)
+        for (unsigned y = 0; y < gr.y; y++)
+            for (unsigned x = 0; x < gr.x; x++) w.push_back(x | y << ZT_BY | z << ZT_CH);
+    return w;
+}
+
+template <int NP, int KIND, int TYPE, int RKS, int MODE, int RB>
+static void bl_zs(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL((kb_zsmooth<NP, KIND, TYPE, RKS, MODE, RB>), dim3(n), dim3(TX * RB), 0, s, (const BZArgs *)a, t);
+}
+
+static void bl_curl(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_curl, dim3(n), dim3(NT), 0, s, (const BCurlArgs *)a, t);
+}
+
+static void bl_rhs_a(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_rhs_a, dim3(n), dim3(NT), 0, s, (const BRhsAArgs *)a, t);
+}
+
+static void bl_rhs_b(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_rhs_b, dim3(n), dim3(NT), 0, s, (const BRhsBArgs *)a, t);
+}
+
+template <int PART, int RB>
+static void bl_rhs_b2(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL((kb_rhs_b2<PART, RB>), dim3(n), dim3(TX * RB), 0, s, (const BRhsB2Args *)a, t);
+}
+
+static void bl_fill(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_fill_state, dim3(n), dim3(256), 0, s, (const BFillArgs *)a, t);
+}
+
+static void bl_dt_partial(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_dt_partial, dim3(n), dim3(256), 0, s, (const BDtArgs *)a, t);
+}
+
+static void bl_dt_final(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_dt_final, dim3(n), dim3(256), 0, s, (const BDtArgs *)a, t);
+}
+
+static void bl_restore(const void *a, const BEntry *t, unsigned n, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_restore, dim3(n), dim3(256), 0, s, (const BRestoreArgs *)a, t);
+}
+
+// blocks per run of the batched collapse undo (it copies only after a collapse)
+static constexpr unsigned RESTORE_BLOCKS = 128;
+
 template <int NP, int KIND, int TYPE, int RKS, int MODE>
-static void launch_zs(mhd_handle *h, dim3 grid, const unsigned *tab, int nch, const double *a, double *b,
+static void launch_zs(mhd_handle *h, dim3 grid, const ZTab *zt, int nch, const double *a, double *b,
                       const RKArgs &rk)
 {
+    const unsigned *tab = zt ? zt->d : nullptr;
+    if (h->rec) {
+        BZArgs z = zeroed<BZArgs>();
+        z.g = h->g; z.A = a; z.B = b; z.rk = rk; z.kc = h->zkc;
+        std::vector<unsigned> w;
+        if (zt) {
+            w = zt->nat;
+        } else {
+            // the full march's enumeration (k_zsmooth ZM_ALL): x tile fastest,
+            // then y tile, chunk, field
+            const unsigned nf = grid.z / (unsigned)nch;
+            for (unsigned m = 0; m < nf; m++)
+                for (unsigned ch = 0; ch < (unsigned)nch; ch++)
+                    for (unsigned y = 0; y < grid.y; y++)
+                        for (unsigned x = 0; x < grid.x; x++) w.push_back(x | y << ZT_BY | ch << ZT_CH | m << ZT_F);
+        }
+        BLaunch fn = bl_zs<NP, KIND, TYPE, RKS, MODE, ZB>;
+        if constexpr (MODE == ZM_FACE) {
+            if (face_rows(h) == 12) fn = bl_zs<NP, KIND, TYPE, RKS, MODE, 12>;
+        }
+        rec_add(h, fn, z, std::move(w));
+        return;
+    }
     if constexpr (MODE == ZM_FACE) {
         if (face_rows(h) == 12) {
             hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, MODE, 12>), grid, dim3(TX * 12), 0, h->stream, h->g, a,
@@ -687,8 +877,8 @@ static void launch_zsmooth(mhd_handle *h, const double *a, double *b, const doub
         launch_zs<NP, KIND, TYPE, RKS, ZM_ALL>(h, grid, nullptr, nch, a, b, rk);
     } else {
         const ZTab *zt = NP == 3 ? h->zt3 : h->zt[NF == 8 ? 1 : 0];
-        if (zt[0].n) launch_zs<NP, KIND, TYPE, RKS, ZM_PLAIN>(h, dim3(zt[0].n), zt[0].d, 0, a, b, rk);
-        if (zt[1].n) launch_zs<NP, KIND, TYPE, RKS, ZM_FACE>(h, dim3(zt[1].n), zt[1].d, 0, a, b, rk);
+        if (zt[0].n) launch_zs<NP, KIND, TYPE, RKS, ZM_PLAIN>(h, dim3(zt[0].n), &zt[0], 0, a, b, rk);
+        if (zt[1].n) launch_zs<NP, KIND, TYPE, RKS, ZM_FACE>(h, dim3(zt[1].n), &zt[1], 0, a, b, rk);
     }
 }
 
@@ -729,10 +919,17 @@ static double *smooth_aux(mhd_handle *h, double *a, double *b, int kind, int kk,
 // current (t3dsdsc.f:2452-2500) of state X into h->cur
 static void current(mhd_handle *h, const double *X)
 {
-    PROF_BEGIN(PT_CURL);
-    hipLaunchKernelGGL(k_curl, tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->eq + h->S, h->eq + 2 * h->S,
-                       h->eq + 3 * h->S, h->cA, h->kc);
-    PROF_END(PT_CURL);
+    if (h->rec) {
+        BCurlArgs a = zeroed<BCurlArgs>();
+        a.g = h->g; a.X = X; a.bxi = h->eq + h->S; a.byi = h->eq + 2 * h->S; a.bzi = h->eq + 3 * h->S;
+        a.cur = h->cA; a.kc = h->kc;
+        rec_add(h, bl_curl, a, grid_words(h, tile_grid(h)));
+    } else {
+        PROF_BEGIN(PT_CURL);
+        hipLaunchKernelGGL(k_curl, tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->eq + h->S, h->eq + 2 * h->S,
+                           h->eq + 3 * h->S, h->cA, h->kc);
+        PROF_END(PT_CURL);
+    }
     h->cur = smooth_aux(h, h->cA, h->cB, KIND_RADIAL, 4);
 }
 
@@ -751,10 +948,18 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
                              hipMemcpyDeviceToDevice, h->stream);
         h->rec_n++;
     }
-    PROF_BEGIN(PT_RHSA);
-    hipLaunchKernelGGL(k_rhs_a, tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->eq + 4 * h->S, h->cur, h->xresd,
-                       eta, c.hall, c.di, h->xdA, h->eA, h->hA, h->kc);
-    PROF_END(PT_RHSA);
+    if (h->rec) {
+        BRhsAArgs a = zeroed<BRhsAArgs>();
+        a.g = h->g; a.X = X; a.pri = h->eq + 4 * h->S; a.cur = h->cur; a.xresd = h->xresd;
+        a.eta = eta; a.cht = c.di; a.hall = c.hall; a.kc = h->kc;
+        a.XD = h->xdA; a.E = h->eA; a.XH = h->hA;
+        rec_add(h, bl_rhs_a, a, grid_words(h, tile_grid(h)));
+    } else {
+        PROF_BEGIN(PT_RHSA);
+        hipLaunchKernelGGL(k_rhs_a, tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->eq + 4 * h->S, h->cur,
+                           h->xresd, eta, c.hall, c.di, h->xdA, h->eA, h->hA, h->kc);
+        PROF_END(PT_RHSA);
+    }
     double *E = smooth_aux(h, h->eA, h->eB, KIND_RADIAL, 4);
     // with the Hall term the last avrgh sweep leaves E + xdh, the field whose
     // curl enters the induction equation (t3dsdsc.f:1426-1433)
@@ -762,7 +967,23 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
     const double *XH = nullptr;
     bool vd = fmu != 0.0 || frho != 0.0 || fpr != 0.0;
     PROF_BEGIN(PT_RHSB);
-    if (vd) {
+    if (h->rec && !vd && h->rb2) {
+        const int nyt = h->g.yhi - h->g.ylo + 1;
+        const int rb = h->rb2 == 2 ? 8 : 16;
+        dim3 grid((unsigned)blocks(h->NX, TX - 4), (unsigned)blocks(nyt, rb - 4), (unsigned)blocks(h->NZ - 4, RB_KC));
+        BRhsB2Args a = zeroed<BRhsB2Args>();
+        a.g = h->g; a.X = X; a.cur = h->cur; a.E = E; a.eta = eta; a.gam = c.gamma; a.XD = h->xdA;
+        rec_add(h, rb == 8 ? bl_rhs_b2<0, 8> : bl_rhs_b2<0, 16>, a, grid_words(h, grid));
+        rec_add(h, rb == 8 ? bl_rhs_b2<1, 8> : bl_rhs_b2<1, 16>, a, grid_words(h, grid));
+        BFillArgs f = zeroed<BFillArgs>();
+        f.g = h->g; f.XD = h->xdA; f.nf = 4;
+        rec_add(h, bl_fill, f, lin_words((unsigned)blocks(face_count(h->g), 256)));
+    } else if (h->rec && !vd) {
+        BRhsBArgs a = zeroed<BRhsBArgs>();
+        a.g = h->g; a.X = X; a.cur = h->cur; a.E = E; a.XH = XH; a.eta = eta; a.gam = c.gamma;
+        a.fmu = fmu; a.frho = frho; a.fpr = fpr; a.XD = h->xdA; a.kc = h->kc;
+        rec_add(h, bl_rhs_b, a, grid_words(h, tile_grid(h)));
+    } else if (vd) {
         hipLaunchKernelGGL((k_rhs_b<true>), tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->cur, E, XH, eta,
                            c.gamma, fmu, frho, fpr, h->xdA, h->kc);
     } else if (h->rb2) {
@@ -880,8 +1101,16 @@ static void dt_kernels(mhd_handle *h, int advance_time, double *part = nullptr)
     const mhd_config &c = h->cfg;
     double wcoef = (c.whistler_dt && c.hall && c.di != 0.0) ? 3.141592654 * c.di : 0.0;
     if (!part) part = h->red;
-    PROF_BEGIN(PT_DT);
     const double *rr = h->mag && c.magnetosphere ? h->rr : nullptr;
+    if (h->rec) {
+        BDtArgs a = zeroed<BDtArgs>();
+        a.g = h->g; a.X = h->x; a.rr = rr; a.eta = c.eta0; a.fmu = c.fmu0; a.cfl = c.cfl; a.gam = c.gamma;
+        a.wcoef = wcoef; a.part = part; a.dtv = h->dtv; a.nb = h->nred; a.advance_time = advance_time;
+        rec_add(h, bl_dt_partial, a, lin_words((unsigned)h->nred));
+        rec_add(h, bl_dt_final, a, lin_words(1));
+        return;
+    }
+    PROF_BEGIN(PT_DT);
     hipLaunchKernelGGL(k_dt_partial, dim3(h->nred), dim3(256), 0, h->stream, h->g, h->x, c.eta0, c.fmu0, c.cfl,
                        c.gamma, wcoef, rr, part);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, h->stream, part, h->nred, h->dtv, advance_time);
@@ -891,6 +1120,12 @@ static void dt_kernels(mhd_handle *h, int advance_time, double *part = nullptr)
 // the guarded loop's undo of a step after a collapse (see k_dt_final)
 static void restore_after_collapse(mhd_handle *h)
 {
+    if (h->rec) {
+        BRestoreArgs a = zeroed<BRestoreArgs>();
+        a.dtv = h->dtv; a.XF = h->xfold; a.X = h->x; a.n = 8 * h->S; a.nb = RESTORE_BLOCKS;
+        rec_add(h, bl_restore, a, lin_words(RESTORE_BLOCKS));
+        return;
+    }
     hipLaunchKernelGGL(k_restore, dim3(1024), dim3(256), 0, h->stream, h->dtv, h->xfold, h->x, 8 * h->S);
 }
 
@@ -1959,14 +2194,42 @@ struct BatchGraphs {
     hipGraphExec_t next = nullptr;
 };
 
+// The batched-kernel form: each sweep of a step pair is one launch over
+// the runs of a wave (a merged {run, block word} table and a table of the
+// runs' arguments), the wave's launches captured once into a graph on the
+// stream of its first run.  Runs whose launch sequences differ (Hall on /
+// off, face-window rows) go to different waves; a wave holds at most
+// wave_cells physical cells unless it is one run (0: no limit).  The waves
+// replay concurrently.  The argument and block tables live in device memory
+// held by the graphs.
+struct KWave {
+    std::vector<int> runs;
+    hipStream_t s = nullptr;
+    hipGraphExec_t first = nullptr;
+    hipGraphExec_t next = nullptr;
+};
+
+struct KernGraphs {
+    std::vector<double *> xs;              // the runs' x buffers when recorded
+    std::vector<KWave> waves;
+    std::vector<void *> mem;
+    size_t bytes = 0;                      // device bytes of the tables
+    int groups = 0, launches = 0;          // run groups, launches per step pair and wave
+};
+
+enum { FORM_RUNS = MHD_FORM_RUNS, FORM_BATCH_GRAPH = MHD_FORM_BATCH_GRAPH, FORM_KERNELS = MHD_FORM_KERNELS };
+
 struct mhd_ensemble {
     std::vector<mhd_handle *> runs;
     std::vector<std::pair<int, int>> chunks;   // each run's own (kc, zkc), restored at destroy
     std::vector<int> face12s;                  // and its face-window rows choice
+    std::vector<int> rb2s;                     // and its right-part-2 kernels
     int policy = 0;                            // z-chunk policy in force (ENS_POLICIES)
-    bool batch = false;                        // replay one graph of the whole batch
+    int form = FORM_RUNS;                      // replay form (FORM_*)
+    long wave_cells = 0;                       // batched kernels: cells per wave (0: no limit)
     std::vector<RunGraphs> graphs;         // two per run
     BatchGraphs bg[2];
+    KernGraphs kg[2];
     std::vector<int> status;
     std::vector<hipEvent_t> done;          // end of each run's work
     hipStream_t lead = nullptr;            // timing stream
@@ -1975,10 +2238,12 @@ struct mhd_ensemble {
 };
 
 // tile shape of a handle: z chunk lengths of its tile and smoothing
-// kernels and the rows of its face windows (the smoothing block tables are
-// rebuilt).  Results do not depend on them.
-static int set_shape(mhd_handle *h, int kc, int zkc, int face12)
+// kernels, the rows of its face windows (the smoothing block tables are
+// rebuilt) and the form of right part 2 (rb2 < 0 keeps the handle's).
+// Results do not depend on them.
+static int set_shape(mhd_handle *h, int kc, int zkc, int face12, int rb2 = -1)
 {
+    if (rb2 >= 0) h->rb2 = rb2;
     if (h->kc == kc && h->zkc == zkc && h->face12 == face12) return 0;
     HCHK(hipStreamSynchronize(h->stream));
     h->kc = kc;
@@ -2088,6 +2353,195 @@ static int batch_graphs(mhd_ensemble *e, BatchGraphs **out)
     return MHD_OK;
 }
 
+static void drop_kern(mhd_ensemble *e, KernGraphs &g)
+{
+    (void)e;
+    for (auto &w : g.waves) {
+        if (w.s && (w.first || w.next)) (void)hipStreamSynchronize(w.s);
+        if (w.first) (void)hipGraphExecDestroy(w.first);
+        if (w.next) (void)hipGraphExecDestroy(w.next);
+    }
+    for (void *p : g.mem) (void)hipFree(p);
+    g = KernGraphs{};
+}
+
+static void drop_all(mhd_ensemble *e)
+{
+    for (auto &g : e->graphs) drop_graphs(g);
+    for (auto &g : e->bg) drop_batch(g);
+    for (auto &g : e->kg) drop_kern(e, g);
+}
+
+// the launches of one step pair of run h, recorded instead of issued.  The
+// run's stream is put in capture mode meanwhile, so that a launch without a
+// batched form (viscosity, a non-default kernel switch) lands in a graph
+// that is never run; such a run is refused.
+static int record_pair(mhd_handle *h, int opening, std::vector<RecLaunch> &out)
+{
+    out.clear();
+    h->rec_err.clear();
+    HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    h->rec = &out;
+    dt_kernels(h, opening ? 0 : 1);
+    step_body(h);
+    restore_after_collapse(h);
+    dt_kernels(h, 1);
+    step_body(h);
+    restore_after_collapse(h);
+    h->rec = nullptr;
+    hipGraph_t gr = nullptr;
+    HCHK(hipStreamEndCapture(h->stream, &gr));
+    size_t nodes = 0;
+    const hipError_t st = hipGraphGetNodes(gr, nullptr, &nodes);
+    (void)hipGraphDestroy(gr);
+    HCHK(st);
+    if (nodes != 0) {
+        h->err = "a launch of this run's step has no batched form (viscosity or a non-default kernel switch)";
+        return MHD_ERR_ARG;
+    }
+    if (!h->rec_err.empty()) {
+        h->err = h->rec_err;
+        return MHD_ERR_ARG;
+    }
+    return MHD_OK;
+}
+
+// the batched-kernel graphs for the runs' current pointer states
+static int kern_graphs(mhd_ensemble *e, KernGraphs **out)
+{
+    const int n = (int)e->runs.size();
+    std::vector<double *> xs;
+    for (auto *h : e->runs) xs.push_back(h->x);
+    KernGraphs *slot = nullptr;
+    for (auto &g : e->kg) {
+        if (!g.waves.empty() && g.xs == xs) { *out = &g; return MHD_OK; }
+        if (!slot && g.waves.empty()) slot = &g;
+    }
+    if (!slot) { slot = &e->kg[0]; drop_kern(e, *slot); }
+    // record both pair kinds of every run
+    std::vector<std::vector<RecLaunch>> seq[2];
+    for (int o = 0; o < 2; o++) {
+        seq[o].resize(n);
+        for (int r = 0; r < n; r++) {
+            const int rc = record_pair(e->runs[r], o == 0 ? 1 : 0, seq[o][r]);
+            if (rc) { e->err = "run " + std::to_string(r) + ": " + e->runs[r]->err; return rc; }
+        }
+    }
+    // group the runs by launch sequence
+    std::vector<std::vector<int>> groups;
+    {
+        std::map<std::vector<BLaunch>, int> gid;
+        for (int r = 0; r < n; r++) {
+            std::vector<BLaunch> sig;
+            for (int o = 0; o < 2; o++)
+                for (const auto &l : seq[o][r]) sig.push_back(l.fn);
+            auto it = gid.find(sig);
+            if (it == gid.end()) {
+                gid.emplace(sig, (int)groups.size());
+                groups.push_back({r});
+            } else {
+                groups[it->second].push_back(r);
+            }
+        }
+    }
+    // waves: consecutive runs of a group up to the cell budget
+    KernGraphs &K = *slot;
+    for (const auto &grp : groups) {
+        long cells = 0;
+        bool open = false;
+        for (int r : grp) {
+            const mhd_handle *h = e->runs[r];
+            const long c = (long)h->cfg.mx * h->cfg.my * h->cfg.mz;
+            if (!open || (e->wave_cells > 0 && cells + c > e->wave_cells)) {
+                K.waves.emplace_back();
+                cells = 0;
+                open = true;
+            }
+            K.waves.back().runs.push_back(r);
+            cells += c;
+        }
+    }
+    for (auto &w : K.waves) w.s = e->runs[w.runs[0]]->stream;
+    // merged tables, uploaded once (identical tables shared)
+    std::map<std::string, void *> blobs;
+    hipError_t st = hipSuccess;
+    auto upload = [&](std::string &&bytes) -> void * {
+        auto it = blobs.find(bytes);
+        if (it != blobs.end()) return it->second;
+        void *d = nullptr;
+        if (st == hipSuccess) st = hipMalloc(&d, bytes.size());
+        if (st == hipSuccess) st = hipMemcpy(d, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
+        if (st != hipSuccess) { if (d) (void)hipFree(d); return nullptr; }
+        K.mem.push_back(d);
+        K.bytes += bytes.size();
+        blobs.emplace(std::move(bytes), d);
+        return d;
+    };
+    struct Plan {
+        BLaunch fn;
+        const void *args;
+        const BEntry *tab;
+        unsigned nblk;
+    };
+    std::vector<std::vector<Plan>> plan[2];
+    for (int o = 0; o < 2 && st == hipSuccess; o++)
+        for (const auto &wv : K.waves) {
+            const std::vector<int> &grp = wv.runs;
+            plan[o].emplace_back();
+            const size_t nl = seq[o][grp[0]].size();
+            for (size_t i = 0; i < nl && st == hipSuccess; i++) {
+                std::string args;
+                std::vector<BEntry> nat;
+                for (size_t k = 0; k < grp.size(); k++) {
+                    const RecLaunch &l = seq[o][grp[k]][i];
+                    args += l.arg;
+                    for (unsigned w : l.words) nat.push_back(BEntry{(unsigned)k, w});
+                }
+                if (nat.empty()) continue;
+                // dispatch position L runs on XCD L % 8: each XCD takes a
+                // contiguous range of the merged enumeration
+                const unsigned T = (unsigned)nat.size(), per = T / NXCD, rem = T % NXCD;
+                std::vector<BEntry> tab(T);
+                for (unsigned L = 0; L < T; L++) {
+                    const unsigned xcd = L % NXCD;
+                    tab[L] = nat[xcd * per + (xcd < rem ? xcd : rem) + L / NXCD];
+                }
+                void *da = upload(std::move(args));
+                void *dt = upload(std::string((const char *)tab.data(), T * sizeof(BEntry)));
+                if (!da || !dt) break;
+                plan[o].back().push_back(Plan{seq[o][grp[0]][i].fn, da, (const BEntry *)dt, T});
+            }
+        }
+    if (st != hipSuccess) {
+        e->err = std::string("batched tables: ") + hipGetErrorString(st);
+        drop_kern(e, K);
+        return MHD_ERR_NOMEM;
+    }
+    // capture each wave's pair on its stream
+    for (size_t w = 0; w < K.waves.size(); w++)
+        for (int o = 0; o < 2; o++) {
+            KWave &wv = K.waves[w];
+            hipGraph_t gr = nullptr;
+            st = hipStreamBeginCapture(wv.s, hipStreamCaptureModeThreadLocal);
+            if (st == hipSuccess) {
+                for (const Plan &p : plan[o][w]) p.fn(p.args, p.tab, p.nblk, wv.s);
+                st = hipStreamEndCapture(wv.s, &gr);
+            }
+            if (st == hipSuccess) st = hipGraphInstantiate(o == 0 ? &wv.first : &wv.next, gr, nullptr, nullptr, 0);
+            if (gr) (void)hipGraphDestroy(gr);
+            if (st != hipSuccess) {
+                e->err = std::string("batched capture: ") + hipGetErrorString(st);
+                drop_kern(e, K);
+                return MHD_ERR_HIP;
+            }
+        }
+    K.xs = xs;
+    K.groups = (int)groups.size();
+    K.launches = K.waves.empty() ? 0 : (int)plan[1][0].size();
+    *out = &K;
+    return MHD_OK;
+}
+
 // the graphs of run r for its current pointer state (captured when missing)
 static int run_graphs(mhd_ensemble *e, int r, RunGraphs **out)
 {
@@ -2124,8 +2578,7 @@ static std::pair<int, int> policy_chunks(const mhd_ensemble *e, int r, int p)
 
 static int apply_policy(mhd_ensemble *e, int p)
 {
-    for (auto &g : e->graphs) drop_graphs(g);   // captured with the old launch arguments
-    for (auto &g : e->bg) drop_batch(g);
+    drop_all(e);   // captured with the old launch arguments
     for (size_t r = 0; r < e->runs.size(); r++) {
         const auto c = policy_chunks(e, (int)r, p);
         const int rc = set_chunks(e->runs[r], c.first, c.second);
@@ -2164,6 +2617,7 @@ int mhd_ensemble_create(mhd_handle *const *runs, int n, mhd_ensemble **out)
     for (int r = 0; r < n; r++) {
         e->chunks.emplace_back(e->runs[r]->kc, e->runs[r]->zkc);
         e->face12s.push_back(e->runs[r]->face12);
+        e->rb2s.push_back(e->runs[r]->rb2);
     }
     return apply_policy(e, 0);
 }
@@ -2180,14 +2634,23 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
             e->err = "run " + std::to_string(r) + (h->prof ? ": profiling is on" : ": equilibrium not set");
             return MHD_ERR_ARG;
         }
-        if (nsteps >= 2 && !e->batch) {
+        if (nsteps >= 2 && e->form == FORM_RUNS) {
             const int rc = run_graphs(e, r, &gs[r]);
             if (rc) { e->err = "run " + std::to_string(r) + ": " + h->err; return rc; }
         }
     }
-    BatchGraphs *bgs = nullptr;
-    if (nsteps >= 2 && e->batch) {
+    // the whole-batch graph: one graph per step pair on the lead stream;
+    // the batched kernels: one graph per wave and step pair
+    hipGraphExec_t gfirst = nullptr, gnext = nullptr;
+    KernGraphs *kgs = nullptr;
+    if (nsteps >= 2 && e->form == FORM_BATCH_GRAPH) {
+        BatchGraphs *bgs = nullptr;
         const int rc = batch_graphs(e, &bgs);
+        if (rc) return rc;
+        gfirst = bgs->first;
+        gnext = bgs->next;
+    } else if (nsteps >= 2 && e->form == FORM_KERNELS) {
+        const int rc = kern_graphs(e, &kgs);
         if (rc) return rc;
     }
     auto hip_fail = [&](hipError_t st, const char *what) {
@@ -2207,14 +2670,27 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
         if ((st = hipStreamWaitEvent(e->runs[r]->stream, e->e0, 0)) != hipSuccess) return hip_fail(st, "hipStreamWaitEvent");
     // step pairs round-robin over the runs (or the batch graph on the lead
     // stream), then an odd last step eagerly
-    if (bgs) {
+    if (gfirst) {
         for (int p = 0; p < nsteps / 2; p++)
-            if ((st = hipGraphLaunch(p == 0 ? bgs->first : bgs->next, e->lead)) != hipSuccess)
+            if ((st = hipGraphLaunch(p == 0 ? gfirst : gnext, e->lead)) != hipSuccess)
                 return hip_fail(st, "hipGraphLaunch");
         if ((st = hipEventRecord(e->fork, e->lead)) != hipSuccess) return hip_fail(st, "hipEventRecord");
         for (int r = 0; r < n; r++)
             if ((st = hipStreamWaitEvent(e->runs[r]->stream, e->fork, 0)) != hipSuccess)
                 return hip_fail(st, "hipStreamWaitEvent");
+    } else if (kgs) {
+        for (int p = 0; p < nsteps / 2; p++)
+            for (const KWave &w : kgs->waves)
+                if ((st = hipGraphLaunch(p == 0 ? w.first : w.next, w.s)) != hipSuccess)
+                    return hip_fail(st, "hipGraphLaunch");
+        // the other runs of a wave continue after it (odd step, join)
+        for (const KWave &w : kgs->waves) {
+            const int r0 = w.runs[0];
+            if ((st = hipEventRecord(e->done[r0], w.s)) != hipSuccess) return hip_fail(st, "hipEventRecord");
+            for (size_t k = 1; k < w.runs.size(); k++)
+                if ((st = hipStreamWaitEvent(e->runs[w.runs[k]]->stream, e->done[r0], 0)) != hipSuccess)
+                    return hip_fail(st, "hipStreamWaitEvent");
+        }
     } else {
         for (int p = 0; p < nsteps / 2; p++)
             for (int r = 0; r < n; r++)
@@ -2301,15 +2777,18 @@ int mhd_ensemble_tune(mhd_ensemble *e, int trial, const double *dt0, const doubl
 
 int mhd_ensemble_set_shape(mhd_ensemble *e, const mhd_tile_shape *sh)
 {
-    if (!e || !sh || sh->kc < 1 || sh->zkc < 4 || sh->zkc > ZKC_MAX || (sh->face_rows != 12 && sh->face_rows != 16))
+    if (!e || !sh || sh->kc < 1 || sh->zkc < 4 || sh->zkc > ZKC_MAX || (sh->face_rows != 12 && sh->face_rows != 16) ||
+        sh->form < FORM_RUNS || sh->form > FORM_KERNELS || sh->rhs_rows < -1 || sh->rhs_rows > 2 ||
+        sh->wave_cells < 0)
         return MHD_ERR_ARG;
-    for (auto &g : e->graphs) drop_graphs(g);
-    for (auto &g : e->bg) drop_batch(g);
+    drop_all(e);
     for (size_t r = 0; r < e->runs.size(); r++) {
-        const int rc = set_shape(e->runs[r], sh->kc, sh->zkc, sh->face_rows == 12 ? 1 : 0);
+        const int rc = set_shape(e->runs[r], sh->kc, sh->zkc, sh->face_rows == 12 ? 1 : 0,
+                                 sh->rhs_rows < 0 ? e->rb2s[r] : sh->rhs_rows);
         if (rc) { e->err = "run " + std::to_string(r) + ": " + e->runs[r]->err; return rc; }
     }
-    e->batch = sh->batch_graph != 0;
+    e->form = sh->form;
+    e->wave_cells = sh->wave_cells;
     return MHD_OK;
 }
 
@@ -2334,13 +2813,14 @@ int mhd_ensemble_tune_shapes(mhd_ensemble *e, int trial, const double *dt0, cons
         }
     }
     auto apply = [&](const mhd_tile_shape &sh) {
-        for (auto &g : e->graphs) drop_graphs(g);
-        for (auto &g : e->bg) drop_batch(g);
+        drop_all(e);
         for (int r = 0; r < n; r++) {
-            const int rc = set_shape(e->runs[r], sh.kc, sh.zkc, sh.face_rows == 12 ? 1 : 0);
+            const int rc = set_shape(e->runs[r], sh.kc, sh.zkc, sh.face_rows == 12 ? 1 : 0,
+                                     sh.rhs_rows < 0 ? e->rb2s[r] : sh.rhs_rows);
             if (rc) { e->err = "run " + std::to_string(r) + ": " + e->runs[r]->err; return rc; }
         }
-        e->batch = sh.batch_graph != 0;
+        e->form = sh.form;
+        e->wave_cells = sh.wave_cells;
         return (int)MHD_OK;
     };
     int count = 0, rc = MHD_OK;
@@ -2363,12 +2843,29 @@ int mhd_ensemble_tune_shapes(mhd_ensemble *e, int trial, const double *dt0, cons
         }
         if (count < max_trials) trials[count] = sh;
         count++;
+        // runs the batched kernels cannot take (viscosity) rule that form out
+        if (r0 == MHD_ERR_ARG && sh.form == FORM_KERNELS) return (int)MHD_OK;
         return (r0 == MHD_OK || r0 == MHD_ERR_DT_COLLAPSE) ? (int)MHD_OK : r0;
     };
-    // coordinate search from the long-chunk shape: the smoothing chunk, the
-    // tile-kernel chunk, the face-window rows, then the graph form
-    mhd_tile_shape cur{32, 32, 12, 0, 0.0};
+    // coordinate search from the long-chunk shape: the replay form (per-run
+    // graphs, or batched kernels over all runs at once or in waves of 1/4,
+    // 1/8, 1/16 of the cells), the smoothing chunk, the tile-kernel chunk,
+    // the face-window rows, the right-part-2 kernels, then the other forms
+    long total = 0;
+    for (auto *h : e->runs) total += (long)h->cfg.mx * h->cfg.my * h->cfg.mz;
+    mhd_tile_shape cur{32, 32, 12, FORM_RUNS, -1, 0, 0.0};
     rc = measure(cur);
+    mhd_tile_shape kb = cur;
+    kb.ms = 1e30;
+    for (int d : {1, 4, 8, 16}) {
+        if (rc) break;
+        mhd_tile_shape s = cur;
+        s.form = FORM_KERNELS;
+        s.wave_cells = d == 1 ? 0 : (int)std::min<long>(INT_MAX, (total + d - 1) / d);
+        rc = measure(s);
+        if (!rc && s.ms < kb.ms) kb = s;
+    }
+    if (!rc && kb.ms < cur.ms) cur = kb;
     const int zkcs[] = {16, 24, 48, 64};
     const int kcs[] = {8, 16, 64};
     for (int v : zkcs) {
@@ -2391,9 +2888,20 @@ int mhd_ensemble_tune_shapes(mhd_ensemble *e, int trial, const double *dt0, cons
         rc = measure(s);
         if (!rc && s.ms < cur.ms) cur = s;
     }
-    if (!rc) {
+    // right part 2 in one kind for every run: the tile kernel, the 16-row
+    // and the 8-row windowed kernels
+    for (int v = 0; v <= 2 && !rc; v++) {
         mhd_tile_shape s = cur;
-        s.batch_graph = 1;
+        s.rhs_rows = v;
+        rc = measure(s);
+        if (!rc && s.ms < cur.ms) cur = s;
+    }
+    const mhd_tile_shape at = cur;
+    for (int f = FORM_RUNS; f <= FORM_KERNELS && !rc; f++) {
+        if (f == at.form) continue;
+        mhd_tile_shape s = at;
+        s.form = f;
+        if (f == FORM_KERNELS) s.wave_cells = kb.wave_cells;
         rc = measure(s);
         if (!rc && s.ms < cur.ms) cur = s;
     }
@@ -2418,10 +2926,9 @@ void mhd_ensemble_destroy(mhd_ensemble *e)
     if (!e) return;
     if (e->lead) (void)hipStreamSynchronize(e->lead);
     for (auto *h : e->runs) (void)hipStreamSynchronize(h->stream);
-    for (auto &g : e->graphs) drop_graphs(g);
-    for (auto &g : e->bg) drop_batch(g);
+    drop_all(e);
     for (size_t r = 0; r < e->chunks.size(); r++)
-        (void)set_shape(e->runs[r], e->chunks[r].first, e->chunks[r].second, e->face12s[r]);
+        (void)set_shape(e->runs[r], e->chunks[r].first, e->chunks[r].second, e->face12s[r], e->rb2s[r]);
     if (e->fork) (void)hipEventDestroy(e->fork);
     for (auto ev : e->done)
         if (ev) (void)hipEventDestroy(ev);

@@ -32,26 +32,27 @@ __device__ inline void zface_store(const Geo &g, double *__restrict__ B, long q,
 // ---------------------------------------------------------------------------
 // current (t3dsdsc.f:2465-2491) + bndryc: J = curl(B - B_i)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X, const double *__restrict__ bxi,
-                                             const double *__restrict__ byi, const double *__restrict__ bzi,
-                                             double *__restrict__ cur, int kc)
+__device__ __forceinline__ void curl_body(Geo g, const double *__restrict__ X, const double *__restrict__ bxi,
+                                          const double *__restrict__ byi, const double *__restrict__ bzi,
+                                          double *__restrict__ cur, int kc, BlockPos bp)
 {
     __shared__ double sp[3][TY + 4][TX + 4];
     __shared__ BCScratch<3> bs;
-    BlockPos bp = block_pos();
     Tile t = tile_of(g, kc, bp);
     if (t.kb >= t.ke) return;
-    const long S = g.S, sz = g.sz;
+    const long S = g.S;
+    const unsigned sz = (unsigned)g.sz;
     const bool ld = loadable(g, t.i, t.j);
-    const long q0 = ld ? (long)t.i + g.sy * rmap(g, t.j) : 0;
+    // element offsets (fields hold fewer than 2^29 points, checked at create)
+    const unsigned e0 = ld ? (unsigned)t.i + (unsigned)g.sy * (unsigned)rmap(g, t.j) : 0u;
     const double *b5 = X + 4 * S, *b6 = X + 5 * S, *b7 = X + 6 * S;
     double uq[5], vq[5];
     if (ld) {
         #pragma unroll
         for (int s = 0; s < 4; s++) {
-            long q = q0 + sz * (t.kb - 2 + s);
-            uq[s] = b5[q] - bxi[q];
-            vq[s] = b6[q] - byi[q];
+            const unsigned e = e0 + sz * (unsigned)(t.kb - 2 + s);
+            uq[s] = fld(b5, e) - fld(bxi, e);
+            vq[s] = fld(b6, e) - fld(byi, e);
         }
     }
     double ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
@@ -60,14 +61,15 @@ __global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X
         ay[0] = g.y1a[t.j]; ay[1] = g.y1b[t.j]; ay[2] = g.y1c[t.j]; ay[3] = g.y1d[t.j];
     }
     for (int k = t.kb; k < t.ke; k++) {
-        const long kz = sz * k;
+        const unsigned kz = sz * (unsigned)k;
         if (ld) {
-            long q = q0 + kz + 2 * sz;
-            uq[4] = b5[q] - bxi[q];
-            vq[4] = b6[q] - byi[q];
+            const unsigned e = e0 + kz + 2 * sz;
+            uq[4] = fld(b5, e) - fld(bxi, e);
+            vq[4] = fld(b6, e) - fld(byi, e);
         }
-        load_plane<2, 3>(g, t, sp, kz, [&](int m, long q) {
-            return m == 0 ? b5[q] - bxi[q] : m == 1 ? b6[q] - byi[q] : b7[q] - bzi[q];
+        load_plane<2, 3>(g, t, sp, (long)kz, [&](int m, long q) {
+            const unsigned e = (unsigned)q;
+            return m == 0 ? fld(b5, e) - fld(bxi, e) : m == 1 ? fld(b6, e) - fld(byi, e) : fld(b7, e) - fld(bzi, e);
         });
         __syncthreads();
         double v[3] = {0, 0, 0};
@@ -86,9 +88,9 @@ __global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X
         }
         plane_rules<3, BC_AUX>(g, t, bs, v);
         if (t.own) {
-            long q = q0 + kz;
-            st(cur + q, v[0]); st(cur + q + S, v[1]); st(cur + q + 2 * S, v[2]);
-            zface_store<3>(g, cur, q0, k, v);
+            const unsigned e = e0 + kz;
+            st(fldp(cur, e), v[0]); st(fldp(cur + S, e), v[1]); st(fldp(cur + 2 * S, e), v[2]);
+            zface_store<3>(g, cur, (long)e0, k, v);
         }
         __syncthreads();
         #pragma unroll
@@ -326,8 +328,8 @@ constexpr int RKS_ADD = -2;
 constexpr int ZPF = 3;            // planes loaded ahead
 
 template <int NP, int KIND, int TYPE, int RKS, int MODE, int RB = ZB>
-__global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE == ZM_FACE ? 4 : 8, 8))) void k_zsmooth(Geo g, const double *__restrict__ A, double *__restrict__ B,
-                                                     RKArgs rk, const unsigned *__restrict__ ztab, int nch, int kc)
+__device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restrict__ A, double *__restrict__ B,
+                                             const RKArgs &rk, int bx, int by, int chunk, int m, int kc)
 {
     static_assert(NP >= 1 && (NP <= 2 || KIND == KIND_EIGHTH), "the radial coefficient ring holds planes t-2..t");
     static_assert(NP <= 3 && (RKS != RKS_ADD || NP == 3), "passes per launch");
@@ -342,32 +344,6 @@ __global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE ==
     __shared__ double szw[2][NZW];             // z weights of the chunk's planes
     const int lane = threadIdx.x % TX;
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);   // uniform over the wavefront
-    // the block's tile, chunk and field: from the grid position (ZM_ALL), or
-    // from the host's block table by dispatch position (work-groups go to
-    // XCD id % 8)
-    int bx, by, chunk, m;
-    if (MODE == ZM_ALL) {
-        // the grid order (x tile fastest, then y tile, chunk, field) handed to
-        // the XCDs in contiguous ranges: neighbouring tiles of a field share
-        // halo rows in one L2
-        const unsigned gx = gridDim.x, gy = gridDim.y;
-        const unsigned T = gx * gy * gridDim.z;
-        const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-        const unsigned per = T / NXCD, rem = T % NXCD, xcd = L % NXCD;
-        unsigned V = xcd * per + (xcd < rem ? xcd : rem) + L / NXCD;
-        bx = __builtin_amdgcn_readfirstlane(V % gx);
-        V /= gx;
-        by = __builtin_amdgcn_readfirstlane(V % gy);
-        V /= gy;
-        chunk = __builtin_amdgcn_readfirstlane(V % (unsigned)nch);
-        m = __builtin_amdgcn_readfirstlane(V / (unsigned)nch);
-    } else {
-        const unsigned zw = ztab[blockIdx.x];
-        bx = zw & ((1u << ZT_BY) - 1);
-        by = (zw >> ZT_BY) & ((1u << (ZT_CH - ZT_BY)) - 1);
-        chunk = (zw >> ZT_CH) & ((1u << (ZT_F - ZT_CH)) - 1);
-        m = zw >> ZT_F;
-    }
     // columns: lane l holds column i0 + l
     const int nomx = bx * W;
     const int i0 = max(0, min(nomx - NP, g.NX - TX));
@@ -684,12 +660,13 @@ struct Rec {
     double x2, x3, x4, t22, t23, t24, t33, t34, t44, p;
 };
 
-__device__ inline Rec rec_at(const double *__restrict__ X, const double *__restrict__ pri, long S, long q)
+// e: element offset of the point (wave-uniform field bases, 32-bit offsets)
+__device__ inline Rec rec_at(const double *__restrict__ X, const double *__restrict__ pri, long S, unsigned e)
 {
     Rec r;
-    double x1 = X[q];
-    r.x2 = X[q + S]; r.x3 = X[q + 2 * S]; r.x4 = X[q + 3 * S]; r.p = X[q + 7 * S];
-    double dp = r.p - pri[q];
+    double x1 = fld(X, e);
+    r.x2 = fld(X + S, e); r.x3 = fld(X + 2 * S, e); r.x4 = fld(X + 3 * S, e); r.p = fld(X + 7 * S, e);
+    double dp = r.p - fld(pri, e);
     // the six quotients by rho from one rounded reciprocal (divc: equal to
     // the IEEE quotients)
     const double y = 1.0 / x1;
@@ -720,40 +697,43 @@ __device__ inline double rec_field(const Rec &r, int m)
 
 // E at point q from the momentum (x2, x3, x4) the caller already holds
 __device__ inline void e_point_m(const double *__restrict__ X, const double *__restrict__ cur, double eta, long S,
-                                 long q, double x2, double x3, double x4, double *__restrict__ E)
+                                 unsigned e, double x2, double x3, double x4, double *__restrict__ E)
 {
-    double x1 = X[q];
-    double x5 = X[q + 4 * S], x6 = X[q + 5 * S], x7 = X[q + 6 * S];
+    double x1 = fld(X, e);
+    double x5 = fld(X + 4 * S, e), x6 = fld(X + 5 * S, e), x7 = fld(X + 6 * S, e);
     const double y = 1.0 / x1;
-    st(E + q, divc(-x3 * x7 + x4 * x6, x1, y) + eta * cur[q]);
-    st(E + q + S, divc(-x4 * x5 + x2 * x7, x1, y) + eta * cur[q + S]);
-    st(E + q + 2 * S, divc(-x2 * x6 + x3 * x5, x1, y) + eta * cur[q + 2 * S]);
+    st(fldp(E, e), divc(-x3 * x7 + x4 * x6, x1, y) + eta * fld(cur, e));
+    st(fldp(E + S, e), divc(-x4 * x5 + x2 * x7, x1, y) + eta * fld(cur + S, e));
+    st(fldp(E + 2 * S, e), divc(-x2 * x6 + x3 * x5, x1, y) + eta * fld(cur + 2 * S, e));
 }
 
 __device__ inline void e_point(const double *__restrict__ X, const double *__restrict__ cur, double eta, long S,
-                               long q, double *__restrict__ E)
+                               unsigned e, double *__restrict__ E)
 {
-    e_point_m(X, cur, eta, S, q, X[q + S], X[q + 2 * S], X[q + 3 * S], E);
+    e_point_m(X, cur, eta, S, e, fld(X + S, e), fld(X + 2 * S, e), fld(X + 3 * S, e), E);
 }
 
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_rhs_a(Geo g, const double *__restrict__ X, const double *__restrict__ pri,
-                                              const double *__restrict__ cur, const double *__restrict__ xresd,
-                                              double eta, int hall, double cht, double *__restrict__ XD,
-                                              double *__restrict__ E, double *__restrict__ XH, int kc)
+// GT: Geo (a kernel argument, copied) or const Geo & (a run's block of a batch)
+template <class GT>
+__device__ __forceinline__ void rhs_a_body(GT g, const double *__restrict__ X, const double *__restrict__ pri,
+                                           const double *__restrict__ cur, const double *__restrict__ xresd,
+                                           double eta, int hall, double cht, double *__restrict__ XD,
+                                           double *__restrict__ E, double *__restrict__ XH, int kc, BlockPos bp)
 {
     __shared__ double sp[NLA][TY + 4][TX + 4];
     __shared__ BCScratch<3> bs;
-    BlockPos bp = block_pos();
     Tile t = tile_of(g, kc, bp);
     if (t.kb >= t.ke) return;
-    const long S = g.S, sz = g.sz;
+    const long S = g.S;
+    const unsigned sz = (unsigned)g.sz;
     const bool ld = loadable(g, t.i, t.j);
-    const long q0 = ld ? (long)t.i + g.sy * rmap(g, t.j) : 0;
+    // element offsets (fields hold fewer than 2^29 points, checked at create)
+    const unsigned e0 = ld ? (unsigned)t.i + (unsigned)g.sy * (unsigned)rmap(g, t.j) : 0u;
     if (t.own) {   // E on the planes outside the interior z range
-        if (t.kb == 2) { e_point(X, cur, eta, S, q0, E); e_point(X, cur, eta, S, q0 + sz, E); }
+        if (t.kb == 2) { e_point(X, cur, eta, S, e0, E); e_point(X, cur, eta, S, e0 + sz, E); }
         if (t.ke == g.NZ - 2) {
-            e_point(X, cur, eta, S, q0 + sz * (g.NZ - 2), E);
-            e_point(X, cur, eta, S, q0 + sz * (g.NZ - 1), E);
+            e_point(X, cur, eta, S, e0 + sz * (unsigned)(g.NZ - 2), E);
+            e_point(X, cur, eta, S, e0 + sz * (unsigned)(g.NZ - 1), E);
         }
     }
     double qx4[5], qt24[5], qt34[5], qt44[5], qp[5];
@@ -761,7 +741,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     if (ld) {
         #pragma unroll
         for (int s = 0; s < 4; s++) {
-            Rec r = rec_at(X, pri, S, q0 + sz * (t.kb - 2 + s));
+            Rec r = rec_at(X, pri, S, e0 + sz * (unsigned)(t.kb - 2 + s));
             qx4[s] = r.x4; qt24[s] = r.t24; qt34[s] = r.t34; qt44[s] = r.t44; qp[s] = r.p;
             if (s >= 2) {
                 yx2[s - 2] = r.x2; yx3[s - 2] = r.x3; yt22[s - 2] = r.t22; yt23[s - 2] = r.t23; yt33[s - 2] = r.t33;
@@ -774,10 +754,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         ay[0] = g.y1a[t.j]; ay[1] = g.y1b[t.j]; ay[2] = g.y1c[t.j]; ay[3] = g.y1d[t.j];
     }
     for (int k = t.kb; k < t.ke; k++) {
-        const long kz = sz * k;
-        const long qk = q0 + kz;
+        const unsigned kz = sz * (unsigned)k;
+        const unsigned ek = e0 + kz;
         if (ld) {
-            Rec r = rec_at(X, pri, S, qk + 2 * sz);
+            Rec r = rec_at(X, pri, S, ek + 2 * sz);
             qx4[4] = r.x4; qt24[4] = r.t24; qt34[4] = r.t34; qt44[4] = r.t44; qp[4] = r.p;
             yx2[2] = r.x2; yx3[2] = r.x3; yt22[2] = r.t22; yt23[2] = r.t23; yt33[2] = r.t33;
             const int y = t.ty + 2, x = t.tx + 2;
@@ -785,12 +765,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             sp[LT24][y][x] = qt24[2]; sp[LT33][y][x] = yt33[0]; sp[LT34][y][x] = qt34[2]; sp[LP][y][x] = qp[2];
         }
         // plane k's momentum is in the z queues (loaded two planes ago)
-        if (t.own) e_point_m(X, cur, eta, S, qk, yx2[0], yx3[0], qx4[2], E);
+        if (t.own) e_point_m(X, cur, eta, S, ek, yx2[0], yx3[0], qx4[2], E);
         // halo only (the own value is in place already)
         if (t.tx < 2 || t.tx >= TX - 2) {
             int hx = t.tx < 2 ? -2 : 2;
             if (loadable(g, t.i + hx, t.j)) {
-                Rec r = rec_at(X, pri, S, t.i + hx + g.sy * rmap(g, t.j) + kz);
+                Rec r = rec_at(X, pri, S, (unsigned)(t.i + hx) + (unsigned)g.sy * (unsigned)rmap(g, t.j) + kz);
                 #pragma unroll
                 for (int m = 0; m < NLA; m++) sp[m][t.ty + 2][t.tx + 2 + hx] = rec_field(r, m);
             }
@@ -798,7 +778,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         if (t.ty < 2 || t.ty >= TY - 2) {
             int hy = t.ty < 2 ? -2 : 2;
             if (loadable(g, t.i, t.j + hy)) {
-                Rec r = rec_at(X, pri, S, t.i + g.sy * rmap(g, t.j + hy) + kz);
+                Rec r = rec_at(X, pri, S, (unsigned)t.i + (unsigned)g.sy * (unsigned)rmap(g, t.j + hy) + kz);
                 #pragma unroll
                 for (int m = 0; m < NLA; m++) sp[m][t.ty + 2 + hy][t.tx + 2] = rec_field(r, m);
             }
@@ -815,12 +795,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             double xd1 = -DXL(LT22) - DYL(LT23) - DZQ(qt24);
             double xd2 = -DXL(LT23) - DYL(LT33) - DZQ(qt34);
             double xd3 = -DXL(LT24) - DYL(LT34) - DZQ(qt44);
-            double x1 = X[qk], x5 = X[qk + 4 * S], x6 = X[qk + 5 * S], x7 = X[qk + 6 * S];
-            double c1 = cur[qk], c2 = cur[qk + S], c3 = cur[qk + 2 * S];
-            xd1 = xd1 - xresd[qk] + c2 * x7 - c3 * x6;
-            xd2 = xd2 - xresd[qk + S] + c3 * x5 - c1 * x7;
-            xd3 = xd3 - xresd[qk + 2 * S] + c1 * x6 - c2 * x5;
-            st(XD + qk, xd0); st(XD + qk + S, xd1); st(XD + qk + 2 * S, xd2); st(XD + qk + 3 * S, xd3);
+            double x1 = fld(X, ek), x5 = fld(X + 4 * S, ek), x6 = fld(X + 5 * S, ek), x7 = fld(X + 6 * S, ek);
+            double c1 = fld(cur, ek), c2 = fld(cur + S, ek), c3 = fld(cur + 2 * S, ek);
+            xd1 = xd1 - fld(xresd, ek) + c2 * x7 - c3 * x6;
+            xd2 = xd2 - fld(xresd + S, ek) + c3 * x5 - c1 * x7;
+            xd3 = xd3 - fld(xresd + 2 * S, ek) + c1 * x6 - c2 * x5;
+            st(fldp(XD, ek), xd0); st(fldp(XD + S, ek), xd1); st(fldp(XD + 2 * S, ek), xd2);
+            st(fldp(XD + 3 * S, ek), xd3);
             if (hall) {
                 const double y = 1.0 / x1;
                 hv[0] = divc(cht * (c2 * x7 - c3 * x6 - DXL(LP)), x1, y);
@@ -834,8 +815,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         if (hall) {
             plane_rules<3, BC_AUX>(g, t, bs, hv);
             if (t.own) {
-                st(XH + qk, hv[0]); st(XH + qk + S, hv[1]); st(XH + qk + 2 * S, hv[2]);
-                zface_store<3>(g, XH, q0, k, hv);
+                st(fldp(XH, ek), hv[0]); st(fldp(XH + S, ek), hv[1]); st(fldp(XH + 2 * S, ek), hv[2]);
+                zface_store<3>(g, XH, (long)e0, k, hv);
             }
         }
         __syncthreads();
@@ -858,16 +839,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
 // ---------------------------------------------------------------------------
 enum { LE1 = 0, LE2, LE3, LPP, LVX, LVY, LVZ, LRH };
 
-template <bool VD>
-__global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ X, const double *__restrict__ cur,
-                                              const double *__restrict__ E, const double *__restrict__ XH,
-                                              double eta, double gam, double fmu, double frho, double fpr,
-                                              double *__restrict__ XD, int kc)
+template <bool VD, class GT = Geo>
+__device__ __forceinline__ void rhs_b_body(GT g, const double *__restrict__ X, const double *__restrict__ cur,
+                                           const double *__restrict__ E, const double *__restrict__ XH,
+                                           double eta, double gam, double fmu, double frho, double fpr,
+                                           double *__restrict__ XD, int kc, BlockPos bp)
 {
     constexpr int NL = VD ? 8 : 6;
     __shared__ double sp[NL][TY + 4][TX + 4];
     __shared__ BCScratch<8> bs;
-    BlockPos bp = block_pos();
     Tile t = tile_of(g, kc, bp);
     if (t.kb >= t.ke) return;
     const long S = g.S, sz = g.sz;
@@ -992,9 +972,9 @@ __global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ 
 constexpr int RB_KC = 32;        // interior planes per block
 
 template <int PART, int RB>
-__global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restrict__ X, const double *__restrict__ cur,
-                                                   const double *__restrict__ E, double eta, double gam,
-                                                   double *__restrict__ XD)
+__device__ __forceinline__ void rhs_b2_body(Geo g, const double *__restrict__ X, const double *__restrict__ cur,
+                                            const double *__restrict__ E, double eta, double gam,
+                                            double *__restrict__ XD, int bx, int by, int bz)
 {
     constexpr int W = TX - 4, H = RB - 4;
     constexpr int NO = PART == 0 ? 3 : 1;        // output fields
@@ -1002,12 +982,12 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
     __shared__ double sx[NO][2][TX];            // rows 2 / NY-3 of the outputs
     const int lane = threadIdx.x % TX;
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
-    const int nomx = blockIdx.x * W;
+    const int nomx = bx * W;
     const int i0 = max(0, min(nomx - 2, g.NX - TX));
     const int i = i0 + lane;
     const bool vx = i < g.NX;
     const bool ownx = vx && i >= nomx && i < nomx + W;
-    const int nomy = g.ylo + blockIdx.y * H;
+    const int nomy = g.ylo + by * H;
     const int j0 = g.periodic ? max(g.ylo - 2, min(nomy - 2, g.yhi + 3 - RB)) : max(0, min(nomy - 2, g.NY - RB));
     const int j = j0 + r;
     const bool vy = g.periodic || j < g.NY;
@@ -1031,7 +1011,7 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
     const double ax0 = pin ? g.x1a[i] : 0.0, ax1 = pin ? g.x1b[i] : 0.0, ax2 = pin ? g.x1c[i] : 0.0,
                  ax3 = pin ? g.x1d[i] : 0.0;
     const double ay0 = g.y1a[jp], ay1 = g.y1b[jp], ay2 = g.y1c[jp], ay3 = g.y1d[jp];   // row-uniform
-    const int kb = 2 + blockIdx.z * RB_KC, ke = min(kb + RB_KC, g.NZ - 2);
+    const int kb = 2 + bz * RB_KC, ke = min(kb + RB_KC, g.NZ - 2);
     const int t0 = kb - 2, t1 = ke + 1;
 #define DXS(f) fd5(sp[b][f][r][lm2], sp[b][f][r][lm1], sp[b][f][r][lane], sp[b][f][r][lp1], sp[b][f][r][lp2], ax0, ax1, ax2, ax3)
 #define DYS(f) fd5(sp[b][f][rm2][lane], sp[b][f][rm1][lane], sp[b][f][r][lane], sp[b][f][rp1][lane], sp[b][f][rp2][lane], ay0, ay1, ay2, ay3)
@@ -1149,6 +1129,14 @@ __global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restr
 #undef DXS
 #undef DYS
 #undef DZQ
+}
+
+template <int PART, int RB>
+__global__ __launch_bounds__(TX * RB) void k_rhs_b2(Geo g, const double *__restrict__ X, const double *__restrict__ cur,
+                                                   const double *__restrict__ E, double eta, double gam,
+                                                   double *__restrict__ XD)
+{
+    rhs_b2_body<PART, RB>(g, X, cur, E, eta, gam, XD, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 
@@ -1314,13 +1302,211 @@ __device__ inline double state_face(const Geo &g, const double *G, int i, int j,
     return v;
 }
 
-__global__ void k_fill_state(Geo g, double *__restrict__ XD, int nf)
+// block b of the face-point fill (256 points per block)
+__device__ __forceinline__ void fill_state_body(const Geo &g, double *__restrict__ XD, int nf, unsigned b)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = (long)b * blockDim.x + threadIdx.x;
     int i, j, k;
     if (!face_point(g, t, i, j, k)) return;
     const long q = IDX(g, i, j, k);
     for (int m = 0; m < nf; m++) XD[q + m * g.S] = state_face(g, XD + m * g.S, i, j, k);
+}
+
+__global__ void k_fill_state(Geo g, double *__restrict__ XD, int nf)
+{
+    fill_state_body(g, XD, nf, blockIdx.x);
+}
+
+
+// ---------------------------------------------------------------------------
+// launch forms of the tile and smoothing bodies.  One run: the block position
+// comes from the grid (XCD-contiguous ranges, block_pos) or from the run's
+// block table.  A batch of runs: each block reads {run, word} from a merged
+// block table and the run's arguments from a device table, so one launch
+// covers the same sweep of every run.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X, const double *__restrict__ bxi,
+                                             const double *__restrict__ byi, const double *__restrict__ bzi,
+                                             double *__restrict__ cur, int kc)
+{
+    curl_body(g, X, bxi, byi, bzi, cur, kc, block_pos());
+}
+
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_rhs_a(
+    Geo g, const double *__restrict__ X, const double *__restrict__ pri, const double *__restrict__ cur,
+    const double *__restrict__ xresd, double eta, int hall, double cht, double *__restrict__ XD,
+    double *__restrict__ E, double *__restrict__ XH, int kc)
+{
+    rhs_a_body<Geo>(g, X, pri, cur, xresd, eta, hall, cht, XD, E, XH, kc, block_pos());
+}
+
+template <bool VD>
+__global__ __launch_bounds__(NT) void k_rhs_b(Geo g, const double *__restrict__ X, const double *__restrict__ cur,
+                                              const double *__restrict__ E, const double *__restrict__ XH,
+                                              double eta, double gam, double fmu, double frho, double fpr,
+                                              double *__restrict__ XD, int kc)
+{
+    rhs_b_body<VD>(g, X, cur, E, XH, eta, gam, fmu, frho, fpr, XD, kc, block_pos());
+}
+
+// table word of a smoothing block: x tile | y tile << ZT_BY | chunk << ZT_CH | field << ZT_F
+__device__ inline void zs_decode(unsigned zw, int &bx, int &by, int &chunk, int &m)
+{
+    bx = __builtin_amdgcn_readfirstlane(zw & ((1u << ZT_BY) - 1));
+    by = __builtin_amdgcn_readfirstlane((zw >> ZT_BY) & ((1u << (ZT_CH - ZT_BY)) - 1));
+    chunk = __builtin_amdgcn_readfirstlane((zw >> ZT_CH) & ((1u << (ZT_F - ZT_CH)) - 1));
+    m = __builtin_amdgcn_readfirstlane(zw >> ZT_F);
+}
+
+template <int NP, int KIND, int TYPE, int RKS, int MODE, int RB = ZB>
+__global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE == ZM_FACE ? 4 : 8, 8))) void k_zsmooth(
+    Geo g, const double *__restrict__ A, double *__restrict__ B, RKArgs rk, const unsigned *__restrict__ ztab, int nch,
+    int kc)
+{
+    // the block's tile, chunk and field: from the grid position (ZM_ALL), or
+    // from the host's block table by dispatch position (work-groups go to
+    // XCD id % 8)
+    int bx, by, chunk, m;
+    if (MODE == ZM_ALL) {
+        // the grid order (x tile fastest, then y tile, chunk, field) handed to
+        // the XCDs in contiguous ranges: neighbouring tiles of a field share
+        // halo rows in one L2
+        const unsigned gx = gridDim.x, gy = gridDim.y;
+        const unsigned T = gx * gy * gridDim.z;
+        const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+        const unsigned per = T / NXCD, rem = T % NXCD, xcd = L % NXCD;
+        unsigned V = xcd * per + (xcd < rem ? xcd : rem) + L / NXCD;
+        bx = __builtin_amdgcn_readfirstlane(V % gx);
+        V /= gx;
+        by = __builtin_amdgcn_readfirstlane(V % gy);
+        V /= gy;
+        chunk = __builtin_amdgcn_readfirstlane(V % (unsigned)nch);
+        m = __builtin_amdgcn_readfirstlane(V / (unsigned)nch);
+    } else {
+        zs_decode(ztab[blockIdx.x], bx, by, chunk, m);
+    }
+    zsmooth_body<NP, KIND, TYPE, RKS, MODE, RB>(g, A, B, rk, bx, by, chunk, m, kc);
+}
+
+// ---- batched forms ----
+struct BEntry {
+    unsigned run;      // index into the launch's argument table
+    unsigned w;        // smoothing: table word; tile kernels: bx | by << ZT_BY | bz << ZT_CH
+};
+
+struct BZArgs {
+    Geo g;
+    const double *A;
+    double *B;
+    RKArgs rk;
+    int kc;
+};
+
+struct BCurlArgs {
+    Geo g;
+    const double *X, *bxi, *byi, *bzi;
+    double *cur;
+    int kc;
+};
+
+struct BRhsAArgs {
+    Geo g;
+    const double *X, *pri, *cur, *xresd;
+    double eta, cht;
+    int hall, kc;
+    double *XD, *E, *XH;
+};
+
+struct BRhsBArgs {
+    Geo g;
+    const double *X, *cur, *E, *XH;
+    double eta, gam, fmu, frho, fpr;
+    double *XD;
+    int kc;
+};
+
+// a run's argument block, read through the constant address space: the
+// blocks do not change during a launch, so the compiler may load their
+// fields again where needed instead of holding them in registers (as it
+// does for kernel arguments)
+template <class A>
+__device__ inline const A &run_args(const A *args, unsigned run)
+{
+    return *(const A *)((const __attribute__((address_space(4))) A *)args + run);
+}
+
+__device__ inline BlockPos tile_decode(unsigned w)
+{
+    BlockPos b;
+    b.bx = __builtin_amdgcn_readfirstlane(w & ((1u << ZT_BY) - 1));
+    b.by = __builtin_amdgcn_readfirstlane((w >> ZT_BY) & ((1u << (ZT_CH - ZT_BY)) - 1));
+    b.bz = __builtin_amdgcn_readfirstlane(w >> ZT_CH);
+    return b;
+}
+
+template <int NP, int KIND, int TYPE, int RKS, int MODE, int RB = ZB>
+__global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE == ZM_FACE ? 4 : 8, 8))) void kb_zsmooth(
+    const BZArgs *__restrict__ args, const BEntry *__restrict__ tab)
+{
+    const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
+    int bx, by, chunk, m;
+    zs_decode(tab[blockIdx.x].w, bx, by, chunk, m);
+    const BZArgs &a = run_args(args, run);
+    zsmooth_body<NP, KIND, TYPE, RKS, MODE, RB>(a.g, a.A, a.B, a.rk, bx, by, chunk, m, a.kc);
+}
+
+__global__ __launch_bounds__(NT) void kb_curl(const BCurlArgs *__restrict__ args, const BEntry *__restrict__ tab)
+{
+    const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
+    const BCurlArgs &a = run_args(args, run);
+    curl_body(a.g, a.X, a.bxi, a.byi, a.bzi, a.cur, a.kc, tile_decode(tab[blockIdx.x].w));
+}
+
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_rhs_a(
+    const BRhsAArgs *__restrict__ args, const BEntry *__restrict__ tab)
+{
+    const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
+    const BRhsAArgs &a = run_args(args, run);
+    rhs_a_body<const Geo &>(a.g, a.X, a.pri, a.cur, a.xresd, a.eta, a.hall, a.cht, a.XD, a.E, a.XH, a.kc,
+               tile_decode(tab[blockIdx.x].w));
+}
+
+__global__ __launch_bounds__(NT) void kb_rhs_b(const BRhsBArgs *__restrict__ args, const BEntry *__restrict__ tab)
+{
+    const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
+    const BRhsBArgs &a = run_args(args, run);
+    rhs_b_body<false, const Geo &>(a.g, a.X, a.cur, a.E, a.XH, a.eta, a.gam, a.fmu, a.frho, a.fpr, a.XD, a.kc,
+                      tile_decode(tab[blockIdx.x].w));
+}
+
+struct BRhsB2Args {
+    Geo g;
+    const double *X, *cur, *E;
+    double eta, gam;
+    double *XD;
+};
+
+template <int PART, int RB>
+__global__ __launch_bounds__(TX * RB) void kb_rhs_b2(const BRhsB2Args *__restrict__ args,
+                                                    const BEntry *__restrict__ tab)
+{
+    const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
+    const BRhsB2Args &a = run_args(args, run);
+    const BlockPos b = tile_decode(tab[blockIdx.x].w);
+    rhs_b2_body<PART, RB>(a.g, a.X, a.cur, a.E, a.eta, a.gam, a.XD, b.bx, b.by, b.bz);
+}
+
+struct BFillArgs {
+    Geo g;
+    double *XD;
+    int nf;
+};
+
+__global__ void kb_fill_state(const BFillArgs *__restrict__ args, const BEntry *__restrict__ tab)
+{
+    const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
+    const BFillArgs &a = run_args(args, run);
+    fill_state_body(a.g, a.XD, a.nf, tab[blockIdx.x].w);
 }
 
 } // namespace mhd

@@ -53,9 +53,13 @@ class MHDDiag(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+# replay forms of an ensemble's step pairs (mhd_tile_shape.form)
+FORM_RUNS, FORM_BATCH_GRAPH, FORM_KERNELS = 0, 1, 2
+
+
 class MHDTileShape(C.Structure):
-    _fields_ = [("kc", C.c_int), ("zkc", C.c_int), ("face_rows", C.c_int), ("batch_graph", C.c_int),
-                ("ms", C.c_double)]
+    _fields_ = [("kc", C.c_int), ("zkc", C.c_int), ("face_rows", C.c_int), ("form", C.c_int),
+                ("rhs_rows", C.c_int), ("wave_cells", C.c_int), ("ms", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -379,8 +383,14 @@ class MHDEnsemble:
             raise MHDError(rc, self.L.mhd_ensemble_error(self.e).decode())
         return best.as_dict(), [tr[i].as_dict() for i in range(min(nt.value, max_trials))]
 
-    def set_shape(self, kc: int, zkc: int, face_rows: int = 12, batch_graph: bool = False):
-        sh = MHDTileShape(kc=kc, zkc=zkc, face_rows=face_rows, batch_graph=int(batch_graph), ms=0.0)
+    def set_shape(self, kc: int, zkc: int, face_rows: int = 12, form: int = 0, rhs_rows: int = -1,
+                  wave_cells: int = 0):
+        """form: FORM_RUNS, FORM_BATCH_GRAPH or FORM_KERNELS; rhs_rows: right
+        part 2 as the tile kernel (0), 16-row (1) or 8-row (2) windows, or
+        each run's own (-1); wave_cells: most cells of one wave of batched
+        launches (0: no limit)"""
+        sh = MHDTileShape(kc=kc, zkc=zkc, face_rows=face_rows, form=int(form), rhs_rows=int(rhs_rows),
+                          wave_cells=int(wave_cells), ms=0.0)
         rc = self.L.mhd_ensemble_set_shape(self.e, C.byref(sh))
         if rc != MHD_OK:
             raise MHDError(rc, self.L.mhd_ensemble_error(self.e).decode() if self.e else "")

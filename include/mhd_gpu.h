@@ -185,19 +185,40 @@ void mhd_ensemble_destroy(mhd_ensemble *e);
 int mhd_ensemble_tune(mhd_ensemble *e, int trial, const double *dt0, const double *time, int *best,
                       double *ms);
 /* Tile-shape autotuner of the ensemble (no counterpart in the reference):
- * a coordinate search over the shape of every run's launches -- the z chunk
- * of the smoothing kernels (zkc 16..64 planes), of the tile kernels (kc
- * 8..64), the rows of the face windows (12 or 16) and the replay form (each
- * run's own graph on its stream, or one graph of the whole batch) -- timing
- * `trial` batch steps per shape (after two untimed steps that capture the
- * graphs) and restoring the states after each.  The fastest shape stays in
- * force.  trials[max_trials] (may be NULL when max_trials is 0) receives
- * every shape tried with its time, *ntrials their count, *best the choice. */
+ * a coordinate search over the shape of every run's launches -- the replay
+ * form, the z chunk of the smoothing kernels (zkc 16..64 planes), of the
+ * tile kernels (kc 8..64), the rows of the face windows (12 or 16), the
+ * right-part-2 kernels, then the other replay forms -- timing `trial` batch steps per shape (after two
+ * untimed steps that capture the graphs) and restoring the states after
+ * each.  The fastest shape stays in force.  trials[max_trials] (may be NULL
+ * when max_trials is 0) receives every shape tried with its time, *ntrials
+ * their count, *best the choice.
+ *
+ * Replay forms of a step pair (every form gives each run the results of
+ * mhd_advance on that run alone, bit for bit):
+ *   MHD_FORM_RUNS         each run's own captured graph on its own stream;
+ *   MHD_FORM_BATCH_GRAPH  one graph holding every run's pair, a stream each;
+ *   MHD_FORM_KERNELS      batched kernels: each sweep of the pair is one
+ *                         launch over the blocks of the runs of a wave (a
+ *                         merged block table and a table of the runs'
+ *                         arguments), a wave's launches captured into one
+ *                         graph, the waves replayed concurrently on their own
+ *                         streams.  A wave holds runs with the same launch
+ *                         sequence up to wave_cells cells.  Runs with
+ *                         viscosity (fmu0, frho0, fpr0) have no batched
+ *                         form: mhd_ensemble_advance returns MHD_ERR_ARG. */
+#define MHD_FORM_RUNS 0
+#define MHD_FORM_BATCH_GRAPH 1
+#define MHD_FORM_KERNELS 2
 typedef struct mhd_tile_shape {
     int kc;           /* z planes per chunk of the tile kernels (curl, RHS)    */
     int zkc;          /* interior z planes per chunk of the smoothing kernels  */
     int face_rows;    /* rows of a face window's block (12 or 16)              */
-    int batch_graph;  /* 1: one graph of the whole batch per step pair         */
+    int form;         /* replay form, MHD_FORM_*                               */
+    int rhs_rows;     /* right part 2: 0 the tile kernel, 1 windows of 16     */
+                      /* rows, 2 windows of 8 rows, -1 each run's own choice  */
+    int wave_cells;   /* MHD_FORM_KERNELS: most physical cells of the runs    */
+                      /* one wave of batched launches covers (0: no limit)    */
     double ms;        /* the trial's time                                      */
 } mhd_tile_shape;
 int mhd_ensemble_tune_shapes(mhd_ensemble *e, int trial, const double *dt0, const double *time, int max_trials,

@@ -59,6 +59,11 @@ _C_LAYOUT = r"""
 #define F(T, m) printf(#T " " #m " %zu\n", offsetof(T, m))
 int main(void) {
   printf("mhd_config size %zu\nmhd_diag size %zu\n", sizeof(mhd_config), sizeof(mhd_diag));
+  printf("mhd_tile_shape size %zu\n", sizeof(mhd_tile_shape));
+  F(mhd_tile_shape, kc); F(mhd_tile_shape, zkc); F(mhd_tile_shape, face_rows); F(mhd_tile_shape, form);
+  F(mhd_tile_shape, rhs_rows); F(mhd_tile_shape, wave_cells); F(mhd_tile_shape, ms);
+  printf("form runs %d\nform batch_graph %d\nform kernels %d\n", MHD_FORM_RUNS, MHD_FORM_BATCH_GRAPH,
+         MHD_FORM_KERNELS);
   F(mhd_config, mx); F(mhd_config, my); F(mhd_config, mz); F(mhd_config, periodic_y);
   F(mhd_config, hall); F(mhd_config, whistler_dt); F(mhd_config, di); F(mhd_config, eta0);
   F(mhd_config, fmu0); F(mhd_config, frho0); F(mhd_config, fpr0); F(mhd_config, gamma);
@@ -89,6 +94,11 @@ def test_struct_layout_c_vs_ctypes(tmp_path):
         assert got[("mhd_config", name)] == getattr(mhd.MHDConfig, name).offset, name
     for name, _ in mhd.MHDDiag._fields_:
         assert got[("mhd_diag", name)] == getattr(mhd.MHDDiag, name).offset, name
+    assert got[("mhd_tile_shape", "size")] == C.sizeof(mhd.MHDTileShape)
+    for name, _ in mhd.MHDTileShape._fields_:
+        assert got[("mhd_tile_shape", name)] == getattr(mhd.MHDTileShape, name).offset, name
+    assert (got[("form", "runs")], got[("form", "batch_graph")], got[("form", "kernels")]) == \
+        (mhd.FORM_RUNS, mhd.FORM_BATCH_GRAPH, mhd.FORM_KERNELS)
 
 
 _F_LAYOUT = """

@@ -152,14 +152,31 @@ def test_member_close_waits_for_the_ensemble():
     assert not members[1].h
 
 
-@pytest.mark.parametrize("shape", [dict(kc=8, zkc=16, face_rows=16, batch_graph=True),
-                                   dict(kc=64, zkc=48, face_rows=12, batch_graph=False),
-                                   dict(kc=16, zkc=24, face_rows=12, batch_graph=True)])
-def test_shapes_and_batch_graph_match_single_runs(shape):
-    """any tile shape and the whole-batch graph give the runs alone, bit for
-    bit (pairs from both pointer states and an odd step)"""
+@pytest.mark.parametrize("shape", [dict(kc=8, zkc=16, face_rows=16, form=1),
+                                   dict(kc=64, zkc=48, face_rows=12, form=0, rhs_rows=1),
+                                   dict(kc=16, zkc=24, face_rows=12, form=1, rhs_rows=2),
+                                   dict(kc=32, zkc=32, face_rows=12, form=2),
+                                   dict(kc=8, zkc=64, face_rows=16, form=2, rhs_rows=1),
+                                   dict(kc=16, zkc=16, face_rows=12, form=2, rhs_rows=2, more=True),
+                                   dict(kc=16, zkc=32, face_rows=12, form=2, wave_cells=5000, more=True),
+                                   dict(kc=32, zkc=24, face_rows=12, form=2, wave_cells=9000, more=True)])
+def test_shapes_and_forms_match_single_runs(shape):
+    """any tile shape and every replay form (per-run graphs, the
+    whole-batch graph, the batched kernels) give the runs alone, bit for bit
+    (pairs from both pointer states and an odd step).  The runs alone use
+    the tile kernel for right part 2, so the windowed kernels (rhs_rows 1,
+    2) must give its values exactly.  The three runs differ
+    in their launch sequences (Hall off in one, non-periodic y with 16-row
+    face windows in another), so the batched kernels run three groups; with
+    ``more`` two runs of each kind share their groups, and a wave budget
+    (wave_cells) splits them into waves of one run or keeps the pairs."""
     from dasha_amd.mhd import MHDEnsemble, MHDStep
+    shape = dict(shape)
     cases = _cases()
+    if shape.pop("more", False):
+        cases += [harris.case_small(28, 8, 20, seed=24, di=0.2),
+                  harris.case_small(16, 8, 24, stretched=False, seed=25, di=0.0),
+                  harris.case_small(20, 12, 24, periodic_y=False, seed=26, di=0.1)]
     print(f"[ensemble] shape {shape}", flush=True)
     singles = [MHDStep(c) for c in cases]
     members = [MHDStep(c) for c in cases]
@@ -189,8 +206,32 @@ def test_shape_tuner_keeps_states():
     ens = MHDEnsemble(members)
     best, trials = ens.tune_shapes(2, np.zeros(len(cases)), dt0)
     print(f"[ensemble] tuned shape {best}; {len(trials)} trials", flush=True)
-    assert len(trials) == 10 and all(t["ms"] > 0 for t in trials)
+    assert len(trials) == 18 and all(t["ms"] > 0 for t in trials)
+    assert sorted({t["form"] for t in trials}) == [0, 1, 2]
     assert best["ms"] == min(t["ms"] for t in trials)
     for g, x in zip(members, x_before):
         assert np.array_equal(g.get_state(), x)
+    ens.close()
+
+
+def test_batched_kernels_refuse_viscosity():
+    """a run with viscosity has no batched form: the kernels form refuses
+    the ensemble and leaves the states alone; the per-run form still runs it"""
+    from dasha_amd.mhd import FORM_KERNELS, FORM_RUNS, MHDEnsemble, MHDError, MHDStep
+    cases = _cases()[:2]
+    cases[1].fmu0 = 1e-3
+    members = [MHDStep(c) for c in cases]
+    for g in members:
+        g.init_residual()
+    dt0 = [g.compute_dt() for g in members]
+    x_before = [g.get_state() for g in members]
+    ens = MHDEnsemble(members)
+    ens.set_shape(32, 32, 12, FORM_KERNELS)
+    with pytest.raises(MHDError):
+        ens.advance(2, np.zeros(2), dt0)
+    for g, x in zip(members, x_before):
+        assert np.array_equal(g.get_state(), x)
+    ens.set_shape(32, 32, 12, FORM_RUNS)
+    t, _ = ens.advance(2, np.zeros(2), dt0)
+    assert all(v > 0 for v in t)
     ens.close()
