@@ -427,6 +427,16 @@ struct mhd_handle {
     int NX, NY, NZ;
     long S, N;
     hipStream_t stream = nullptr;
+    // second stream for launches independent of the main stream's current
+    // one (the face-window launch of a smoothing sweep beside its plain
+    // launch), forked from and joined back into the main stream by events
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // and a second such pair for the first Hall smoothing passes, which run
+    // beside the smoothing of E
+    hipStream_t aux = nullptr, aux_side = nullptr;
+    hipEvent_t aux_ev = nullptr;
+    int overlap = 1;
     Geo g;
     double *geo = nullptr;                 // all 1-D device arrays
     double *x = nullptr, *xfold = nullptr, *xm = nullptr, *xdA = nullptr, *xdB = nullptr;
@@ -822,10 +832,25 @@ static void bl_restore(const void *a, const BEntry *t, unsigned n, hipStream_t s
 // blocks per run of the batched collapse undo (it copies only after a collapse)
 static constexpr unsigned RESTORE_BLOCKS = 128;
 
+// the side stream starts after the main stream's work so far (fork) and the
+// main stream continues after the side stream's (join)
+static void fork_side(mhd_handle *h)
+{
+    HCHK_V(hipEventRecord(h->fork_ev, h->stream));
+    HCHK_V(hipStreamWaitEvent(h->side, h->fork_ev, 0));
+}
+
+static void join_side(mhd_handle *h)
+{
+    HCHK_V(hipEventRecord(h->join_ev, h->side));
+    HCHK_V(hipStreamWaitEvent(h->stream, h->join_ev, 0));
+}
+
 template <int NP, int KIND, int TYPE, int RKS, int MODE>
 static void launch_zs(mhd_handle *h, dim3 grid, const ZTab *zt, int nch, const double *a, double *b,
-                      const RKArgs &rk)
+                      const RKArgs &rk, hipStream_t st = nullptr)
 {
+    if (!st) st = h->stream;
     const unsigned *tab = zt ? zt->d : nullptr;
     if (h->rec) {
         BZArgs z = zeroed<BZArgs>();
@@ -851,13 +876,13 @@ static void launch_zs(mhd_handle *h, dim3 grid, const ZTab *zt, int nch, const d
     }
     if constexpr (MODE == ZM_FACE) {
         if (face_rows(h) == 12) {
-            hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, MODE, 12>), grid, dim3(TX * 12), 0, h->stream, h->g, a,
-                               b, rk, tab, nch, h->zkc);
+            hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, MODE, 12>), grid, dim3(TX * 12), 0, st, h->g, a, b, rk,
+                               tab, nch, h->zkc);
             return;
         }
     }
-    hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, MODE>), grid, dim3(TX * ZB), 0, h->stream, h->g, a, b, rk, tab,
-                       nch, h->zkc);
+    hipLaunchKernelGGL((k_zsmooth<NP, KIND, TYPE, RKS, MODE>), grid, dim3(TX * ZB), 0, st, h->g, a, b, rk, tab, nch,
+                       h->zkc);
 }
 
 // NP smoothing passes of NF fields a -> b.  The RK stage sweeps (two more
@@ -877,6 +902,14 @@ static void launch_zsmooth(mhd_handle *h, const double *a, double *b, const doub
         launch_zs<NP, KIND, TYPE, RKS, ZM_ALL>(h, grid, nullptr, nch, a, b, rk);
     } else {
         const ZTab *zt = NP == 3 ? h->zt3 : h->zt[NF == 8 ? 1 : 0];
+        if (zt[0].n && zt[1].n && h->overlap && !h->rec) {
+            // the face windows on the side stream beside the plain ones
+            fork_side(h);
+            launch_zs<NP, KIND, TYPE, RKS, ZM_FACE>(h, dim3(zt[1].n), &zt[1], 0, a, b, rk, h->side);
+            launch_zs<NP, KIND, TYPE, RKS, ZM_PLAIN>(h, dim3(zt[0].n), &zt[0], 0, a, b, rk);
+            join_side(h);
+            return;
+        }
         if (zt[0].n) launch_zs<NP, KIND, TYPE, RKS, ZM_PLAIN>(h, dim3(zt[0].n), &zt[0], 0, a, b, rk);
         if (zt[1].n) launch_zs<NP, KIND, TYPE, RKS, ZM_FACE>(h, dim3(zt[1].n), &zt[1], 0, a, b, rk);
     }
@@ -960,10 +993,28 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
                            h->xresd, eta, c.hall, c.di, h->xdA, h->eA, h->hA, h->kc);
         PROF_END(PT_RHSA);
     }
-    double *E = smooth_aux(h, h->eA, h->eB, KIND_RADIAL, 4);
     // with the Hall term the last avrgh sweep leaves E + xdh, the field whose
     // curl enters the induction equation (t3dsdsc.f:1426-1433)
-    if (c.hall) E = smooth_aux(h, h->hA, h->hB, KIND_EIGHTH, 5, E);
+    double *E;
+    if (c.hall && h->overlap && h->zs && h->zs3 && !h->rec && !h->prof) {
+        // the first two avrgh passes need no E: they run on the second
+        // stream pair (swapped in as the handle's streams for the launch)
+        // while E is smoothed, and the last three add E on the main stream
+        HCHK_V(hipEventRecord(h->aux_ev, h->stream));
+        HCHK_V(hipStreamWaitEvent(h->aux, h->aux_ev, 0));
+        std::swap(h->stream, h->aux);
+        std::swap(h->side, h->aux_side);
+        launch_zsmooth<3, 2, KIND_EIGHTH, BC_AUX, -1>(h, h->hA, h->hB);
+        std::swap(h->stream, h->aux);
+        std::swap(h->side, h->aux_side);
+        E = smooth_aux(h, h->eA, h->eB, KIND_RADIAL, 4);
+        HCHK_V(hipEventRecord(h->aux_ev, h->aux));
+        HCHK_V(hipStreamWaitEvent(h->stream, h->aux_ev, 0));
+        E = smooth_aux(h, h->hB, h->hA, KIND_EIGHTH, 3, E);
+    } else {
+        E = smooth_aux(h, h->eA, h->eB, KIND_RADIAL, 4);
+        if (c.hall) E = smooth_aux(h, h->hA, h->hB, KIND_EIGHTH, 5, E);
+    }
     const double *XH = nullptr;
     bool vd = fmu != 0.0 || frho != 0.0 || fpr != 0.0;
     PROF_BEGIN(PT_RHSB);
@@ -991,14 +1042,21 @@ static void rhs(mhd_handle *h, const double *X, bool smooth_xdif, bool init, int
         const int nyt = h->g.yhi - h->g.ylo + 1;
         const int rb = h->rb2 == 2 ? 8 : 16;
         dim3 grid((unsigned)blocks(h->NX, TX - 4), (unsigned)blocks(nyt, rb - 4), (unsigned)blocks(h->NZ - 4, RB_KC));
+        // the two parts write different fields: part 0 on the side stream
+        hipStream_t s0 = h->stream;
+        if (h->overlap) {
+            fork_side(h);
+            s0 = h->side;
+        }
         if (rb == 8) {
-            hipLaunchKernelGGL((k_rhs_b2<0, 8>), grid, dim3(TX * 8), 0, h->stream, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
+            hipLaunchKernelGGL((k_rhs_b2<0, 8>), grid, dim3(TX * 8), 0, s0, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
             hipLaunchKernelGGL((k_rhs_b2<1, 8>), grid, dim3(TX * 8), 0, h->stream, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
         } else {
-            hipLaunchKernelGGL((k_rhs_b2<0, 16>), grid, dim3(TX * 16), 0, h->stream, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
+            hipLaunchKernelGGL((k_rhs_b2<0, 16>), grid, dim3(TX * 16), 0, s0, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
             hipLaunchKernelGGL((k_rhs_b2<1, 16>), grid, dim3(TX * 16), 0, h->stream, h->g, X, h->cur, E, eta, c.gamma, h->xdA);
         }
         hipLaunchKernelGGL(k_fill_state, dim3(blocks(face_count(h->g), 256)), dim3(256), 0, h->stream, h->g, h->xdA, 4);
+        if (h->overlap) join_side(h);
     } else {
         hipLaunchKernelGGL((k_rhs_b<false>), tile_grid(h), dim3(NT), 0, h->stream, h->g, X, h->cur, E, XH, eta,
                            c.gamma, fmu, frho, fpr, h->xdA, h->kc);
@@ -1516,6 +1574,12 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
         if (r) { *out = h; return r; }
     }
     HCHK(hipStreamCreate(&h->stream));
+    HCHK(hipStreamCreate(&h->side));
+    HCHK(hipStreamCreate(&h->aux));
+    HCHK(hipStreamCreate(&h->aux_side));
+    HCHK(hipEventCreateWithFlags(&h->aux_ev, hipEventDisableTiming));
+    HCHK(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+    HCHK(hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming));
 
     // ---- 1-D geometry on the host (ext-indexed) ----
     int NX = h->NX, NY = h->NY, NZ = h->NZ;
@@ -1697,6 +1761,7 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     if (const char *e = getenv("MHD_ZORD")) h->zord = std::min(1, std::max(0, atoi(e)));
     if (const char *e = getenv("MHD_ZS3")) h->zs3 = atoi(e);
     if (const char *e = getenv("MHD_FACE12")) h->face12 = atoi(e);
+    if (const char *e = getenv("MHD_OVERLAP")) h->overlap = atoi(e);
     for (int c = 0; c < 2; c++) {
         int r = zs_tables(h, c ? 8 : 3, 2, h->zt[c]);
         if (r) { *out = h; return r; }
@@ -2267,9 +2332,21 @@ static int set_shape(mhd_handle *h, int kc, int zkc, int face12, int rb2 = -1)
 
 static int set_chunks(mhd_handle *h, int kc, int zkc) { return set_shape(h, kc, zkc, h->face12); }
 
+// The ensemble keeps every launch of a run on the run's one stream: the
+// side-stream launches (overlap) stay with the single-run loop (a 48-run
+// ensemble of per-run graphs with forked branches ended in a host-side
+// fault inside the runtime).
+struct NoOverlap {
+    mhd_handle *h;
+    int saved;
+    explicit NoOverlap(mhd_handle *hh) : h(hh), saved(hh->overlap) { h->overlap = 0; }
+    ~NoOverlap() { h->overlap = saved; }
+};
+
 static int capture_pair(mhd_handle *h, int opening, hipGraphExec_t *out)
 {
     hipGraph_t gr = nullptr;
+    NoOverlap single(h);
     HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     dt_kernels(h, opening ? 0 : 1);
     step_body(h);
@@ -2316,6 +2393,7 @@ static int capture_batch(mhd_ensemble *e, int opening, hipGraphExec_t *out)
         mhd_handle *h = e->runs[r];
         st = hipStreamWaitEvent(h->stream, e->fork, 0);
         if (st != hipSuccess) break;
+        NoOverlap single(h);
         dt_kernels(h, opening ? 0 : 1);
         step_body(h);
         restore_after_collapse(h);
@@ -2699,6 +2777,7 @@ int mhd_ensemble_advance(mhd_ensemble *e, int nsteps, const double *dt0, double 
     }
     if (nsteps % 2)
         for (int r = 0; r < n; r++) {
+            NoOverlap single(e->runs[r]);
             dt_kernels(e->runs[r], nsteps > 1 ? 1 : 0);
             step_body(e->runs[r]);
             restore_after_collapse(e->runs[r]);
@@ -2942,6 +3021,9 @@ void mhd_destroy(mhd_handle *h)
 {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->side) hipStreamSynchronize(h->side);
+    if (h->aux) hipStreamSynchronize(h->aux);
+    if (h->aux_side) hipStreamSynchronize(h->aux_side);
     double *ps[] = {h->geo, h->x, h->xfold, h->xm, h->xdA, h->xdB, h->eq, h->xresd, h->cA, h->cB,
                     h->eA, h->eB, h->hA, h->hB, h->red, h->dtv, h->crad, h->rr, h->chtab, h->vel, h->mgeo};
     for (auto p : ps)
@@ -2953,6 +3035,12 @@ void mhd_destroy(mhd_handle *h)
         if (t.d) hipFree(t.d);
     for (auto e : h->pool) (void)hipEventDestroy(e);
     if (h->stage) (void)hipHostFree(h->stage);
+    if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+    if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+    if (h->side) (void)hipStreamDestroy(h->side);
+    if (h->aux_ev) (void)hipEventDestroy(h->aux_ev);
+    if (h->aux) (void)hipStreamDestroy(h->aux);
+    if (h->aux_side) (void)hipStreamDestroy(h->aux_side);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
