@@ -31,12 +31,17 @@ __device__ inline void zface_store(const Geo &g, double *__restrict__ B, long q,
 
 // ---------------------------------------------------------------------------
 // current (t3dsdsc.f:2465-2491) + bndryc: J = curl(B - B_i)
+// Iteration t loads plane t of B - B_i into LDS (halo 2), forms the x/y
+// derivative parts of plane t and pushes the column's bx - bxi, by - byi
+// into the z queues; plane t-2 is completed from the queues.  Every value of
+// B and B_i is loaded once per plane (planes outside the chunk: the column
+// only), and the sums keep the one-pass order of operations.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void curl_body(Geo g, const double *__restrict__ X, const double *__restrict__ bxi,
                                           const double *__restrict__ byi, const double *__restrict__ bzi,
                                           double *__restrict__ cur, int kc, BlockPos bp)
 {
-    __shared__ double sp[3][TY + 4][TX + 4];
+    __shared__ double sp[2][3][TY + 4][TX + 4];   // plane t by parity
     __shared__ BCScratch<3> bs;
     Tile t = tile_of(g, kc, bp);
     if (t.kb >= t.ke) return;
@@ -46,55 +51,66 @@ __device__ __forceinline__ void curl_body(Geo g, const double *__restrict__ X, c
     // element offsets (fields hold fewer than 2^29 points, checked at create)
     const unsigned e0 = ld ? (unsigned)t.i + (unsigned)g.sy * (unsigned)rmap(g, t.j) : 0u;
     const double *b5 = X + 4 * S, *b6 = X + 5 * S, *b7 = X + 6 * S;
-    double uq[5], vq[5];
-    if (ld) {
-        #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const unsigned e = e0 + sz * (unsigned)(t.kb - 2 + s);
-            uq[s] = fld(b5, e) - fld(bxi, e);
-            vq[s] = fld(b6, e) - fld(byi, e);
-        }
-    }
+    double uq[5] = {0, 0, 0, 0, 0}, vq[5] = {0, 0, 0, 0, 0};   // planes t-4..t
+    double pa[3] = {0, 0, 0}, pb[3] = {0, 0, 0}, pc[3] = {0, 0, 0};   // planes t-2..t
     double ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
     if (t.inner) {
         ax[0] = g.x1a[t.i]; ax[1] = g.x1b[t.i]; ax[2] = g.x1c[t.i]; ax[3] = g.x1d[t.i];
         ay[0] = g.y1a[t.j]; ay[1] = g.y1b[t.j]; ay[2] = g.y1c[t.j]; ay[3] = g.y1d[t.j];
     }
-    for (int k = t.kb; k < t.ke; k++) {
-        const unsigned kz = sz * (unsigned)k;
-        if (ld) {
-            const unsigned e = e0 + kz + 2 * sz;
-            uq[4] = fld(b5, e) - fld(bxi, e);
-            vq[4] = fld(b6, e) - fld(byi, e);
-        }
-        load_plane<2, 3>(g, t, sp, (long)kz, [&](int m, long q) {
-            const unsigned e = (unsigned)q;
-            return m == 0 ? fld(b5, e) - fld(bxi, e) : m == 1 ? fld(b6, e) - fld(byi, e) : fld(b7, e) - fld(bzi, e);
-        });
-        __syncthreads();
-        double v[3] = {0, 0, 0};
-        if (t.inner) {
+    for (int tp = t.kb - 2; tp < t.ke + 2; tp++) {
+        const unsigned kz = sz * (unsigned)tp;
+        double u = 0.0, w = 0.0, xa = 0.0, xb = 0.0, xc = 0.0;
+        if (tp >= t.kb && tp < t.ke) {
+            // block-uniform: one barrier per plane (the LDS planes alternate)
+            double (*sq)[TY + 4][TX + 4] = sp[tp & 1];
+            load_plane<2, 3>(g, t, sq, (long)kz, [&](int m, long q) {
+                const unsigned e = (unsigned)q;
+                return m == 0 ? fld(b5, e) - fld(bxi, e) : m == 1 ? fld(b6, e) - fld(byi, e) : fld(b7, e) - fld(bzi, e);
+            });
+            __syncthreads();
             const int X0 = t.tx + 2, Y0 = t.ty + 2;
-            double az[4] = {g.z1a[k], g.z1b[k], g.z1c[k], g.z1d[k]};
-#define DXS(f) fd5(sp[f][Y0][X0 - 2], sp[f][Y0][X0 - 1], sp[f][Y0][X0], sp[f][Y0][X0 + 1], sp[f][Y0][X0 + 2], ax[0], ax[1], ax[2], ax[3])
-#define DYS(f) fd5(sp[f][Y0 - 2][X0], sp[f][Y0 - 1][X0], sp[f][Y0][X0], sp[f][Y0 + 1][X0], sp[f][Y0 + 2][X0], ay[0], ay[1], ay[2], ay[3])
-            double vz = fd5(vq[0], vq[1], vq[2], vq[3], vq[4], az[0], az[1], az[2], az[3]);
-            double uz = fd5(uq[0], uq[1], uq[2], uq[3], uq[4], az[0], az[1], az[2], az[3]);
-            v[0] = DYS(2) - vz;
-            v[1] = uz - DXS(2);
-            v[2] = DXS(1) - DYS(0);
+            if (ld) { u = sq[0][Y0][X0]; w = sq[1][Y0][X0]; }
+            if (t.inner) {
+#define DXS(f) fd5(sq[f][Y0][X0 - 2], sq[f][Y0][X0 - 1], sq[f][Y0][X0], sq[f][Y0][X0 + 1], sq[f][Y0][X0 + 2], ax[0], ax[1], ax[2], ax[3])
+#define DYS(f) fd5(sq[f][Y0 - 2][X0], sq[f][Y0 - 1][X0], sq[f][Y0][X0], sq[f][Y0 + 1][X0], sq[f][Y0 + 2][X0], ay[0], ay[1], ay[2], ay[3])
+                xa = DYS(2);
+                xb = DXS(2);
+                xc = DXS(1) - DYS(0);
 #undef DXS
 #undef DYS
+            }
+        } else if (ld) {
+            const unsigned e = e0 + kz;
+            u = fld(b5, e) - fld(bxi, e);
+            w = fld(b6, e) - fld(byi, e);
         }
+        #pragma unroll
+        for (int s = 0; s < 4; s++) { uq[s] = uq[s + 1]; vq[s] = vq[s + 1]; }
+        uq[4] = u; vq[4] = w;
+        pa[0] = pa[1]; pa[1] = pa[2]; pa[2] = xa;
+        pb[0] = pb[1]; pb[1] = pb[2]; pb[2] = xb;
+        pc[0] = pc[1]; pc[1] = pc[2]; pc[2] = xc;
+        const int k = tp - 2;
+        if (k < t.kb) continue;   // block-uniform
+        double v[3] = {0, 0, 0};
+        if (t.inner) {
+            const double az0 = g.z1a[k], az1 = g.z1b[k], az2 = g.z1c[k], az3 = g.z1d[k];
+            const double vz = fd5(vq[0], vq[1], vq[2], vq[3], vq[4], az0, az1, az2, az3);
+            const double uz = fd5(uq[0], uq[1], uq[2], uq[3], uq[4], az0, az1, az2, az3);
+            v[0] = pa[0] - vz;
+            v[1] = uz - pb[0];
+            v[2] = pc[0];
+        }
+        // past the chunk no plane load separates two calls of the rules,
+        // whose LDS slots the previous call may still be reading
+        if (tp >= t.ke) __syncthreads();
         plane_rules<3, BC_AUX>(g, t, bs, v);
         if (t.own) {
-            const unsigned e = e0 + kz;
+            const unsigned e = e0 + sz * (unsigned)k;
             st(fldp(cur, e), v[0]); st(fldp(cur + S, e), v[1]); st(fldp(cur + 2 * S, e), v[2]);
             zface_store<3>(g, cur, (long)e0, k, v);
         }
-        __syncthreads();
-        #pragma unroll
-        for (int s = 0; s < 4; s++) { uq[s] = uq[s + 1]; vq[s] = vq[s + 1]; }
     }
 }
 
@@ -1325,7 +1341,7 @@ __global__ void k_fill_state(Geo g, double *__restrict__ XD, int nf)
 // block table and the run's arguments from a device table, so one launch
 // covers the same sweep of every run.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_curl(Geo g, const double *__restrict__ X, const double *__restrict__ bxi,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_curl(Geo g, const double *__restrict__ X, const double *__restrict__ bxi,
                                              const double *__restrict__ byi, const double *__restrict__ bzi,
                                              double *__restrict__ cur, int kc)
 {
@@ -1455,7 +1471,7 @@ __global__ __launch_bounds__(TX * RB) __attribute__((amdgpu_waves_per_eu(MODE ==
     zsmooth_body<NP, KIND, TYPE, RKS, MODE, RB>(a.g, a.A, a.B, a.rk, bx, by, chunk, m, a.kc);
 }
 
-__global__ __launch_bounds__(NT) void kb_curl(const BCurlArgs *__restrict__ args, const BEntry *__restrict__ tab)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6, 8))) void kb_curl(const BCurlArgs *__restrict__ args, const BEntry *__restrict__ tab)
 {
     const unsigned run = __builtin_amdgcn_readfirstlane(tab[blockIdx.x].run);
     const BCurlArgs &a = run_args(args, run);

@@ -24,6 +24,8 @@ KNOBS = {
 CASES = {
     "wide_nonperiodic_hall": lambda: harris.case_small(130, 30, 40, periodic_y=False, seed=17),
     "stretched_periodic_hall": lambda: harris.case_small(24, 8, 24),
+    # several windows in x and in periodic y, chunks ending inside the volume
+    "multiwindow_periodic_hall": lambda: harris.case_small(130, 36, 72, seed=5),
 }
 
 
