@@ -101,13 +101,14 @@ __device__ __forceinline__ void dt_partial_body(const Geo &g, const double *__re
                                                 double *__restrict__ part, unsigned b, unsigned nb)
 {
     __shared__ double sm[256];
-    int j0 = g.periodic ? 2 : 0, j1 = g.periodic ? g.NY - 3 : g.NY - 2;
-    long ni = g.NX - 2, nj = j1 - j0 + 1, nk = g.NZ - 2;
-    long n = ni * nj * nk;
+    const int j0 = g.periodic ? 2 : 0, j1 = g.periodic ? g.NY - 3 : g.NY - 2;
+    const int ni = g.NX - 2, nj = j1 - j0 + 1, nk = g.NZ - 2;
+    const unsigned nrow = (unsigned)nj * (unsigned)nk;
     double best = 100.;
-    for (long t = (long)b * blockDim.x + threadIdx.x; t < n; t += (long)nb * blockDim.x) {
-        int i = (int)(t % ni); long r = t / ni;
-        int j = j0 + (int)(r % nj); int k = (int)(r / nj);
+    // block b takes the rows (j, k) b, b + nb, ...; its threads run along x
+    for (unsigned row = b; row < nrow; row += nb)
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        const int j = j0 + (int)(row % (unsigned)nj), k = (int)(row / (unsigned)nj);
         long q = IDX(g, i, j, k), S = g.S;
         if (rr && !(rr[q] >= 1.)) continue;
         // quotients with a shared or precomputed rounded reciprocal (divc:

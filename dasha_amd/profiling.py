@@ -89,6 +89,40 @@ def kernel_stats(d: str):
     return out
 
 
+def family_calls(d: str):
+    """wall time of each call of each kernel family from the kernel trace:
+    a family's dispatches in start order, a new call where a kernel of the
+    current call would repeat or an alternative of the family (another RK
+    stage) begins (the kernels of one call -- a sweep's face and
+    face-free launches, the parts of right part 2 -- may run side by side on
+    two streams); wall = last end - first start"""
+    hits = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
+    if not hits:
+        return {}
+    disp = collections.defaultdict(list)
+    for r in csv.DictReader(open(hits[0])):
+        fam = family(r["Kernel_Name"])
+        if fam is None or fam == "copy_probe":
+            continue
+        disp[fam].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    out = {}
+    for fam, ds in disp.items():
+        ds.sort()
+        calls, cur, names, grp = [], None, set(), None
+        for s, e, n in ds:
+            g = "*" if fam in _TOGETHER else launch_group(n)
+            if cur is None or n in names or g != grp:
+                if cur is not None:
+                    calls.append(cur[1] - cur[0])
+                cur, names, grp = [s, e], set(), g
+            cur[0], cur[1] = min(cur[0], s), max(cur[1], e)
+            names.add(n)
+        if cur is not None:
+            calls.append(cur[1] - cur[0])
+        out[fam] = (len(calls), sum(calls) / len(calls) / 1e6)
+    return out
+
+
 def counter_per_kernel(d: str, counter: str):
     """mean value of `counter` per dispatch, by kernel name (KiB)"""
     acc = collections.defaultdict(list)
@@ -154,12 +188,18 @@ def main():
         g = fam_t[r["family"]]["*" if r["family"] in _TOGETHER else launch_group(r["name"])]
         g[0] += r["total_ms"]
         g[1] = max(g[1], r["calls"])
-    lines += ["", "Per launch of each kernel family (the kernels one call of the family runs; "
-              "what bench.py brackets with HIP events):", "",
-              "| kernel family | launches | avg ms |", "|---|---|---|"]
+    walls = family_calls(args.stats)
+    lines += ["", "Per call of each kernel family (the kernels one call of the family runs; "
+              "what bench.py brackets with HIP events). Summed: the kernels' durations added; "
+              "wall: first start to last end of the call in the trace (the face and face-free "
+              "launches of a sweep and the two parts of right part 2 run side by side on two "
+              "streams; the first two Hall passes run beside the smoothing of E):", "",
+              "| kernel family | calls | summed avg ms | wall avg ms |", "|---|---|---|---|"]
     for fam, groups in sorted(fam_t.items(), key=lambda kv: -sum(v[0] for v in kv[1].values())):
         calls = sum(v[1] for v in groups.values())
-        lines.append(f"| {fam} | {calls} | {sum(v[0] for v in groups.values()) / max(calls, 1):.3f} |")
+        w = walls.get(fam)
+        lines.append(f"| {fam} | {calls} | {sum(v[0] for v in groups.values()) / max(calls, 1):.3f} | "
+                     f"{'%.3f' % w[1] if w else '-'} |")
     if args.fetch and args.write:
         tr, fs, ws = traffic(args.fetch, args.write, args.copy_bytes)
         lines += ["", "HBM traffic per launch (FETCH_SIZE x %.3f, WRITE_SIZE x %.3f: the factors that make "
