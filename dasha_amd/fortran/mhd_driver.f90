@@ -7,7 +7,10 @@
 ! Inputs in the run directory (written by dasha_amd.rundir):
 !   mhd.nml      namelist /run/ (grid sizes, physics flags and constants,
 !                nstop, nst, ncase, dtime, dnstep)
-!   grid.bin     xx, yy, zz, dx, dy, dz
+!   grid.bin     xx, yy, zz, dx, dy, dz; with gridgen = .true. the program
+!                builds the mesh itself as gridpntnew2 does (t3dsdsc.f:4380-4690,
+!                switches uniformx/y/z, halfx/y/z and the extents of the namelist;
+!                non-periodic y) and writes gridx.dat and grid.dat like the reference
 !   equil.bin    rhoi, bxi, byi, bzi, pri
 !   mhd<nst><000> restart record with the initial state (readin format)
 ! Magnetosphere mode (magneto = .true.; the reference's own case): with
@@ -38,9 +41,14 @@ program mhd_driver
   logical :: periody, hall, whistler
   logical :: magneto, initia, mxdat, mxdatfc, inflow, radial_hall, partial_e
   real(c_double) :: di, eta0, fmu0, frho0, fpr0, gamma, cfl, dtime, rcr
+  ! gridpntnew2's switches and extents (mhd03.for)
+  logical :: gridgen, uniformx, uniformy, uniformz, halfx, halfy, halfz
+  real(c_double) :: xmin0, xmin, xmax, ymin, ymax, zmin0, zmin, zmax, dymin, dymax, yhr, dzmin, dzmax, zhr
   namelist /run/ mx, my, mz, periody, hall, whistler, di, eta0, fmu0, frho0, fpr0, gamma, cfl, &
        nstop, nst, ncase, dtime, dnstep, nend, nint, nadj, adjobj, &
-       magneto, initia, mxdat, mxdatfc, inflow, radial_hall, partial_e, rcr
+       magneto, initia, mxdat, mxdatfc, inflow, radial_hall, partial_e, rcr, &
+       gridgen, uniformx, uniformy, uniformz, halfx, halfy, halfz, &
+       xmin0, xmin, xmax, ymin, ymax, zmin0, zmin, zmax, dymin, dymax, yhr, dzmin, dzmax, zhr
   type(mhd_config) :: cfg
   type(mhd_diag) :: dg
   type(c_ptr) :: h
@@ -60,6 +68,12 @@ program mhd_driver
   magneto = .false.; initia = .false.; mxdat = .false.; mxdatfc = .false.
   inflow = .false.; radial_hall = .false.; partial_e = .false.; rcr = 1.d0
   nstp = 0
+  ! mhd03.for's mesh (default real literals)
+  gridgen = .false.
+  uniformx = .true.; uniformy = .true.; uniformz = .true.
+  halfx = .false.; halfy = .false.; halfz = .false.
+  xmin0 = -2.1; xmin = -2.1; xmax = 5.65; ymin = 0.; ymax = 2.5; zmin0 = -2.5; zmin = 0.; zmax = 2.5
+  dymin = 0.01; dymax = 1.0; yhr = 0.0; dzmin = 0.01; dzmax = 1.0; zhr = 0.0
   open(newunit=u, file='mhd.nml', status='old', action='read', iostat=ios)
   if (ios /= 0) call fail('cannot open mhd.nml', 1)
   read(u, nml=run)
@@ -68,14 +82,18 @@ program mhd_driver
   allocate(xx(mx+4), yy(my), zz(mz+4), dx(mx+4), dy(my), dz(mz+4))
   allocate(rhoi(mx+4,my,mz+4), bxi(mx+4,my,mz+4), byi(mx+4,my,mz+4), bzi(mx+4,my,mz+4), &
        pri(mx+4,my,mz+4), x(mx+4,my,mz+4,8))
-  open(newunit=u, file='grid.bin', form='unformatted', status='old', action='read')
-  read(u) xx
-  read(u) yy
-  read(u) zz
-  read(u) dx
-  read(u) dy
-  read(u) dz
-  close(u)
+  if (gridgen) then
+     call build_mesh()
+  else
+     open(newunit=u, file='grid.bin', form='unformatted', status='old', action='read')
+     read(u) xx
+     read(u) yy
+     read(u) zz
+     read(u) dx
+     read(u) dy
+     read(u) dz
+     close(u)
+  end if
   if (initia) then
      call build_initia()
   else
@@ -284,6 +302,29 @@ contains
     write(v) x
     close(v)
   end subroutine build_initia
+
+  subroutine build_mesh()
+    ! gridpntnew2 (t3dsdsc.f:4380-4690) and input's placement of the global
+    ! mesh in the one domain's padded arrays (t3dsdsc.f:296-305: physical
+    ! points at jx = 3..mx+2, ghosts 0; y as it is), with the reference's
+    ! gridx.dat (t3dsdsc.f:4453-4458) and grid.dat (t3dsdsc.f:287-293)
+    real(c_double) :: xxt(mx), dxt(mx), zzt(mz), dzt(mz)
+    integer :: v, ios, j
+    if (periody) call fail('gridgen builds the reference''s non-periodic mesh; periody needs grid.bin', 1)
+    call mesh_x(mx, xmin0, xmin, xmax, uniformx, halfx, xxt, dxt, ios)
+    if (ios /= 0) call fail('the graded x mesh of gridpntnew2 has 1441 points', 1)
+    call mesh_yz(my, ymin, ymax, -ymax, uniformy, halfy, dymin, dymax, yhr, yy, dy)
+    call mesh_yz(mz, zmin, zmax, zmin0, uniformz, halfz, dzmin, dzmax, zhr, zzt, dzt)
+    xx = 0.d0; dx = 0.d0; zz = 0.d0; dz = 0.d0
+    xx(3:mx+2) = xxt; dx(3:mx+2) = dxt
+    zz(3:mz+2) = zzt; dz(3:mz+2) = dzt
+    open(newunit=v, file='gridx.dat', status='replace', form='formatted')
+    write(v, '(2(i5,2(1x,f13.5)))') (j, xxt(j), dxt(j), j = 1, mx)
+    close(v)
+    open(newunit=v, file='grid.dat', status='replace', form='formatted')
+    write(v, '(6(1x,f13.5))') (xxt(j), dxt(j), j = 1, mx), (zzt(j), dzt(j), j = 1, mz), (yy(j), dy(j), j = 1, my)
+    close(v)
+  end subroutine build_mesh
 
   subroutine set_face()
     ! fcinp/bin* (t3dsdsc.f:728-768) -> bxinfc, byinfc, bzinfc

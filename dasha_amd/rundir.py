@@ -99,12 +99,15 @@ def read_fcinp_field(path: str, my: int, mz: int) -> np.ndarray:
 
 def write_case(rundir: str, case, *, nstop: int, nst: int = 1, ncase: int = 1, dtime: float = 1.0,
                dnstep: int = 10, nadj: int = 0, adjobj: int = 0, initia: bool = False,
-               mxdat: bool = False, mxdatfc: bool = False):
+               mxdat: bool = False, mxdatfc: bool = False, mesh: dict | None = None):
     """Write grid.bin, equil.bin, mhd.nml and the initial restart file
     (nadj > 0: the driver differentiates nadj steps instead of running).
     For a magnetosphere case the namelist carries its switches; initia: the
     driver builds the initial state itself (mxdat: from inp/bin*), mxdatfc:
-    it reads the inflow face data from fcinp/bin*."""
+    it reads the inflow face data from fcinp/bin*.  mesh: the driver builds
+    the mesh itself (gridgen) from these namelist entries (gridpntnew2's
+    switches and extents: uniformx, halfx, xmin0, xmin, xmax, ymax, dymin,
+    ...; booleans or floats) instead of reading grid.bin."""
     os.makedirs(rundir, exist_ok=True)
     with open(os.path.join(rundir, "grid.bin"), "wb") as f:
         for a in (case.xx, case.yy, case.zz, case.dx, case.dy, case.dz):
@@ -126,6 +129,10 @@ def write_case(rundir: str, case, *, nstop: int, nst: int = 1, ncase: int = 1, d
             f.write(f",\n magneto=.true., initia={b(initia)}, mxdat={b(mxdat)}, mxdatfc={b(mxdatfc)},\n")
             f.write(f" inflow={b(case.inflow)}, radial_hall={b(case.radial_hall)}, "
                     f"partial_e={b(case.partial_e)}, rcr={case.rcr!r}")
+        if mesh is not None:
+            f.write(",\n gridgen=.true.")
+            for k, v in mesh.items():
+                f.write(f", {k}={b(v) if isinstance(v, bool) else repr(float(v))}")
         f.write("\n/\n")
     write_restart(os.path.join(rundir, "mhd" + cn(nst) + cn(0)), case.x0, ncase=ncase, nstep=0,
                   time=0.0, nst=nst)

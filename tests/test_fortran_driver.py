@@ -10,7 +10,9 @@ through the same library, with the fort.20 and current.dat cadence of
 t3dsdsc.f:171-181 and 1174-1182; a second one checks the m2d/m3d plot
 files of an output step against the state and current at that step; a third
 runs the driver's adjoint mode (mhd_adjoint_run called from Fortran) and
-requires the Python call's value and gradients bit for bit.
+requires the Python call's value and gradients bit for bit; the last two
+run the magnetosphere mode from the driver's own initial state, and from
+the mesh the driver builds itself (gridgen).
 """
 import os
 import subprocess
@@ -196,3 +198,44 @@ def test_driver_magnetosphere_initia(tmp_path):
     t, _ = g.run(nstop + 1)
     assert np.array_equal(xf, g.get_state())
     assert hdr["time"] == t
+
+
+@pytest.mark.gpu
+def test_driver_builds_mesh(tmp_path):
+    """gridgen: the driver builds the reference's uniform mesh itself
+    (gridpntnew2 with mhd03.for's switches, the extents of the small
+    magnetosphere) instead of reading grid.bin; the run equals the Python
+    binding run on the Python-built mesh bit for bit, and gridx.dat /
+    grid.dat hold the global mesh in the reference's formats"""
+    import dataclasses
+    from dasha_amd import magneto
+    from dasha_amd.mhd import MHDStep
+    drv = _driver()
+    case = magneto.case_small()
+    lo = float(case.xx[2])
+    mesh = dict(uniformx=True, uniformy=True, uniformz=True, halfx=False, halfy=False, halfz=False,
+                xmin0=lo, xmin=lo, xmax=2.5, ymin=0.0, ymax=1.5, zmin=0.0, zmax=1.5, zmin0=-1.5)
+    nstop = 2
+    rundir.write_case(str(tmp_path), case, nstop=nstop, dtime=1e9, initia=True, mesh=mesh)
+    # a grid.bin the driver must not read
+    os.remove(tmp_path / "grid.bin")
+    print(f"[driver] gridgen, case={case.name}", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "110", drv], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    _, x0 = rundir.read_restart(str(tmp_path / "mhdinit"), case.x0.shape)
+    np.testing.assert_allclose(x0, case.x0, rtol=1e-13, atol=1e-300)
+    hdr, xf = rundir.read_restart(str(tmp_path / "mhdfinal"), case.x0.shape)
+    # the Python run from the driver's initia state on the Python-built mesh
+    g = MHDStep(dataclasses.replace(case, x0=x0, rhoi=x0[0].copy(), bxi=x0[4].copy(), byi=x0[5].copy(),
+                                    bzi=x0[6].copy(), pri=x0[7].copy()))
+    t, _ = g.run(nstop + 1)
+    assert np.array_equal(xf, g.get_state()) and hdr["time"] == t
+    # gridx.dat: (jx, x, dx) twice per line; grid.dat: x/dx, z/dz, y/dy pairs
+    tok = open(tmp_path / "gridx.dat").read().split()
+    assert len(tok) == 3 * case.mx and int(tok[-3]) == case.mx
+    assert np.allclose([float(t) for t in tok[1::3]], case.xx[2:-2], atol=6e-6)
+    vals = np.array(open(tmp_path / "grid.dat").read().split(), dtype=float)
+    want = np.concatenate([np.stack([case.xx[2:-2], case.dx[2:-2]], 1).ravel(),
+                           np.stack([case.zz[2:-2], case.dz[2:-2]], 1).ravel(),
+                           np.stack([case.yy, case.dy], 1).ravel()])
+    assert vals.shape == want.shape and np.allclose(vals, want, atol=6e-6)
