@@ -89,6 +89,51 @@ __global__ void k_copy(long n, const double *__restrict__ a, double *__restrict_
     if (t < n) st(b + t, a[t]);
 }
 
+// U(-1, 1) from the splitmix64 finaliser of (seed, index): the top 53 bits
+// of the mixed word scaled to [0, 1), then 2u - 1 (exact in FP64)
+__device__ inline double splitmix_u(unsigned long long seed, unsigned long long index)
+{
+    unsigned long long z = seed * 0x9E3779B97F4A7C15ull + (index + 1ull) * 0xD1B54A32D192ED03ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+}
+
+// Harris sheet (mhd_init_harris): one thread per point (i, j, k) of the
+// device volume, all 8 state fields and the 5 equilibrium fields
+__global__ void k_harris(Geo g, mhd_harris hp, int nyu, double *__restrict__ X, double *__restrict__ EQ)
+{
+    const long n = (long)g.NX * g.NY * g.NZ;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % g.NX);
+    const long r = t / g.NX;
+    const int j = (int)(r % g.NY), k = (int)(r / g.NY);
+    const long q = IDX(g, i, j, k), S = g.S;
+    const double Z = g.zc[k], xr = g.xc[i] - hp.x_offset;
+    const double ch = cosh(Z / hp.lam);
+    const double sech2 = 1.0 / (ch * ch);
+    const double pri = 0.5 * hp.beta_inf + 0.5 * sech2;
+    const double rhoi = pri / hp.temp;
+    const double bxi = tanh(Z / hp.lam);
+    const double w2 = hp.width * hp.width;
+    const double gp = hp.amp * exp(-(xr * xr + Z * Z) / w2);
+    const double dbx = 2.0 * Z / w2 * gp, dbz = -2.0 * xr / w2 * gp;
+    EQ[q] = rhoi; EQ[q + S] = bxi; EQ[q + 2 * S] = hp.guide; EQ[q + 3 * S] = 0.0; EQ[q + 4 * S] = pri;
+    X[q] = rhoi;
+    X[q + 4 * S] = bxi + dbx;
+    X[q + 5 * S] = hp.guide;
+    X[q + 6 * S] = 0.0 + dbz;
+    X[q + 7 * S] = pri;
+    const int ju = g.periodic ? ((j - 2) % nyu + nyu) % nyu : j;
+    for (int c = 0; c < 3; c++) {
+        const unsigned long long idx =
+            (((unsigned long long)c * g.NZ + k) * (unsigned long long)nyu + ju) * g.NX + i;
+        X[q + (1 + c) * S] = rhoi * hp.noise * splitmix_u(hp.seed, idx);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // setdt, t3dsdsc.f:4709-4808: minimum over jz = 3..mz, jy = 1..my-1,
 // jx = 3..mx (k = 0..NZ-3, j = 0..NY-2, i = 0..NX-3); with periodic y the
@@ -1794,6 +1839,23 @@ int mhd_set_state(mhd_handle *h, const double *x)
 {
     if (!h || !x) return MHD_ERR_ARG;
     return upload_fields(h, h->x, x, 8);
+}
+
+int mhd_init_harris(mhd_handle *h, const mhd_harris *p)
+{
+    if (!h || !p) return MHD_ERR_ARG;
+    if (h->mag) { set_err(h, "mhd_init_harris builds the Harris sheet, not the magnetosphere"); return MHD_ERR_ARG; }
+    if (!(p->lam != 0.0) || !(p->temp != 0.0) || !(p->width != 0.0)) {
+        set_err(h, "mhd_init_harris: lam, temp and width must be nonzero");
+        return MHD_ERR_ARG;
+    }
+    const int nyu = h->g.periodic ? h->NY - 4 : h->NY;
+    const long n = (long)h->NX * h->NY * h->NZ;
+    hipLaunchKernelGGL(k_harris, dim3((unsigned)blocks(n, 256)), dim3(256), 0, h->stream, h->g, *p, nyu, h->x, h->eq);
+    HCHK(hipGetLastError());
+    HCHK(hipStreamSynchronize(h->stream));
+    h->have_eq = true;
+    return MHD_OK;
 }
 
 int mhd_get_state(mhd_handle *h, double *x)

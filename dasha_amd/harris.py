@@ -107,6 +107,9 @@ class MHDCase:
     fmu0: float = 0.0
     frho0: float = 0.0
     fpr0: float = 0.0
+    # the Harris-sheet parameters (mhd_init_harris); with the 3-D arrays left
+    # out (None) the device builds the equilibrium and the state itself
+    harris: dict | None = None
 
     @property
     def ny(self) -> int:
@@ -134,10 +137,12 @@ def harris_case(name: str, mx: int, ny: int, mz: int, *, stretched: bool, lx: fl
                 eta0: float = 0.005, di: float = 0.1, amp: float = 0.1, width: float = 1.0,
                 noise: float = 1e-4, seed: int = 1, cfl: float = 0.4, steps: int = 100,
                 periodic_y: bool = True, hall: bool | None = None, whistler_dt: bool = True,
-                x_offset: float = 0.0, gamma: float = 5.0 / 3.0) -> MHDCase:
+                x_offset: float = 0.0, gamma: float = 5.0 / 3.0, arrays: bool = True) -> MHDCase:
     """Harris sheet Bx = tanh(z/lam), p = p_inf + 0.5 sech^2, rho = p/temp,
     uniform guide By, plus dAy = amp*exp(-(x^2+z^2)/width^2) and seeded
-    noise rho*v_i = rho*noise*U(-1,1) on the unique physical points."""
+    noise rho*v_i = rho*noise*U(-1,1) on the unique physical points.
+    arrays=False: the mesh and parameters only (the device builds the 3-D
+    fields, mhd_init_harris, with the same arithmetic)."""
     if stretched:
         px, sx = stretched_points(mx, dmin, ratio)
         pz, sz = stretched_points(mz, dmin, ratio)
@@ -154,6 +159,15 @@ def harris_case(name: str, mx: int, ny: int, mz: int, *, stretched: bool, lx: fl
         yy, dy = uniform_points(ny, -ly / 2, ly / 2)
         my = ny
     xx, zz, dx, dz = pad2(px), pad2(pz), pad2(sx), pad2(sz)
+    if hall is None:
+        hall = di != 0.0
+    params = dict(lam=lam, beta_inf=beta_inf, guide=guide, temp=temp, amp=amp, width=width,
+                  noise=noise, x_offset=x_offset, seed=seed)
+    if not arrays:
+        return MHDCase(name=name, mx=mx, my=my, mz=mz, periodic_y=periodic_y, hall=hall, di=di,
+                       eta0=eta0, gamma=gamma, cfl=cfl, whistler_dt=whistler_dt, steps=steps,
+                       xx=xx, yy=yy, zz=zz, dx=dx, dy=dy, dz=dz, rhoi=None, bxi=None, byi=None,
+                       bzi=None, pri=None, x0=None, harris=params)
     shape = (mz + 4, my, mx + 4)
     Z = np.zeros(shape)
     Xc = np.zeros(shape)
@@ -194,30 +208,28 @@ def harris_case(name: str, mx: int, ny: int, mz: int, *, stretched: bool, lx: fl
     for a in (rhoi, pri, bxi, byi, bzi):
         a[:2] = 0; a[-2:] = 0; a[:, :, :2] = 0; a[:, :, -2:] = 0
     x0[:, :2] = 0; x0[:, -2:] = 0; x0[:, :, :, :2] = 0; x0[:, :, :, -2:] = 0
-    if hall is None:
-        hall = di != 0.0
     return MHDCase(name=name, mx=mx, my=my, mz=mz, periodic_y=periodic_y, hall=hall, di=di,
                    eta0=eta0, gamma=gamma, cfl=cfl, whistler_dt=whistler_dt, steps=steps,
                    xx=xx, yy=yy, zz=zz, dx=dx, dy=dy, dz=dz, rhoi=rhoi, bxi=bxi, byi=byi,
-                   bzi=bzi, pri=pri, x0=x0)
+                   bzi=bzi, pri=pri, x0=x0, harris=params)
 
 
-def case_c1() -> MHDCase:
+def case_c1(arrays: bool = True) -> MHDCase:
     """64x32x64 uniform, box 20x10x20, eta 0.005, d_i 0.1, 100 steps."""
     return harris_case("C1_64x32x64", 64, 32, 64, stretched=False, lam=0.5, beta_inf=0.2,
-                       eta0=0.005, di=0.1, amp=0.1, seed=1, steps=100)
+                       eta0=0.005, di=0.1, amp=0.1, seed=1, steps=100, arrays=arrays)
 
 
-def case_c2() -> MHDCase:
+def case_c2(arrays: bool = True) -> MHDCase:
     """256x128x256 stretched (dmin 0.05, ratio 3), eta 0.003, d_i 0.2, 1000 steps."""
     return harris_case("C2_256x128x256", 256, 128, 256, stretched=True, lam=0.5, beta_inf=0.2,
-                       eta0=0.003, di=0.2, amp=0.05, seed=2, steps=1000)
+                       eta0=0.003, di=0.2, amp=0.05, seed=2, steps=1000, arrays=arrays)
 
 
-def case_c3() -> MHDCase:
+def case_c3(arrays: bool = True) -> MHDCase:
     """512x256x512 stretched, beta 0.1, guide By 0.1, eta 0.002, d_i 0.25, 200 steps."""
     return harris_case("C3_512x256x512", 512, 256, 512, stretched=True, lam=0.5, beta_inf=0.1,
-                       guide=0.1, eta0=0.002, di=0.25, amp=0.05, seed=3, steps=200)
+                       guide=0.1, eta0=0.002, di=0.25, amp=0.05, seed=3, steps=200, arrays=arrays)
 
 
 def case_small(mx: int = 24, ny: int = 8, mz: int = 24, *, stretched: bool = True,

@@ -29,7 +29,8 @@ EXPORTS = ("mhd_create", "mhd_set_equilibrium", "mhd_set_state", "mhd_get_state"
            "mhd_last_error", "mhd_destroy", "mhd_ensemble_create", "mhd_ensemble_advance",
            "mhd_ensemble_status", "mhd_ensemble_error", "mhd_ensemble_destroy",
            "mhd_adjoint_run", "mhd_adjoint_rhs", "mhd_ensemble_tune", "mhd_compute_dt_noalloc",
-           "mhd_set_inflow_face", "mhd_rhs_stage", "mhd_ensemble_tune_shapes", "mhd_ensemble_set_shape")
+           "mhd_set_inflow_face", "mhd_rhs_stage", "mhd_ensemble_tune_shapes", "mhd_ensemble_set_shape",
+           "mhd_init_harris")
 
 MHD_ADJ_RESIDUAL = 1
 OBJECTIVES = {"magnetic_energy": 0, "reconnected_flux": 1}
@@ -42,6 +43,11 @@ class MHDConfig(C.Structure):
                 ("fpr0", C.c_double), ("gamma", C.c_double), ("cfl", C.c_double),
                 ("magnetosphere", C.c_int), ("inflow", C.c_int), ("radial_hall", C.c_int),
                 ("partial_e", C.c_int), ("rcr", C.c_double)]
+
+
+class MHDHarris(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("lam", "beta_inf", "guide", "temp", "amp", "width", "noise",
+                                           "x_offset")] + [("seed", C.c_ulonglong)]
 
 
 class MHDDiag(C.Structure):
@@ -85,6 +91,7 @@ def load_library(path: str = LIBPATH):
     P, D, I = C.c_void_p, C.POINTER(C.c_double), C.c_int
     L.mhd_create.argtypes = [C.POINTER(MHDConfig), D, D, D, D, D, D, C.POINTER(P)]
     L.mhd_set_equilibrium.argtypes = [P, D, D, D, D, D]
+    L.mhd_init_harris.argtypes = [P, C.POINTER(MHDHarris)]
     for n in ("mhd_set_state", "mhd_get_state", "mhd_get_current", "mhd_get_divb",
               "mhd_get_xdif", "mhd_get_residual", "mhd_set_residual"):
         getattr(L, n).argtypes = [P, D]
@@ -159,6 +166,10 @@ class MHDStep:
         self.h = h
         self._check(rc)
         self.shape3 = (case.mz + 4, case.my, case.mx + 4)
+        if case.x0 is None:
+            # a Harris case without host arrays: the device builds them
+            self.init_harris(case.harris)
+            return
         eq = [np.ascontiguousarray(a, dtype=np.float64)
               for a in (case.rhoi, case.bxi, case.byi, case.bzi, case.pri)]
         self._check(self.L.mhd_set_equilibrium(self.h, *[_p(a) for a in eq]))
@@ -192,6 +203,12 @@ class MHDStep:
         out = np.empty((nf,) + self.shape3)
         self._check(getattr(self.L, fn)(self.h, _p(out)))
         return out
+
+    def init_harris(self, params: dict):
+        """the Harris equilibrium and initial state built on the device
+        (mhd_init_harris) from the mesh and harris_case's parameters"""
+        hp = MHDHarris(**{k: v for k, v in params.items() if k != "seed"}, seed=int(params["seed"]))
+        self._check(self.L.mhd_init_harris(self.h, C.byref(hp)))
 
     def set_state(self, x):
         a = np.ascontiguousarray(x, dtype=np.float64)

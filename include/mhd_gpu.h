@@ -89,6 +89,22 @@ int mhd_set_equilibrium(mhd_handle *h, const double *rhoi, const double *bxi,
 /* x(mx+4, my, mz+4, 8) (mhd02.for:1, /var21/) */
 int mhd_set_state(mhd_handle *h, const double *x);
 int mhd_get_state(mhd_handle *h, double *x);
+/* The Harris-sheet initial state of the benchmark workloads, built on the
+ * device from the handle's mesh: it takes the place of mhd_set_equilibrium
+ * + mhd_set_state for these cases without 3-D host arrays (no counterpart
+ * in the reference, whose initia, t3dsdsc.f:612-850, builds the
+ * magnetosphere).  Equilibrium Bx = tanh(z/lam), By = guide, Bz = 0,
+ * p = beta_inf/2 + sech^2(z/lam)/2, rho = p/temp; state = equilibrium plus
+ * dBx = -d(dAy)/dz, dBz = d(dAy)/dx of dAy = amp exp(-((x-x_offset)^2+z^2)/width^2)
+ * and rho v_c = rho noise U(-1,1) with U from the splitmix64 finaliser of
+ * (seed, ((c mz + k) ny + j) mx + i) over the unique physical points
+ * (0-based i < mx, j < ny unique rows, k < mz, component c < 3).  Image
+ * rows of periodic y repeat their unique rows. */
+typedef struct mhd_harris {
+    double lam, beta_inf, guide, temp, amp, width, noise, x_offset;
+    unsigned long long seed;
+} mhd_harris;
+int mhd_init_harris(mhd_handle *h, const mhd_harris *p);
 /* xresd = RHS momentum at t=0 with eta=mu=0 (t3dsdsc.f:816-834) */
 int mhd_init_residual(mhd_handle *h);
 /* setdt (t3dsdsc.f:4694-4810) */
