@@ -21,6 +21,12 @@ module mhd_gpu
      real(c_double) :: divb_max, divb_rms
   end type mhd_diag
 
+  ! Harris-sheet initial state built on the device (mhd_init_harris)
+  type, bind(C) :: mhd_harris
+     real(c_double) :: lam, beta_inf, guide, temp, amp, width, noise, x_offset
+     integer(c_long_long) :: seed
+  end type mhd_harris
+
   integer(c_int), parameter :: MHD_OK = 0, MHD_ERR_ARG = -1, MHD_ERR_HIP = -2, &
        MHD_ERR_NOMEM = -3, MHD_ERR_DT_COLLAPSE = -4, MHD_ERR_BLOWUP = -5
   integer(c_int), parameter :: MHD_ADJ_RESIDUAL = 1
@@ -51,6 +57,12 @@ module mhd_gpu
        type(c_ptr), value :: h
        real(c_double), intent(out) :: x(*)
      end function mhd_get_state
+
+     integer(c_int) function mhd_init_harris(h, p) bind(C, name='mhd_init_harris')
+       import :: c_int, c_ptr, mhd_harris
+       type(c_ptr), value :: h
+       type(mhd_harris), intent(in) :: p
+     end function mhd_init_harris
 
      integer(c_int) function mhd_init_residual(h) bind(C, name='mhd_init_residual')
        import :: c_int, c_ptr
