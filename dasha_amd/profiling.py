@@ -7,7 +7,8 @@ FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs into a short markdown summary and,
 with the counters, ``profiles/traffic_NAME.json``: HBM bytes per launch of
 each kernel family, scaled by what the same counters read for the library's
 plain copy kernel ``k_copy`` (bytes known: the probe copies ``--copy-bytes``
-per launch, read and write).
+per launch, read and write).  ``--sq DIR`` instead summarises one
+``rocprofv3 --pmc`` run of eight SQ counters per wave of each step kernel.
 """
 from __future__ import annotations
 
@@ -92,8 +93,9 @@ def kernel_stats(d: str):
 def family_calls(d: str):
     """wall time of each call of each kernel family from the kernel trace:
     a family's dispatches in start order, a new call where a kernel of the
-    current call would repeat or an alternative of the family (another RK
-    stage) begins (the kernels of one call -- a sweep's face and
+    current call would repeat, an alternative of the family (another RK
+    stage) begins or more than 20 us pass after the call's last end (the
+    kernels of one call -- a sweep's face and
     face-free launches, the parts of right part 2 -- may run side by side on
     two streams); wall = last end - first start"""
     hits = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
@@ -111,7 +113,9 @@ def family_calls(d: str):
         calls, cur, names, grp = [], None, set(), None
         for s, e, n in ds:
             g = "*" if fam in _TOGETHER else launch_group(n)
-            if cur is None or n in names or g != grp:
+            # a kernel of one call starts before, or within 20 us of, the end
+            # of the call's other kernels
+            if cur is None or n in names or g != grp or s > cur[1] + 20000:
                 if cur is not None:
                     calls.append(cur[1] - cur[0])
                 cur, names, grp = [s, e], set(), g
@@ -161,9 +165,32 @@ def traffic(fetch_dir: str, write_dir: str, copy_bytes: int):
     return res, f_scale, w_scale
 
 
+_SQ = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_WAVE_CYCLES", "SQ_WAIT_INST_ANY",
+       "SQ_LDS_BANK_CONFLICT", "SQ_BUSY_CYCLES")
+
+
+def sq_table(d: str, pattern: str = r"k_zsmooth|k_rhs_a\b|k_curl\b|k_rhs_b2"):
+    """per kernel (matching pattern): the SQ counters of one --pmc run,
+    averaged over the kernel's dispatches, per wave (SQ_WAVES per dispatch)"""
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(_one(d, "*counter_collection.csv"))):
+        if re.search(pattern, r["Kernel_Name"]):
+            acc[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    rows = []
+    for k, cs in acc.items():
+        mean = {c: sum(v) / len(v) for c, v in cs.items()}
+        w = mean.get("SQ_WAVES", 0.0)
+        if w <= 0:
+            continue
+        nm = re.sub(r"\(.*", "", k.replace("(anonymous namespace)::", ""))
+        rows.append((nm, w, {c: mean.get(c, 0.0) / w for c in _SQ[1:]}))
+    return sorted(rows, key=lambda r: r[0])
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--stats", required=True)
+    ap.add_argument("--stats")
+    ap.add_argument("--sq", help="directory of a rocprofv3 --pmc run of the SQ counters")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--copy-bytes", type=int, default=2 << 30)
@@ -172,6 +199,25 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--cells", type=int, default=0, help="physical cells (for bytes per cell)")
     args = ap.parse_args()
+    if args.sq:
+        lines = [f"# {args.workload}: SQ counters (rocprofv3)", "", "Command:", "", "```"]
+        lines += args.command.split("\\n")
+        lines += ["```", "", "Template arguments of k_zsmooth: <passes, coefficient kind, fill type, RK stage, "
+                  "launch class (0 all windows, 1 face-free, 2 face), rows per block>.  Per dispatch, averaged "
+                  "over the kernel's dispatches, divided by SQ_WAVES; SQ_WAVE_CYCLES counts 4-cycle units.", "",
+                  "| kernel | waves per launch | VALU / wave | LDS / wave | SALU / wave | wave cycles / wave "
+                  "| waiting / wave | LDS bank conflicts / wave | busy cycles / wave |",
+                  "|---|---|---|---|---|---|---|---|---|"]
+        for nm, w, v in sq_table(args.sq):
+            lines.append(f"| `{nm}` | {w:.0f} | {v['SQ_INSTS_VALU']:.0f} | {v['SQ_INSTS_LDS']:.0f} | "
+                         f"{v['SQ_INSTS_SALU']:.0f} | {v['SQ_WAVE_CYCLES']:.0f} | {v['SQ_WAIT_INST_ANY']:.0f} | "
+                         f"{v['SQ_LDS_BANK_CONFLICT']:.1f} | {v['SQ_BUSY_CYCLES']:.1f} |")
+        with open(args.out, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        print(args.out)
+        return
+    if not args.stats:
+        ap.error("--stats or --sq is required")
     ks = kernel_stats(args.stats)
     lines = [f"# {args.workload}: kernel times (rocprofv3)", "", "Commands:", "", "```"]
     lines += args.command.split("\\n")
