@@ -1310,8 +1310,10 @@ static void mag_rhs(mhd_handle *h, double t, double ddt, int km, bool smooth_xdi
     const double *X = h->x;
     mag_current(h, X);
     PROF_BEGIN(PT_RHSA);
+    // the flux tensor once per point into xdB (free until the xdif smoothing)
+    hipLaunchKernelGGL(km_flux, vol_grid(h), dim3(256), 0, h->stream, h->g, X, h->eq, h->xdB);
     hipLaunchKernelGGL(km_rhs_a, vol_grid(h), dim3(256), 0, h->stream, h->g, a, X, h->eq, h->cur, h->xresd, eta,
-                       h->xdA, h->eA, h->vel);
+                       h->xdB, h->xdA, h->eA, h->vel);
     double *E = mag_smooth<KIND_RADIAL, BC_AUX>(h, h->eA, h->eB, 3, 4, st);
     PROF_END(PT_RHSA);
     if (c.hall) {

@@ -255,25 +255,48 @@ __device__ inline void mg_flux(const double *__restrict__ X, const double *__res
     else { f = x2 * x4 / x1; gg = x3 * x4 / x1; h = x4 * x4 / x1 + (x8 - pi0); }
 }
 
-__global__ void km_rhs_a(Geo g, Mag a, const double *__restrict__ X, const double *__restrict__ eq,
-                         const double *__restrict__ cur, const double *__restrict__ xresd, double eta,
-                         double *__restrict__ XD, double *__restrict__ E, double *__restrict__ vel)
+// the momentum-flux tensor of mg_flux at every point, formed once for the
+// divergences of km_rhs_a: T22, T23, T24, T33, T34, T44 (T34 = x4 x3 / x1,
+// equal to mg_flux's x3 x4 / x1)
+__global__ void km_flux(Geo g, const double *__restrict__ X, const double *__restrict__ eq, double *__restrict__ T)
 {
     int i, j, k;
     long q;
     if (!mg_point(g, i, j, k, q)) return;
     const long S = g.S;
-    const double *pri = eq + 4 * S;
+    double f, gg, h, t33, t44;
+    mg_flux(X, eq + 4 * S, S, q, 1, f, gg, h);
+    T[q] = f; T[q + S] = gg; T[q + 2 * S] = h;
+    mg_flux(X, eq + 4 * S, S, q, 2, f, t33, h);
+    T[q + 3 * S] = t33; T[q + 4 * S] = h;
+    mg_flux(X, eq + 4 * S, S, q, 3, f, gg, t44);
+    T[q + 5 * S] = t44;
+}
+
+__global__ void km_rhs_a(Geo g, Mag a, const double *__restrict__ X, const double *__restrict__ eq,
+                         const double *__restrict__ cur, const double *__restrict__ xresd, double eta,
+                         const double *__restrict__ T, double *__restrict__ XD, double *__restrict__ E,
+                         double *__restrict__ vel)
+{
+    int i, j, k;
+    long q;
+    if (!mg_point(g, i, j, k, q)) return;
+    const long S = g.S;
     const double x1 = X[q], x2 = X[q + S], x3 = X[q + 2 * S], x4 = X[q + 3 * S];
     const double x5 = X[q + 4 * S], x6 = X[q + 5 * S], x7 = X[q + 6 * S];
     const double c1 = cur[q], c2 = cur[q + S], c3 = cur[q + 2 * S];
     if (mg_inner(g, i, j, k)) {
+        // flux(m) of t3dsdsc.f:2304-2347: (x2, x3, x4) for the mass, the
+        // tensor rows (T22, T23, T24), (T23, T33, T34), (T24, T34, T44)
+        const double *fb[4][3] = {{X + S, X + 2 * S, X + 3 * S},
+                                  {T, T + S, T + 2 * S},
+                                  {T + S, T + 3 * S, T + 4 * S},
+                                  {T + 2 * S, T + 4 * S, T + 5 * S}};
         double d[4];
         for (int m = 0; m < 4; m++) {
-            auto fx = [&](long p) { double f, gg, h; mg_flux(X, pri, S, p, m, f, gg, h); return f; };
-            auto fy = [&](long p) { double f, gg, h; mg_flux(X, pri, S, p, m, f, gg, h); return gg; };
-            auto fz = [&](long p) { double f, gg, h; mg_flux(X, pri, S, p, m, f, gg, h); return h; };
-            d[m] = -mg_dx(g, i, q, fx) - mg_dy(g, j, q, fy) - mg_dz(g, k, q, fz);
+            const double *fx = fb[m][0], *fy = fb[m][1], *fz = fb[m][2];
+            d[m] = -mg_dx(g, i, q, [&](long p) { return fx[p]; }) - mg_dy(g, j, q, [&](long p) { return fy[p]; }) -
+                   mg_dz(g, k, q, [&](long p) { return fz[p]; });
         }
         XD[q] = d[0];
         XD[q + S] = d[1] - xresd[q] + c2 * x7 - c3 * x6;
