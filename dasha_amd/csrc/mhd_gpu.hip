@@ -516,6 +516,7 @@ struct mhd_handle {
     ZTab zt3[2];                           // the same for three passes on 3 fields
     int zs3 = 1;                           // last three avrgh passes (+ E) in one launch
     int face12 = 1;                        // 12-row blocks for the face windows (periodic y)
+    int adj_overlap = 1;                   // adjoint: the Hall chain beside the E chain
     // stage recording of the adjoint's forward replay: when rec_x is set,
     // right() copies its state and current into slot rec_n and advances it
     double *rec_x = nullptr, *rec_c = nullptr;
@@ -1551,13 +1552,29 @@ static void adj_rhs(mhd_handle *h, AdjBufs &A, const double *X, const double *C,
     const double fmu = init ? 0.0 : c.fmu0, frho = init ? 0.0 : c.frho0, fpr = init ? 0.0 : c.fpr0;
     if (fmu != 0.0 || frho != 0.0 || fpr != 0.0)   // viscosity, diffusion (1585-1663)
         hipLaunchKernelGGL(k_adj_visc, grid, blk, 0, h->stream, g, a, X, fmu, frho, fpr, A.vb, xb);
+    const bool side = c.hall && h->adj_overlap && !h->prof;
     if (c.hall) {
         HCHK_V(hipMemcpyAsync(A.hb, A.eb, 3 * h->S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-        adj_smooth<KIND_EIGHTH>(h, A.hb, A.t3, 3, 5, BC_AUX);         // avrgh x5 (1424)
-        adj_bc(h, A.hb, 3, BC_AUX);                                    // bndryh (1423)
-        hipLaunchKernelGGL(k_adj_hall, grid, blk, 0, h->stream, g, A.hb, X, C, c.di, A.jb, xb);
+        // the Hall chain reads only its copy of E's adjoint: beside the E
+        // chain on the second stream (its temporary the free half of the
+        // state smoothing's pair), joined before the flux adjoint, which
+        // accumulates into the same J and x adjoints after it
+        double *hb = A.hb, *ht = side ? t8 : A.t3;
+        if (side) {
+            HCHK_V(hipEventRecord(h->aux_ev, h->stream));
+            HCHK_V(hipStreamWaitEvent(h->aux, h->aux_ev, 0));
+            std::swap(h->stream, h->aux);
+        }
+        adj_smooth<KIND_EIGHTH>(h, hb, ht, 3, 5, BC_AUX);              // avrgh x5 (1424)
+        adj_bc(h, hb, 3, BC_AUX);                                      // bndryh (1423)
+        hipLaunchKernelGGL(k_adj_hall, grid, blk, 0, h->stream, g, hb, X, C, c.di, A.jb, xb);
+        if (side) {
+            std::swap(h->stream, h->aux);
+            HCHK_V(hipEventRecord(h->aux_ev, h->aux));
+        }
     }
     adj_smooth<KIND_RADIAL>(h, A.eb, A.t3, 3, 4, BC_AUX);              // avrgE x4 (1389)
+    if (side) HCHK_V(hipStreamWaitEvent(h->stream, h->aux_ev, 0));
     hipLaunchKernelGGL(k_adj_ext_flux, grid, blk, 0, h->stream, g, a, A.eb, A.vb, X, C, eta, xb, A.jb, rb, A.acc);
     adj_smooth<KIND_RADIAL>(h, A.jb, A.t3, 3, 4, BC_AUX);              // avrgc x4 (2495)
     adj_bc(h, A.jb, 3, BC_AUX);                                        // bndryc (2493)
@@ -1809,7 +1826,13 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
     if (const char *e = getenv("MHD_ZORD")) h->zord = std::min(1, std::max(0, atoi(e)));
     if (const char *e = getenv("MHD_ZS3")) h->zs3 = atoi(e);
     if (const char *e = getenv("MHD_FACE12")) h->face12 = atoi(e);
+    // side-stream launches beside the main stream's (face windows beside the
+    // face-free ones, the parts of right part 2, the first Hall passes beside
+    // the smoothing of E): 1.4% faster at 512x256x512 and 256x128x256, 4.7%
+    // slower at 128x64x128, where the launches are short
+    h->overlap = h->S >= (1L << 22) ? 1 : 0;
     if (const char *e = getenv("MHD_OVERLAP")) h->overlap = atoi(e);
+    if (const char *e = getenv("MHD_ADJOVL")) h->adj_overlap = atoi(e);
     for (int c = 0; c < 2; c++) {
         int r = zs_tables(h, c ? 8 : 3, 2, h->zt[c]);
         if (r) { *out = h; return r; }

@@ -5,7 +5,8 @@ read by mhd_create): the point-parallel one-pass sweeps with a separate RK
 update and post-step filter (MHD_ZS=0, MHD_FUSE_RK=0), and the measured
 defaults swapped for their alternatives (two-pass instead of three-pass last
 avrgh launch, grid-order block tables, 64-plane chunks, the windowed right
-part 2 on small volumes, an unpadded field stride).  Each must give the
+part 2 on small volumes, an unpadded field stride, the side-stream launches
+on small volumes).  Each must give the
 oracle's step like the default path.
 """
 import os
@@ -20,6 +21,8 @@ pytestmark = pytest.mark.gpu
 KNOBS = {
     "one_pass_sweeps": {"MHD_ZS": "0", "MHD_FUSE_RK": "0"},
     "alternatives": {"MHD_ZS3": "0", "MHD_ZORD": "0", "MHD_ZKC": "64", "MHD_RB2": "1", "MHD_PAD": "0"},
+    # the side-stream launches (on by default from 2^22 points) on small volumes
+    "side_streams": {"MHD_OVERLAP": "1", "MHD_RB2": "1"},
 }
 CASES = {
     "wide_nonperiodic_hall": lambda: harris.case_small(130, 30, 40, periodic_y=False, seed=17),
