@@ -355,8 +355,11 @@ __device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restr
     constexpr bool TT = TYPE == BC_TT;          // post-step passes on x - eq (bndrytt fills)
     constexpr bool ADD = RKS == RKS_ADD;        // store the output plus a field
     constexpr int XT = TT ? BC_AUX : TYPE;      // x-face rule of the pass
-    __shared__ double s0[3][RB][TX];           // level 0 by plane mod 3
-    __shared__ double sl[NP][2][RB][TX];       // level p (1..NP) by plane parity
+    // the plane levels in LDS, column l of a row at index l + 1: the x and y
+    // neighbours of a point are read there (the two pad columns feed only
+    // the window's edge lanes, whose values no owned point uses)
+    __shared__ double s0[3][RB][TX + 2];       // level 0 by plane mod 3
+    __shared__ double sl[NP][2][RB][TX + 2];   // level p (1..NP) by plane parity
     __shared__ double szw[2][NZW];             // z weights of the chunk's planes
     const int lane = threadIdx.x % TX;
     const int r = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);   // uniform over the wavefront
@@ -489,7 +492,7 @@ __device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restr
                 if (RAD) pc[u] = fld(cbase, cq0 + csz * tz);
             }
             q[0][0] = q[0][1]; q[0][1] = q[0][2]; q[0][2] = v0;
-            s0[(t + 3) % 3][r][lane] = v0;
+            s0[(t + 3) % 3][r][lane + 1] = v0;
             if (RAD) { cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cv; }
             __syncthreads();
             #pragma unroll
@@ -504,17 +507,19 @@ __device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restr
                 const bool kin = k >= 2 && k <= g.NZ - 3;
                 const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
                 const double f0 = q[p - 1][1];              // level p-1 at plane k
-                double ym, yp;
+                double ym, yp, xm, xp;
                 if (p == 1) {
                     const int b = (k + 3) % 3;
-                    ym = s0[b][rm][lane];
-                    yp = s0[b][rp][lane];
+                    ym = s0[b][rm][lane + 1];
+                    yp = s0[b][rp][lane + 1];
+                    xm = s0[b][r][lane];
+                    xp = s0[b][r][lane + 2];
                 } else {
-                    ym = sl[p - 2][k & 1][rm][lane];
-                    yp = sl[p - 2][k & 1][rp][lane];
+                    ym = sl[p - 2][k & 1][rm][lane + 1];
+                    yp = sl[p - 2][k & 1][rp][lane + 1];
+                    xm = sl[p - 2][k & 1][r][lane];
+                    xp = sl[p - 2][k & 1][r][lane + 2];
                 }
-                const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
-                const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
                 double v = 0.0;
                 if (pin && kin) {
                     const double c = KIND == KIND_EIGHTH ? 0.125 : cr[RAD ? 2 - p : 0];
@@ -525,14 +530,14 @@ __device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restr
                 if (xft) v = row_xrule_rl<XT>(g, i, i0, vx && yin, v);
                 if (yex && kin) {
                     // image rows: 0 on the outer row, half of row 2 (NY-3) on the next
-                    sl[p - 1][k & 1][r][lane] = v;
+                    sl[p - 1][k & 1][r][lane + 1] = v;
                     __syncthreads();
                     if (img) {
-                        v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sl[p - 1][k & 1][rsrc][lane];
-                        sl[p - 1][k & 1][r][lane] = v;
+                        v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sl[p - 1][k & 1][rsrc][lane + 1];
+                        sl[p - 1][k & 1][r][lane + 1] = v;
                     }
                 } else if (p < NP) {
-                    sl[p - 1][k & 1][r][lane] = v;
+                    sl[p - 1][k & 1][r][lane + 1] = v;
                 }
                 // z faces above the interior: plane NZ-2 is half of NZ-3, NZ-1 is 0
                 if (k == g.NZ - 2) v = 0.5 * q[p][2];
@@ -589,7 +594,7 @@ __device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restr
                 if (RAD) pc[u] = fld(cbase, cq0 + csz * tz);
             }
             q[0][u] = v0;
-            s0[(t + 3) % 3][r][lane] = v0;
+            s0[(t + 3) % 3][r][lane + 1] = v0;
             if (RAD) cr[u] = cv;
             __syncthreads();
             #pragma unroll
@@ -598,17 +603,19 @@ __device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restr
                 const int k = t - p;
                 const int rm = r > 0 ? r - 1 : 0, rp = r < RB - 1 ? r + 1 : RB - 1;
                 const double zm = q[p - 1][(u + 1) % 3], f0 = q[p - 1][(u + 2) % 3], zp = q[p - 1][u];
-                double ym, yp;
+                double ym, yp, xm, xp;
                 if (p == 1) {
                     const int b = (k + 3) % 3;
-                    ym = s0[b][rm][lane];
-                    yp = s0[b][rp][lane];
+                    ym = s0[b][rm][lane + 1];
+                    yp = s0[b][rp][lane + 1];
+                    xm = s0[b][r][lane];
+                    xp = s0[b][r][lane + 2];
                 } else {
-                    ym = sl[p - 2][k & 1][rm][lane];
-                    yp = sl[p - 2][k & 1][rp][lane];
+                    ym = sl[p - 2][k & 1][rm][lane + 1];
+                    yp = sl[p - 2][k & 1][rp][lane + 1];
+                    xm = sl[p - 2][k & 1][r][lane];
+                    xp = sl[p - 2][k & 1][r][lane + 2];
                 }
-                const double xm = __shfl(f0, lane > 0 ? lane - 1 : 0, TX);
-                const double xp = __shfl(f0, lane < TX - 1 ? lane + 1 : TX - 1, TX);
                 const double c = KIND == KIND_EIGHTH ? 0.125 : cr[RAD ? (u + 3 - p) % 3 : 0];
                 const double s = difw(xm, f0, xp, wpx, wmx) + difw(ym, f0, yp, wpy, wmy) +
                                  difw(zm, f0, zp, szw[0][k - kz0], szw[1][k - kz0]);
@@ -616,18 +623,18 @@ __device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restr
                 if (FACES) {
                     if (xft) v = row_xrule_rl<XT>(g, i, i0, vx && yin, v);
                     if (yex) {
-                        sl[p - 1][k & 1][r][lane] = v;
+                        sl[p - 1][k & 1][r][lane + 1] = v;
                         __syncthreads();
                         if (img) {
-                            v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sl[p - 1][k & 1][rsrc][lane];
-                            sl[p - 1][k & 1][r][lane] = v;
+                            v = (j == 0 || j == g.NY - 1) ? 0.0 : 0.5 * sl[p - 1][k & 1][rsrc][lane + 1];
+                            sl[p - 1][k & 1][r][lane + 1] = v;
                         }
                         if (TT && i0 < 2 && img) v = tt_left(f0, v);
                     } else if (p < NP) {
-                        sl[p - 1][k & 1][r][lane] = v;
+                        sl[p - 1][k & 1][r][lane + 1] = v;
                     }
                 } else if (p < NP) {
-                    sl[p - 1][k & 1][r][lane] = v;
+                    sl[p - 1][k & 1][r][lane + 1] = v;
                 }
                 q[p][u] = v;
                 if (p == NP && own && k >= kb && k < ke) put_with(eo, v, o1, o2);
