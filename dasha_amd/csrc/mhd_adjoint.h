@@ -525,16 +525,16 @@ __global__ void k_adj_rk(long n, int stage, double dt, const double *__restrict_
     if (q >= n) return;
     if (stage == 3) {
         xbm[q] = xb[q];
-        d[q] = xb[q] * dt / 6.;
+        d[q] = div6(xb[q] * dt);
     } else if (stage == 2) {
         xbf[q] = t[q];
-        d[q] = t[q] * dt + xbm[q] * dt / 3.;
+        d[q] = t[q] * dt + div3(xbm[q] * dt);
     } else if (stage == 1) {
         xbf[q] += t[q];
-        d[q] = t[q] * dt / 2. + xbm[q] * dt / 3.;
+        d[q] = t[q] * dt / 2. + div3(xbm[q] * dt);
     } else {
         xbf[q] += t[q] + xbm[q];
-        d[q] = t[q] * dt / 2. + xbm[q] * dt / 6.;
+        d[q] = t[q] * dt / 2. + div6(xbm[q] * dt);
     }
     t[q] = 0.0;
 }

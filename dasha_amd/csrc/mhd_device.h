@@ -224,6 +224,11 @@ __device__ inline double divc(double a, double b, double y)
     return fma(fma(-q0, b, a), y, q0);
 }
 
+// a / 3 and a / 6 of the RK4 register updates the same way, from the rounded
+// reciprocals 1/3 and 1/6
+__device__ inline double div3(double a) { return divc(a, 3.0, 0x1.5555555555555p-2); }
+__device__ inline double div6(double a) { return divc(a, 6.0, 0x1.5555555555555p-3); }
+
 // x-left extrapolation of the auxiliary fills (bndryc/E/h, t3dsdsc.f:2569-2572
 // and the like): f(4) = (ax f5 + bx f6 + cx f7) / (ax + bx + cx), then f(3)
 __device__ inline double xleft1(const Geo &g, double p2, double p3, double p4)

@@ -42,16 +42,16 @@ __global__ void k_rk(long n8, int stage, const double *__restrict__ dtp, const d
     if (q >= n8) return;
     double dt = *dtp, d = XD[q];
     if (stage == 0) {
-        st(XM + q, XF[q] + d * dt / 6.);
+        st(XM + q, XF[q] + div6(d * dt));
         st(X + q, XF[q] + d * dt / 2.);
     } else if (stage == 1) {
-        st(XM + q, XM[q] + d * dt / 3.);
+        st(XM + q, XM[q] + div3(d * dt));
         st(X + q, XF[q] + d * dt / 2.);
     } else if (stage == 2) {
-        st(XM + q, XM[q] + d * dt / 3.);
+        st(XM + q, XM[q] + div3(d * dt));
         st(X + q, XF[q] + d * dt);
     } else {
-        st(X + q, XM[q] + d * dt / 6.);
+        st(X + q, XM[q] + div6(d * dt));
     }
 }
 

@@ -187,16 +187,16 @@ __device__ inline void rk_point(const RKArgs &r, long q, double d, double dt)
 {
     if (RKS == 0) {
         double xf = r.XF[q];
-        st(r.XM + q, xf + d * dt / 6.);
+        st(r.XM + q, xf + div6(d * dt));
         st(r.X + q, xf + d * dt / 2.);
     } else if (RKS == 1) {
-        st(r.XM + q, r.XM[q] + d * dt / 3.);
+        st(r.XM + q, r.XM[q] + div3(d * dt));
         st(r.X + q, r.XF[q] + d * dt / 2.);
     } else if (RKS == 2) {
-        st(r.XM + q, r.XM[q] + d * dt / 3.);
+        st(r.XM + q, r.XM[q] + div3(d * dt));
         st(r.X + q, r.XF[q] + d * dt);
     } else {
-        st(r.X + q, r.XM[q] + d * dt / 6.);
+        st(r.X + q, r.XM[q] + div6(d * dt));
     }
 }
 
@@ -442,19 +442,19 @@ __device__ __forceinline__ void zsmooth_body(const Geo &g, const double *__restr
         };
         auto put_with = [&](unsigned e, double v, double o1, double o2) {
             if (RKS < 0) st(fldp(bo, e), ((TT && eqm) || ADD) ? v + o1 : v);
-            else if (RKS == 0) { st(fldp(xmo, e), o1 + v * dt / 6.); st(fldp(xo, e), o1 + v * dt / 2.); }
-            else if (RKS == 1) { st(fldp(xmo, e), o1 + v * dt / 3.); st(fldp(xo, e), o2 + v * dt / 2.); }
-            else if (RKS == 2) { st(fldp(xmo, e), o1 + v * dt / 3.); st(fldp(xo, e), o2 + v * dt); }
-            else if (RKS == 3) st(fldp(xo, e), o1 + v * dt / 6.);
-            else { const double xn = o1 + v * dt / 6.; st(fldp(xmo, e), eqm ? xn - o2 : xn); }
+            else if (RKS == 0) { st(fldp(xmo, e), o1 + div6(v * dt)); st(fldp(xo, e), o1 + v * dt / 2.); }
+            else if (RKS == 1) { st(fldp(xmo, e), o1 + div3(v * dt)); st(fldp(xo, e), o2 + v * dt / 2.); }
+            else if (RKS == 2) { st(fldp(xmo, e), o1 + div3(v * dt)); st(fldp(xo, e), o2 + v * dt); }
+            else if (RKS == 3) st(fldp(xo, e), o1 + div6(v * dt));
+            else { const double xn = o1 + div6(v * dt); st(fldp(xmo, e), eqm ? xn - o2 : xn); }
         };
         auto put = [&](unsigned e, double v) {
             if (RKS < 0) st(fldp(bo, e), ((TT && eqm) || ADD) ? v + fld(ADD ? addm : eqm, e) : v);
-            else if (RKS == 0) { const double f = fld(xf, e); st(fldp(xmo, e), f + v * dt / 6.); st(fldp(xo, e), f + v * dt / 2.); }
-            else if (RKS == 1) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt / 2.); }
-            else if (RKS == 2) { st(fldp(xmo, e), fld(xmo, e) + v * dt / 3.); st(fldp(xo, e), fld(xf, e) + v * dt); }
-            else if (RKS == 3) st(fldp(xo, e), fld(xmo, e) + v * dt / 6.);
-            else { const double xn = fld(xmo, e) + v * dt / 6.; st(fldp(xmo, e), eqm ? xn - fld(eqm, e) : xn); }
+            else if (RKS == 0) { const double f = fld(xf, e); st(fldp(xmo, e), f + div6(v * dt)); st(fldp(xo, e), f + v * dt / 2.); }
+            else if (RKS == 1) { st(fldp(xmo, e), fld(xmo, e) + div3(v * dt)); st(fldp(xo, e), fld(xf, e) + v * dt / 2.); }
+            else if (RKS == 2) { st(fldp(xmo, e), fld(xmo, e) + div3(v * dt)); st(fldp(xo, e), fld(xf, e) + v * dt); }
+            else if (RKS == 3) st(fldp(xo, e), fld(xmo, e) + div6(v * dt));
+            else { const double xn = fld(xmo, e) + div6(v * dt); st(fldp(xmo, e), eqm ? xn - fld(eqm, e) : xn); }
         };
         // x-left columns of a row from the level held by lanes 2, 3, 4 (every
         // lane of the wavefront calls; the window holds the left face)
