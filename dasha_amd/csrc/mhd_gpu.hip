@@ -504,6 +504,8 @@ struct mhd_handle {
     // profile), v = m / rho, the inflow profiles and face data
     bool mag = false;
     double *rr = nullptr, *chtab = nullptr, *vel = nullptr, *mgeo = nullptr;
+    int *mrow = nullptr;                   // masked column range per row, when the producers mask
+    int fill_split = 0;                    // y and z rules of the fills as two launches
     const double *zd = nullptr, *yd = nullptr, *zd2 = nullptr, *yd2 = nullptr, *face = nullptr;
     bool have_eq = false;
     int kc = 16;                           // z planes per tile chunk
@@ -1246,6 +1248,7 @@ static Mag mag_args(const mhd_handle *h)
     a.partial = h->cfg.partial_e != 0;
     a.zd = h->zd; a.yd = h->yd; a.zd2 = h->zd2; a.yd2 = h->yd2;
     a.fb = h->face;
+    a.mrow = h->mrow;
     return a;
 }
 
@@ -1264,17 +1267,23 @@ static Stage mag_stage(const mhd_handle *h, double t, double ddt, int km)
 
 static dim3 vol_grid(const mhd_handle *h) { return dim3((unsigned)blocks((long)h->NX * h->NY * h->NZ, 256)); }
 
-// a boundary routine: mask (whole volume), then x, y, z rules
+// a boundary routine: mask (by the producer of A as it stores, else a
+// launch over the volume), then the x rules, then the y and z rules
 template <int TYPE>
 static void mag_fill(mhd_handle *h, double *A, int nf, const Stage &s)
 {
     const Mag a = mag_args(h);
     const Geo &g = h->g;
-    if (a.mask) hipLaunchKernelGGL(km_mask, vol_grid(h), dim3(256), 0, h->stream, g, a, A, nf);
+    if (a.mask && !h->mrow) hipLaunchKernelGGL(km_mask, vol_grid(h), dim3(256), 0, h->stream, g, a, A, nf);
     hipLaunchKernelGGL((km_fill_x<TYPE>), dim3(blocks((long)g.NY * g.NZ, 256)), dim3(256), 0, h->stream, g, a, s,
                        h->eq, A, nf);
-    hipLaunchKernelGGL((km_fill_y<TYPE>), dim3(blocks((long)g.NX * g.NZ, 256)), dim3(256), 0, h->stream, g, A, nf);
-    hipLaunchKernelGGL((km_fill_z<TYPE>), dim3(blocks((long)g.NX * g.NY, 256)), dim3(256), 0, h->stream, g, A, nf);
+    if (h->fill_split) {
+        hipLaunchKernelGGL((km_fill_y<TYPE>), dim3(blocks((long)g.NX * g.NZ, 256)), dim3(256), 0, h->stream, g, A, nf);
+        hipLaunchKernelGGL((km_fill_z<TYPE>), dim3(blocks((long)g.NX * g.NY, 256)), dim3(256), 0, h->stream, g, A, nf);
+    } else {
+        const long n = (long)g.NX * (g.NZ - 4) + (long)g.NX * g.NY;
+        hipLaunchKernelGGL((km_fill_yz<TYPE>), dim3(blocks(n, 256)), dim3(256), 0, h->stream, g, A, nf);
+    }
 }
 
 // kk sweeps of nf fields, each followed by the boundary routine TYPE;
@@ -1283,7 +1292,7 @@ template <int KIND, int TYPE>
 static double *mag_smooth(mhd_handle *h, double *a, double *b, int nf, int kk, const Stage &s)
 {
     for (int p = 0; p < kk; p++) {
-        hipLaunchKernelGGL((km_sweep<KIND>), vol_grid(h), dim3(256), 0, h->stream, h->g, a, b, nf);
+        hipLaunchKernelGGL((km_sweep<KIND>), vol_grid(h), dim3(256), 0, h->stream, h->g, a, b, nf, h->mrow);
         mag_fill<TYPE>(h, b, nf, s);
         std::swap(a, b);
     }
@@ -1295,7 +1304,7 @@ static void mag_current(mhd_handle *h, const double *X)
 {
     const Stage none = mag_stage(h, 0.0, 1.0, 1);
     PROF_BEGIN(PT_CURL);
-    hipLaunchKernelGGL(km_curl, vol_grid(h), dim3(256), 0, h->stream, h->g, X, h->eq, h->cA);
+    hipLaunchKernelGGL(km_curl, vol_grid(h), dim3(256), 0, h->stream, h->g, X, h->eq, h->cA, h->mrow);
     mag_fill<BC_AUX>(h, h->cA, 3, none);
     h->cur = mag_smooth<KIND_RADIAL, BC_AUX>(h, h->cA, h->cB, 3, 4, none);
     PROF_END(PT_CURL);
@@ -1767,6 +1776,33 @@ int mhd_create(const mhd_config *cfg, const double *xx, const double *yy, const 
         }
         HCHK(hipMemcpy(h->rr, rr.data(), fb, hipMemcpyHostToDevice));
         HCHK(hipMemset(h->vel, 0, 3 * fb));
+        if (cfg->magnetosphere) {
+            // the masked points of each row (j, k) as one column range: rr is
+            // monotone in |x| along a row; the producers apply the mask when
+            // every masked point is interior and each row's set is one range
+            std::vector<int> mr(2 * (size_t)NY * NZ, 0);
+            bool ok = true;
+            for (int k = 0; k < NZ && ok; k++)
+                for (int j = 0; j < NY && ok; j++) {
+                    int lo = -1, hi = -1;
+                    for (int i = 0; i < NX; i++) {
+                        if (!(rr[IDX(g, i, j, k)] <= cfg->rcr)) continue;
+                        if (lo < 0) lo = i;
+                        else if (hi != i) ok = false;
+                        hi = i + 1;
+                    }
+                    if (lo < 0) continue;
+                    if (lo < 2 || hi > NX - 2 || j < 2 || j > NY - 3 || k < 2 || k > NZ - 3) ok = false;
+                    mr[2 * ((size_t)k * NY + j)] = lo;
+                    mr[2 * ((size_t)k * NY + j) + 1] = hi;
+                }
+            if (const char *e = getenv("MHD_MAGMASK")) ok = ok && atoi(e) != 0;
+            if (ok) {
+                HCHK(hipMalloc(&h->mrow, mr.size() * sizeof(int)));
+                HCHK(hipMemcpy(h->mrow, mr.data(), mr.size() * sizeof(int), hipMemcpyHostToDevice));
+            }
+        }
+        if (const char *e = getenv("MHD_FILLSPLIT")) h->fill_split = atoi(e);
         if (cfg->radial_hall) {
             ch.assign(h->S, 0.0);
             for (size_t q = 0; q < ch.size(); q++)
@@ -3116,6 +3152,7 @@ void mhd_destroy(mhd_handle *h)
                     h->eA, h->eB, h->hA, h->hB, h->red, h->dtv, h->crad, h->rr, h->chtab, h->vel, h->mgeo};
     for (auto p : ps)
         if (p) hipFree(p);
+    if (h->mrow) hipFree(h->mrow);
     for (auto &zc : h->zt)
         for (auto &t : zc)
             if (t.d) hipFree(t.d);

@@ -7,9 +7,12 @@
 // field of the field perturbation plus the curl(v x B0) dipole terms
 // (1370-1386, 1460-1548).  These kernels follow the reference's routine
 // structure one sweep per launch: a producer writes the interior, then a fill
-// runs mask (whole volume), x rules (one thread per row), y rules (per x-z
-// column) and z rules (per x-y column) as separate launches, so the rules see
-// each other's results in the reference's x, y, z order.  Non-periodic y only.
+// runs the x rules (one thread per row) and the y and z rules (one launch) as
+// separate launches, so the rules see each other's results in the reference's
+// x, y, z order.  The inner-sphere mask at the start of every fill is applied
+// by the producers as they store (the masked points are interior points; the
+// host falls back to a mask launch per fill where they are not).
+// Non-periodic y only.
 // Every non-interior point of a buffer is rewritten by the fill that follows
 // its producer; producers write interior points only, sweeps carry the other
 // points of their input along.
@@ -32,7 +35,16 @@ struct Mag {
     const double *zd, *yd;     // inflow profiles tanh((z+0.6)/0.15) - tanh((z-0.6)/0.15), y alike (0.4)
     const double *zd2, *yd2;   // the same with 0.2 (z) and 0.3 (y) for the velocity
     const double *fb;          // face data bxinfc, byinfc, bzinfc: 3 x NZ*NY, [k*NY + j]
+    const int *mrow;           // masked columns [lo, hi) of row (j, k) at [2 (k*NY + j)], or null
 };
+
+// the producers' store of the mask: the point is inside the sphere
+__device__ inline bool mg_masked(const Geo &g, const int *mrow, int i, int j, int k)
+{
+    if (!mrow) return false;
+    const int *r = mrow + 2 * ((long)k * g.NY + j);
+    return i >= r[0] && i < r[1];
+}
 
 // arguments t, ddt, km of the running right(): the time factors of the drive
 // (computed on the host), the reference state of the quotient and the stage
@@ -190,6 +202,61 @@ __global__ void km_fill_z(Geo g, double *__restrict__ A, int nf)
     }
 }
 
+// y and z rules in one launch.  Threads [0, NX (NZ-4)) apply the y rules on
+// the interior planes; the rest, one per x-y column, the z rules, taking
+// plane 2 (NZ-3) of an image row as the y rule makes it from row 2 (NY-3)
+// (0 on the outer rows).  The values are those of the y launch followed by
+// the z launch: the y rule's values on the z-face planes of the image rows
+// are rewritten by the z rules there, and no thread reads what another
+// writes (t3dsdsc.f:1800-1861, 2108-2168, 2592-2644)
+template <int TYPE>
+__global__ void km_fill_yz(Geo g, double *__restrict__ A, int nf)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ny = (long)g.NX * (g.NZ - 4);
+    const long sy = g.sy, sz = g.sz;
+    if (t < ny) {
+        const int i = (int)(t % g.NX), k = 2 + (int)(t / g.NX);
+        if (TYPE == BC_TT && i < 2) return;
+        const long q0 = IDX(g, i, 0, k);
+        const int L = g.NY - 1;
+        for (int m = 0; m < nf; m++) {
+            double *f = A + q0 + m * g.S;
+            f[0] = 0.;
+            f[sy] = 0.5 * f[2 * sy];
+            f[L * sy] = 0.;
+            f[(L - 1) * sy] = 0.5 * f[(L - 2) * sy];
+        }
+        return;
+    }
+    const long u = t - ny;
+    if (u >= (long)g.NX * g.NY) return;
+    const int i = (int)(u % g.NX), j = (int)(u / g.NX);
+    if (TYPE == BC_TT && i < 2) return;
+    const long q0 = IDX(g, i, j, 0);
+    const int L = g.NZ - 1, M = g.NY - 1;
+    const bool outer = j == 0 || j == M, inner = j == 1 || j == M - 1;
+    const long qs = IDX(g, i, j == 1 ? 2 : M - 2, 0);
+    for (int m = 0; m < nf; m++) {
+        double *f = A + q0 + m * g.S;
+        double p2, pl;
+        if (outer) {
+            p2 = pl = 0.;
+        } else if (inner) {
+            const double *s = A + qs + m * g.S;
+            p2 = 0.5 * s[2 * sz];
+            pl = 0.5 * s[(L - 2) * sz];
+        } else {
+            p2 = f[2 * sz];
+            pl = f[(L - 2) * sz];
+        }
+        f[0] = 0.;
+        f[sz] = 0.5 * p2;
+        f[L * sz] = 0.;
+        f[(L - 1) * sz] = 0.5 * pl;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // one smoothing sweep of nf fields A -> B on the interior (t3dsdsc.f:5630-5680):
 // KIND_RADIAL, KIND_EIGHTH, or KIND_STATE (B components 4..6 take cbx).
@@ -198,7 +265,7 @@ __global__ void km_fill_z(Geo g, double *__restrict__ A, int nf)
 // rewrite them (bndrytt keeps those x-face values).
 // ---------------------------------------------------------------------------
 template <int KIND>
-__global__ void km_sweep(Geo g, const double *__restrict__ A, double *__restrict__ B, int nf)
+__global__ void km_sweep(Geo g, const double *__restrict__ A, double *__restrict__ B, int nf, const int *mrow)
 {
     int i, j, k;
     long q;
@@ -208,6 +275,10 @@ __global__ void km_sweep(Geo g, const double *__restrict__ A, double *__restrict
         return;
     }
     const long sy = g.sy, sz = g.sz;
+    if (mg_masked(g, mrow, i, j, k)) {
+        for (int m = 0; m < nf; m++) B[q + m * g.S] = 0.0;
+        return;
+    }
     for (int m = 0; m < nf; m++) {
         const double *f = A + m * g.S;
         double c;
@@ -225,12 +296,17 @@ __global__ void km_sweep(Geo g, const double *__restrict__ A, double *__restrict
 // ---------------------------------------------------------------------------
 // current (t3dsdsc.f:2465-2491): J = curl(B - B_i) on the interior
 // ---------------------------------------------------------------------------
-__global__ void km_curl(Geo g, const double *__restrict__ X, const double *__restrict__ eq, double *__restrict__ cur)
+__global__ void km_curl(Geo g, const double *__restrict__ X, const double *__restrict__ eq, double *__restrict__ cur,
+                        const int *mrow)
 {
     int i, j, k;
     long q;
     if (!mg_point(g, i, j, k, q) || !mg_inner(g, i, j, k)) return;
     const long S = g.S;
+    if (mg_masked(g, mrow, i, j, k)) {
+        cur[q] = cur[q + S] = cur[q + 2 * S] = 0.0;
+        return;
+    }
     auto u = [&](long p) { return X[p + 4 * S] - eq[p + S]; };
     auto v = [&](long p) { return X[p + 5 * S] - eq[p + 2 * S]; };
     auto w = [&](long p) { return X[p + 6 * S] - eq[p + 3 * S]; };
@@ -298,10 +374,14 @@ __global__ void km_rhs_a(Geo g, Mag a, const double *__restrict__ X, const doubl
             d[m] = -mg_dx(g, i, q, [&](long p) { return fx[p]; }) - mg_dy(g, j, q, [&](long p) { return fy[p]; }) -
                    mg_dz(g, k, q, [&](long p) { return fz[p]; });
         }
-        XD[q] = d[0];
-        XD[q + S] = d[1] - xresd[q] + c2 * x7 - c3 * x6;
-        XD[q + 2 * S] = d[2] - xresd[q + S] + c3 * x5 - c1 * x7;
-        XD[q + 3 * S] = d[3] - xresd[q + 2 * S] + c1 * x6 - c2 * x5;
+        if (mg_masked(g, a.mrow, i, j, k)) {
+            XD[q] = XD[q + S] = XD[q + 2 * S] = XD[q + 3 * S] = 0.0;
+        } else {
+            XD[q] = d[0];
+            XD[q + S] = d[1] - xresd[q] + c2 * x7 - c3 * x6;
+            XD[q + 2 * S] = d[2] - xresd[q + S] + c3 * x5 - c1 * x7;
+            XD[q + 3 * S] = d[3] - xresd[q + 2 * S] + c1 * x6 - c2 * x5;
+        }
     }
     vel[q] = x2 / x1;
     vel[q + S] = x3 / x1;
@@ -326,6 +406,10 @@ __global__ void km_hall(Geo g, Mag a, const double *__restrict__ X, const double
     long q;
     if (!mg_point(g, i, j, k, q) || !mg_inner(g, i, j, k)) return;
     const long S = g.S;
+    if (mg_masked(g, a.mrow, i, j, k)) {
+        H[q] = H[q + S] = H[q + 2 * S] = 0.0;
+        return;
+    }
     const double cht = a.cht ? a.cht[q] : di;
     const double x1 = X[q], x5 = X[q + 4 * S], x6 = X[q + 5 * S], x7 = X[q + 6 * S];
     const double c1 = cur[q], c2 = cur[q + S], c3 = cur[q + 2 * S];
@@ -360,6 +444,10 @@ __global__ void km_rhs_b(Geo g, Mag a, const double *__restrict__ X, const doubl
     long q;
     if (!mg_point(g, i, j, k, q) || !mg_inner(g, i, j, k)) return;
     const long S = g.S;
+    if (mg_masked(g, a.mrow, i, j, k)) {
+        XD[q + 4 * S] = XD[q + 5 * S] = XD[q + 6 * S] = XD[q + 7 * S] = 0.0;
+        return;
+    }
     auto e1 = [&](long p) { return E[p]; };
     auto e2 = [&](long p) { return E[p + S]; };
     auto e3 = [&](long p) { return E[p + 2 * S]; };

@@ -121,3 +121,22 @@ def test_magnetosphere_mode_refusals():
     bad.periodic_y = True
     with pytest.raises(MHDError):
         MHDStep(bad)
+
+
+@pytest.mark.parametrize("name", ["defaults", "partial_e"])
+def test_fill_forms_agree(name, monkeypatch):
+    """the fills with the mask applied by the producers and the y and z rules
+    in one launch give the same state, bit for bit, as a mask launch per fill
+    and separate y and z launches (the reference's routine order)"""
+    from dasha_amd.mhd import MHDStep
+    states = []
+    for mask, split in (("1", "0"), ("0", "1")):
+        monkeypatch.setenv("MHD_MAGMASK", mask)
+        monkeypatch.setenv("MHD_FILLSPLIT", split)
+        print(f"[magneto] {name} producer mask={mask} split fills={split}", flush=True)
+        g = MHDStep(CASES[name]())
+        g.run(3, time=20.0)
+        states.append((g.get_state(), g.get_current()))
+        g.close()
+    assert np.array_equal(states[0][0], states[1][0])
+    assert np.array_equal(states[0][1], states[1][1])
