@@ -1260,6 +1260,9 @@ static Stage mag_stage(const mhd_handle *h, double t, double ddt, int km)
     s.sq = sqrt(t * t * t * 7.25 * 7.25 * 7.25);
     s.de = exp(t * 7.25 * (double)0.0069f) + 1.;
     s.ddt = ddt;
+    s.rde = 1.0 / s.de;
+    s.den = ddt + 1.e-9;
+    s.rden = 1.0 / s.den;
     s.ref = km == 1 ? h->xfold : h->xm;
     s.x = h->x;
     return s;
@@ -1275,13 +1278,14 @@ static void mag_fill(mhd_handle *h, double *A, int nf, const Stage &s)
     const Mag a = mag_args(h);
     const Geo &g = h->g;
     if (a.mask && !h->mrow) hipLaunchKernelGGL(km_mask, vol_grid(h), dim3(256), 0, h->stream, g, a, A, nf);
-    hipLaunchKernelGGL((km_fill_x<TYPE>), dim3(blocks((long)g.NY * g.NZ, 256)), dim3(256), 0, h->stream, g, a, s,
-                       h->eq, A, nf);
+    const long rows = (long)g.NY * g.NZ;
+    const long nx = rows * nf + (TYPE == BC_STATE && a.inflow ? 2 * rows : 0);
+    hipLaunchKernelGGL((km_fill_x<TYPE>), dim3(blocks(nx, 256)), dim3(256), 0, h->stream, g, a, s, h->eq, A, nf);
     if (h->fill_split) {
         hipLaunchKernelGGL((km_fill_y<TYPE>), dim3(blocks((long)g.NX * g.NZ, 256)), dim3(256), 0, h->stream, g, A, nf);
         hipLaunchKernelGGL((km_fill_z<TYPE>), dim3(blocks((long)g.NX * g.NY, 256)), dim3(256), 0, h->stream, g, A, nf);
     } else {
-        const long n = (long)g.NX * (g.NZ - 4) + (long)g.NX * g.NY;
+        const long n = ((long)g.NX * (g.NZ - 4) + (long)g.NX * g.NY) * nf;
         hipLaunchKernelGGL((km_fill_yz<TYPE>), dim3(blocks(n, 256)), dim3(256), 0, h->stream, g, A, nf);
     }
 }

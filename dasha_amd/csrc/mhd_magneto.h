@@ -54,6 +54,8 @@ struct Stage {
     double sq;                 // sqrt(t^3 7.25^3)
     double de;                 // exp(7.25 t 0.0069f) + 1
     double ddt;
+    double rde;                // 1 / de, and ddt + 1e-9 with its reciprocal: the
+    double den, rden;          // drive's quotients through divc
     const double *ref;         // xfold (km = 1) or xm (km = 2)
     double *x;
 };
@@ -121,46 +123,51 @@ __device__ inline void mg_inflow(const Geo &g, const Mag &a, const Stage &s, con
     v[1] = MF32(0.8) * v[0] * 0.5 * s.tt * 0.5 * zd2 * 0.5 * yd2;
     v[2] = 0.;
     v[3] = 0.;
-    v[4] = eq[q + S] + a.fb[f] * MF32(0.0017) * s.sq / s.de;
-    v[5] = eq[q + 2 * S] + a.fb[f + F] * MF32(0.0017) * s.sq / s.de;
-    v[6] = eq[q + 3 * S] + a.fb[f + 2 * F] * MF32(0.0017) * s.sq / s.de;
+    v[4] = eq[q + S] + divc(a.fb[f] * MF32(0.0017) * s.sq, s.de, s.rde);
+    v[5] = eq[q + 2 * S] + divc(a.fb[f + F] * MF32(0.0017) * s.sq, s.de, s.rde);
+    v[6] = eq[q + 3 * S] + divc(a.fb[f + 2 * F] * MF32(0.0017) * s.sq, s.de, s.rde);
     v[7] = eq[q + 4 * S] + MF32(0.2) * 0.5 * s.tt * 0.5 * zd * 0.5 * yd;
     for (int m = 0; m < 8; m++) {
         s.x[q + m * S] = v[m];
-        XD[q + m * S] = (v[m] - s.ref[q + m * S]) / (s.ddt + 1.e-9);
+        XD[q + m * S] = divc(v[m] - s.ref[q + m * S], s.den, s.rden);
     }
 }
 
 // x rules over every row and plane: BC_STATE (bndry: 0 / half at x-right,
 // the mirrored rule or the driven inflow at x-left), BC_AUX and BC_TT
-// (bndryc/h/E, bndrytt: one-sided extrapolation at x-left)
+// (bndryc/h/E, bndrytt: one-sided extrapolation at x-left).  One thread per
+// row and field; with the inflow, one more per row and driven column (the
+// driven points read only eq, the reference state and the drive, and no
+// rule of this launch writes columns 0 and 1 then)
 template <int TYPE>
 __global__ void km_fill_x(Geo g, Mag a, Stage s, const double *__restrict__ eq, double *__restrict__ A, int nf)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)g.NY * g.NZ) return;
-    const int j = (int)(t % g.NY), k = (int)(t / g.NY);
-    const long q0 = IDX(g, 0, j, k);
-    const int L = g.NX - 1;
-    for (int m = 0; m < nf; m++) {
-        double *f = A + q0 + m * g.S;
-        if (TYPE == BC_STATE) {
-            if (!a.inflow) {
-                f[0] = 0.;
-                f[1] = 0.5 * f[2];
-            }
-        } else {
-            const double v1 = (g.ax4 * f[2] + g.bx4 * f[3] + g.cx4 * f[4]) / g.dx4;
-            f[1] = v1;
-            f[0] = (g.ax3 * v1 + g.bx3 * f[2] + g.cx3 * f[3]) / g.dx3;
+    const long rows = (long)g.NY * g.NZ, nr = rows * nf;
+    if (t >= nr) {
+        if (TYPE == BC_STATE && a.inflow && t < nr + 2 * rows) {
+            const long u = t - nr, row = u % rows;
+            mg_inflow(g, a, s, eq, A, u < rows ? 1 : 0, (int)(row % g.NY), (int)(row / g.NY));
         }
-        f[L] = 0.;
-        f[L - 1] = 0.5 * f[L - 2];
+        return;
     }
-    if (TYPE == BC_STATE && a.inflow) {
-        mg_inflow(g, a, s, eq, A, 1, j, k);
-        mg_inflow(g, a, s, eq, A, 0, j, k);
+    const int m = (int)(t / rows);
+    const long row = t % rows;
+    const int j = (int)(row % g.NY), k = (int)(row / g.NY);
+    const int L = g.NX - 1;
+    double *f = A + IDX(g, 0, j, k) + m * g.S;
+    if (TYPE == BC_STATE) {
+        if (!a.inflow) {
+            f[0] = 0.;
+            f[1] = 0.5 * f[2];
+        }
+    } else {
+        const double v1 = (g.ax4 * f[2] + g.bx4 * f[3] + g.cx4 * f[4]) / g.dx4;
+        f[1] = v1;
+        f[0] = (g.ax3 * v1 + g.bx3 * f[2] + g.cx3 * f[3]) / g.dx3;
     }
+    f[L] = 0.;
+    f[L - 1] = 0.5 * f[L - 2];
 }
 
 // y rules (non-periodic, 0 / half) over every plane; x range all columns,
@@ -213,48 +220,44 @@ template <int TYPE>
 __global__ void km_fill_yz(Geo g, double *__restrict__ A, int nf)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ny = (long)g.NX * (g.NZ - 4);
+    const long ny = (long)g.NX * (g.NZ - 4), nz = (long)g.NX * g.NY;
     const long sy = g.sy, sz = g.sz;
-    if (t < ny) {
-        const int i = (int)(t % g.NX), k = 2 + (int)(t / g.NX);
+    if (t < ny * nf) {
+        const int m = (int)(t / ny);
+        const long u = t % ny;
+        const int i = (int)(u % g.NX), k = 2 + (int)(u / g.NX);
         if (TYPE == BC_TT && i < 2) return;
-        const long q0 = IDX(g, i, 0, k);
         const int L = g.NY - 1;
-        for (int m = 0; m < nf; m++) {
-            double *f = A + q0 + m * g.S;
-            f[0] = 0.;
-            f[sy] = 0.5 * f[2 * sy];
-            f[L * sy] = 0.;
-            f[(L - 1) * sy] = 0.5 * f[(L - 2) * sy];
-        }
+        double *f = A + IDX(g, i, 0, k) + m * g.S;
+        f[0] = 0.;
+        f[sy] = 0.5 * f[2 * sy];
+        f[L * sy] = 0.;
+        f[(L - 1) * sy] = 0.5 * f[(L - 2) * sy];
         return;
     }
-    const long u = t - ny;
-    if (u >= (long)g.NX * g.NY) return;
+    const long v = t - ny * nf;
+    if (v >= nz * nf) return;
+    const int m = (int)(v / nz);
+    const long u = v % nz;
     const int i = (int)(u % g.NX), j = (int)(u / g.NX);
     if (TYPE == BC_TT && i < 2) return;
-    const long q0 = IDX(g, i, j, 0);
     const int L = g.NZ - 1, M = g.NY - 1;
-    const bool outer = j == 0 || j == M, inner = j == 1 || j == M - 1;
-    const long qs = IDX(g, i, j == 1 ? 2 : M - 2, 0);
-    for (int m = 0; m < nf; m++) {
-        double *f = A + q0 + m * g.S;
-        double p2, pl;
-        if (outer) {
-            p2 = pl = 0.;
-        } else if (inner) {
-            const double *s = A + qs + m * g.S;
-            p2 = 0.5 * s[2 * sz];
-            pl = 0.5 * s[(L - 2) * sz];
-        } else {
-            p2 = f[2 * sz];
-            pl = f[(L - 2) * sz];
-        }
-        f[0] = 0.;
-        f[sz] = 0.5 * p2;
-        f[L * sz] = 0.;
-        f[(L - 1) * sz] = 0.5 * pl;
+    double *f = A + IDX(g, i, j, 0) + m * g.S;
+    double p2, pl;
+    if (j == 0 || j == M) {
+        p2 = pl = 0.;
+    } else if (j == 1 || j == M - 1) {
+        const double *s = A + IDX(g, i, j == 1 ? 2 : M - 2, 0) + m * g.S;
+        p2 = 0.5 * s[2 * sz];
+        pl = 0.5 * s[(L - 2) * sz];
+    } else {
+        p2 = f[2 * sz];
+        pl = f[(L - 2) * sz];
     }
+    f[0] = 0.;
+    f[sz] = 0.5 * p2;
+    f[L * sz] = 0.;
+    f[(L - 1) * sz] = 0.5 * pl;
 }
 
 // ---------------------------------------------------------------------------
