@@ -314,9 +314,9 @@ __global__ __launch_bounds__(TX * BY) void k_sweepz(Geo g, const double *__restr
 // brings plane t of level 0 (loaded ZPF planes ahead) and forms level p at
 // plane t-p, p = 1..NP:
 //   z neighbours  the thread's own register queue of level p-1,
-//   x neighbours  the neighbouring lanes,
-//   y neighbours  level p-1's plane in LDS (written one iteration earlier,
-//                 so one barrier per iteration suffices).
+//   x, y neighbours  level p-1's plane in LDS (written one iteration earlier,
+//                 so one barrier per iteration suffices; rows padded by a
+//                 column on either side).
 // Each pass costs the window one column and row at every edge that is not a
 // domain face, so a tile owns the inner (64-2NP) x (ZB-2NP) columns x rows.
 // Every level gets the boundary rules of its pass: x faces from columns 2,
