@@ -377,14 +377,11 @@ __global__ void km_rhs_a(Geo g, Mag a, const double *__restrict__ X, const doubl
             d[m] = -mg_dx(g, i, q, [&](long p) { return fx[p]; }) - mg_dy(g, j, q, [&](long p) { return fy[p]; }) -
                    mg_dz(g, k, q, [&](long p) { return fz[p]; });
         }
-        if (mg_masked(g, a.mrow, i, j, k)) {
-            XD[q] = XD[q + S] = XD[q + 2 * S] = XD[q + 3 * S] = 0.0;
-        } else {
-            XD[q] = d[0];
-            XD[q + S] = d[1] - xresd[q] + c2 * x7 - c3 * x6;
-            XD[q + 2 * S] = d[2] - xresd[q + S] + c3 * x5 - c1 * x7;
-            XD[q + 3 * S] = d[3] - xresd[q + 2 * S] + c1 * x6 - c2 * x5;
-        }
+        // (masked points: km_rhs_b, which follows, zeroes all eight fields)
+        XD[q] = d[0];
+        XD[q + S] = d[1] - xresd[q] + c2 * x7 - c3 * x6;
+        XD[q + 2 * S] = d[2] - xresd[q + S] + c3 * x5 - c1 * x7;
+        XD[q + 3 * S] = d[3] - xresd[q + 2 * S] + c1 * x6 - c2 * x5;
     }
     vel[q] = x2 / x1;
     vel[q + S] = x3 / x1;
@@ -447,8 +444,8 @@ __global__ void km_rhs_b(Geo g, Mag a, const double *__restrict__ X, const doubl
     long q;
     if (!mg_point(g, i, j, k, q) || !mg_inner(g, i, j, k)) return;
     const long S = g.S;
-    if (mg_masked(g, a.mrow, i, j, k)) {
-        XD[q + 4 * S] = XD[q + 5 * S] = XD[q + 6 * S] = XD[q + 7 * S] = 0.0;
+    if (mg_masked(g, a.mrow, i, j, k)) {   // the mask of the xdif fill, all eight fields
+        for (int m = 0; m < 8; m++) XD[q + m * S] = 0.0;
         return;
     }
     auto e1 = [&](long p) { return E[p]; };
