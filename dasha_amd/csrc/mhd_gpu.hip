@@ -1296,7 +1296,8 @@ template <int KIND, int TYPE>
 static double *mag_smooth(mhd_handle *h, double *a, double *b, int nf, int kk, const Stage &s)
 {
     for (int p = 0; p < kk; p++) {
-        hipLaunchKernelGGL((km_sweep<KIND>), vol_grid(h), dim3(256), 0, h->stream, h->g, a, b, nf, h->mrow);
+        hipLaunchKernelGGL((km_sweep<KIND>), vol_grid(h), dim3(256), 0, h->stream, h->g, a, b, nf, h->mrow,
+                           TYPE == BC_TT ? 1 : 0);
         mag_fill<TYPE>(h, b, nf, s);
         std::swap(a, b);
     }

@@ -14,8 +14,11 @@
 // host falls back to a mask launch per fill where they are not).
 // Non-periodic y only.
 // Every non-interior point of a buffer is rewritten by the fill that follows
-// its producer; producers write interior points only, sweeps carry the other
-// points of their input along.
+// its producer, and with the auxiliary and state fills every value formed from
+// a non-interior point is rewritten again by a later rule of the same fill, so
+// producers write interior points only; before a bndrytt fill, whose y and z
+// rules skip the x-left columns, sweeps carry the other points of their input
+// along, as the reference's in-place sweep leaves them.
 #pragma once
 
 #include "mhd_device.h"
@@ -263,18 +266,21 @@ __global__ void km_fill_yz(Geo g, double *__restrict__ A, int nf)
 // ---------------------------------------------------------------------------
 // one smoothing sweep of nf fields A -> B on the interior (t3dsdsc.f:5630-5680):
 // KIND_RADIAL, KIND_EIGHTH, or KIND_STATE (B components 4..6 take cbx).
-// The reference sweeps in place, so B takes A's other points: the x rules of
-// the fill that follows read face rows and planes before the y and z rules
-// rewrite them (bndrytt keeps those x-face values).
+// The reference sweeps in place; before a bndrytt fill (carry) B takes A's
+// other points, since its x rules read face rows and planes that its y and z
+// rules, which skip jx < 5, do not all rewrite.
 // ---------------------------------------------------------------------------
 template <int KIND>
-__global__ void km_sweep(Geo g, const double *__restrict__ A, double *__restrict__ B, int nf, const int *mrow)
+__global__ void km_sweep(Geo g, const double *__restrict__ A, double *__restrict__ B, int nf, const int *mrow,
+                         int carry)
 {
     int i, j, k;
     long q;
     if (!mg_point(g, i, j, k, q)) return;
     if (!mg_inner(g, i, j, k)) {
-        for (int m = 0; m < nf; m++) B[q + m * g.S] = A[q + m * g.S];
+        // only a bndrytt fill keeps values formed from non-interior points
+        if (carry)
+            for (int m = 0; m < nf; m++) B[q + m * g.S] = A[q + m * g.S];
         return;
     }
     const long sy = g.sy, sz = g.sz;
