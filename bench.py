@@ -253,6 +253,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble timing")
     ap.add_argument("--no-magnetosphere", action="store_true", help="skip the magnetosphere-mode timing")
     ap.add_argument("--ensemble-only", action="store_true", help="time the batched ensemble alone")
+    ap.add_argument("--magnetosphere-only", action="store_true",
+                    help="time the magnetosphere-mode step alone (the reference's own case)")
     ap.add_argument("--ensemble-runs", type=int, default=48)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=500)
@@ -264,6 +266,9 @@ def main():
 
     if args.ensemble_only:
         print(json.dumps({"ensemble": time_ensemble(10, args.ensemble_runs)}), flush=True)
+        return
+    if args.magnetosphere_only:
+        print(json.dumps({"magnetosphere": time_magnetosphere(args.steps or 50, args.warmup)}), flush=True)
         return
 
     from dasha_amd import harris

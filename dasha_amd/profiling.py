@@ -41,6 +41,16 @@ _FAMILIES = [
     (r"k_add_eq\b", "post_add_eq"),
     (r"k_dt_(partial|final)\b", "setdt"),
     (r"k_copy\b", "copy_probe"),
+    # magnetosphere mode (one sweep per launch, fills of two launches)
+    (r"km_sweep<0>", "mag_avrg_radial"),
+    (r"km_sweep<1>", "mag_avrgh"),
+    (r"km_sweep<2>", "mag_avrg_state"),
+    (r"km_fill_(x|y|z|yz)<|km_mask\b", "mag_fill"),
+    (r"km_curl\b", "mag_curl"),
+    (r"km_flux\b|km_rhs_a\b", "mag_rhs_flux_E"),
+    (r"km_hall\b|km_add3\b", "mag_hall"),
+    (r"km_rhs_b\b", "mag_rhs_curlE_pressure"),
+    (r"km_eq\b", "mag_post_eq"),
     # discrete adjoint (reverse sweep)
     (r"k_adj_filter<", "adj_sweep"),
     (r"k_adj_bc_[xz]\b", "adj_fill"),
@@ -54,7 +64,7 @@ _FAMILIES = [
 
 
 # families whose kernels are launched together for one call of the family
-_TOGETHER = {"rhs_curlE_pressure", "setdt"}
+_TOGETHER = {"rhs_curlE_pressure", "setdt", "mag_fill", "mag_rhs_flux_E"}
 
 
 def launch_group(kernel_name: str) -> str:

@@ -34,6 +34,12 @@ def test_kernel_families_and_launch_groups():
         "void mhd::k_adj_filter<2>(mhd::Geo)": "adj_sweep",
         "mhd::k_adj_bc_x(mhd::Geo)": "adj_fill",
         "k_energy_rows(mhd::Geo)": None,
+        "void mhd::km_sweep<0>(mhd::Geo, double const*, double*, int, int const*, int)": "mag_avrg_radial",
+        "void mhd::km_sweep<2>(mhd::Geo, double const*, double*, int, int const*, int)": "mag_avrg_state",
+        "void mhd::km_fill_yz<1>(mhd::Geo, double*, int)": "mag_fill",
+        "mhd::km_curl(mhd::Geo)": "mag_curl",
+        "mhd::km_rhs_a(mhd::Geo)": "mag_rhs_flux_E",
+        "mhd::km_rhs_b(mhd::Geo)": "mag_rhs_curlE_pressure",
     }
     for name, f in fam.items():
         assert profiling.family(name) == f, name
