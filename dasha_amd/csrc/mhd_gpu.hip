@@ -1277,6 +1277,17 @@ static void mag_fill(mhd_handle *h, double *A, int nf, const Stage &s)
 {
     const Mag a = mag_args(h);
     const Geo &g = h->g;
+    if constexpr (TYPE != BC_TT) {
+        // the whole fill in one launch (the mask is the producer's)
+        if (!h->fill_split && (h->mrow || !a.mask)) {
+            const long rows = (long)g.NY * g.NZ;
+            const long n = rows * nf + (TYPE == BC_STATE && a.inflow ? 2 * rows : 0) +
+                           ((long)g.NX * (g.NZ - 4) + (long)g.NX * g.NY) * nf;
+            hipLaunchKernelGGL((km_fill_xyz<TYPE>), dim3(blocks(n, 256)), dim3(256), 0, h->stream, g, a, s, h->eq,
+                               A, nf);
+            return;
+        }
+    }
     if (a.mask && !h->mrow) hipLaunchKernelGGL(km_mask, vol_grid(h), dim3(256), 0, h->stream, g, a, A, nf);
     const long rows = (long)g.NY * g.NZ;
     const long nx = rows * nf + (TYPE == BC_STATE && a.inflow ? 2 * rows : 0);

@@ -115,24 +115,35 @@ __global__ void km_mask(Geo g, Mag a, double *__restrict__ A, int nf)
 
 // driven inflow of bndry at column i (0: jx = 3, 1: jx = 4), row j, plane k,
 // t3dsdsc.f:1714-1772 (density base rhoi at the point itself)
+// the driven value of field m at (i, j, k)
+__device__ inline double mg_inflow_v(const Geo &g, const Mag &a, const Stage &s, const double *__restrict__ eq,
+                                     int m, int j, int k, long q)
+{
+    const long S = g.S, f = (long)k * g.NY + j, F = (long)g.NZ * g.NY;
+    if (m == 0 || m == 1) {
+        const double v0 = eq[q] + 1. * 0.5 * s.tt * 0.5 * a.zd[k] * 0.5 * a.yd[j];
+        return m == 0 ? v0 : MF32(0.8) * v0 * 0.5 * s.tt * 0.5 * a.zd2[k] * 0.5 * a.yd2[j];
+    }
+    if (m <= 3) return 0.;
+    if (m <= 6) return eq[q + (m - 3) * S] + divc(a.fb[f + (m - 4) * F] * MF32(0.0017) * s.sq, s.de, s.rde);
+    return eq[q + 4 * S] + MF32(0.2) * 0.5 * s.tt * 0.5 * a.zd[k] * 0.5 * a.yd[j];
+}
+
+// its xdif: (x_bc - x_ref) / (ddt + 1e-9)
+__device__ inline double mg_inflow_xd(const Geo &g, const Mag &a, const Stage &s, const double *__restrict__ eq,
+                                      int m, int j, int k, long q)
+{
+    return divc(mg_inflow_v(g, a, s, eq, m, j, k, q) - s.ref[q + m * g.S], s.den, s.rden);
+}
+
 __device__ inline void mg_inflow(const Geo &g, const Mag &a, const Stage &s, const double *__restrict__ eq,
                                  double *__restrict__ XD, int i, int j, int k)
 {
     const long q = IDX(g, i, j, k), S = g.S;
-    const long f = (long)k * g.NY + j, F = (long)g.NZ * g.NY;
-    const double zd = a.zd[k], yd = a.yd[j], zd2 = a.zd2[k], yd2 = a.yd2[j];
-    double v[8];
-    v[0] = eq[q] + 1. * 0.5 * s.tt * 0.5 * zd * 0.5 * yd;
-    v[1] = MF32(0.8) * v[0] * 0.5 * s.tt * 0.5 * zd2 * 0.5 * yd2;
-    v[2] = 0.;
-    v[3] = 0.;
-    v[4] = eq[q + S] + divc(a.fb[f] * MF32(0.0017) * s.sq, s.de, s.rde);
-    v[5] = eq[q + 2 * S] + divc(a.fb[f + F] * MF32(0.0017) * s.sq, s.de, s.rde);
-    v[6] = eq[q + 3 * S] + divc(a.fb[f + 2 * F] * MF32(0.0017) * s.sq, s.de, s.rde);
-    v[7] = eq[q + 4 * S] + MF32(0.2) * 0.5 * s.tt * 0.5 * zd * 0.5 * yd;
     for (int m = 0; m < 8; m++) {
-        s.x[q + m * S] = v[m];
-        XD[q + m * S] = divc(v[m] - s.ref[q + m * S], s.den, s.rden);
+        const double v = mg_inflow_v(g, a, s, eq, m, j, k, q);
+        s.x[q + m * S] = v;
+        XD[q + m * S] = divc(v - s.ref[q + m * S], s.den, s.rden);
     }
 }
 
@@ -261,6 +272,113 @@ __global__ void km_fill_yz(Geo g, double *__restrict__ A, int nf)
     f[sz] = 0.5 * p2;
     f[L * sz] = 0.;
     f[(L - 1) * sz] = 0.5 * pl;
+}
+
+// the whole auxiliary or state fill (x, then y, then z rules; not bndrytt) in
+// one launch.  Threads over the rows take the x rules of the interior rows
+// and planes (and, with the inflow, the driven state of every row), threads
+// over the x-z columns the y rules on the interior planes, threads over the
+// x-y columns the z rules.  A thread that needs a value another rule of this
+// fill forms -- an x-face column of row 2 or plane 2, say -- forms it itself
+// with the same arithmetic, so no thread reads what another writes and the
+// values are those of the x, y and z launches in turn.
+template <int TYPE>
+__device__ inline double mg_xrule(const Geo &g, const Mag &a, const Stage &s, const double *__restrict__ eq,
+                                  const double *__restrict__ f, int m, int i, int j, int k)
+{
+    // f: field m at (0, j, k); i an x-face column (0, 1, NX-2, NX-1)
+    const int L = g.NX - 1;
+    if (i == L) return 0.;
+    if (i == L - 1) return 0.5 * f[L - 2];
+    if (TYPE == BC_STATE) {
+        if (a.inflow) return mg_inflow_xd(g, a, s, eq, m, j, k, IDX(g, i, j, k));
+        return i == 0 ? 0. : 0.5 * f[2];
+    }
+    const double v1 = (g.ax4 * f[2] + g.bx4 * f[3] + g.cx4 * f[4]) / g.dx4;
+    return i == 1 ? v1 : (g.ax3 * v1 + g.bx3 * f[2] + g.cx3 * f[3]) / g.dx3;
+}
+
+template <int TYPE>
+__global__ void km_fill_xyz(Geo g, Mag a, Stage s, const double *__restrict__ eq, double *__restrict__ A, int nf)
+{
+    static_assert(TYPE == BC_AUX || TYPE == BC_STATE, "bndrytt keeps its launches");
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long S = g.S, sy = g.sy, sz = g.sz;
+    const int L = g.NX - 1, M = g.NY - 1, N = g.NZ - 1;
+    const long rows = (long)g.NY * g.NZ;
+    auto xface = [&](int i) { return i < 2 || i > L - 2; };
+    // value of field m at (i, j, k) once the x rules have run (j, k interior)
+    auto after_x = [&](int m, int i, int j, int k) {
+        const double *f = A + IDX(g, 0, j, k) + m * S;
+        return xface(i) ? mg_xrule<TYPE>(g, a, s, eq, f, m, i, j, k) : f[i];
+    };
+    // ... and the y rules (k interior)
+    auto after_y = [&](int m, int i, int j, int k) {
+        if (j == 0 || j == M) return 0.;
+        if (j == 1 || j == M - 1) return 0.5 * after_x(m, i, j == 1 ? 2 : M - 2, k);
+        return after_x(m, i, j, k);
+    };
+    // x rules: interior rows and planes, one thread per row and field
+    if (t < rows * nf) {
+        const int m = (int)(t / rows);
+        const long row = t % rows;
+        const int j = (int)(row % g.NY), k = (int)(row / g.NY);
+        if (j < 2 || j > M - 2 || k < 2 || k > N - 2) return;
+        double *f = A + IDX(g, 0, j, k) + m * S;
+        const bool driven = TYPE == BC_STATE && a.inflow;
+        double v[4];
+        #pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int i = c < 2 ? c : L - 3 + c;
+            v[c] = driven && i < 2 ? 0. : mg_xrule<TYPE>(g, a, s, eq, f, m, i, j, k);
+        }
+        if (!driven) { f[0] = v[0]; f[1] = v[1]; }
+        f[L - 1] = v[2];
+        f[L] = v[3];
+        return;
+    }
+    t -= rows * nf;
+    // the driven columns: the state on every row, xdif on the interior ones
+    if (TYPE == BC_STATE && a.inflow) {
+        if (t < 2 * rows) {
+            const long row = t % rows;
+            const int i = t < rows ? 1 : 0, j = (int)(row % g.NY), k = (int)(row / g.NY);
+            const long q = IDX(g, i, j, k);
+            const bool in = j >= 2 && j <= M - 2 && k >= 2 && k <= N - 2;
+            for (int m = 0; m < 8; m++) {
+                const double v = mg_inflow_v(g, a, s, eq, m, j, k, q);
+                s.x[q + m * S] = v;
+                if (in) A[q + m * S] = divc(v - s.ref[q + m * S], s.den, s.rden);
+            }
+            return;
+        }
+        t -= 2 * rows;
+    }
+    // y rules on the interior planes, one thread per x-z column and field
+    const long ny = (long)g.NX * (g.NZ - 4);
+    if (t < ny * nf) {
+        const int m = (int)(t / ny);
+        const long u = t % ny;
+        const int i = (int)(u % g.NX), k = 2 + (int)(u / g.NX);
+        double *f = A + IDX(g, i, 0, k) + m * S;
+        f[0] = 0.;
+        f[sy] = 0.5 * after_x(m, i, 2, k);
+        f[M * sy] = 0.;
+        f[(M - 1) * sy] = 0.5 * after_x(m, i, M - 2, k);
+        return;
+    }
+    t -= ny * nf;
+    // z rules, one thread per x-y column and field
+    const long nz = (long)g.NX * g.NY;
+    if (t >= nz * nf) return;
+    const int m = (int)(t / nz);
+    const long u = t % nz;
+    const int i = (int)(u % g.NX), j = (int)(u / g.NX);
+    double *f = A + IDX(g, i, j, 0) + m * S;
+    f[0] = 0.;
+    f[sz] = 0.5 * after_y(m, i, j, 2);
+    f[N * sz] = 0.;
+    f[(N - 1) * sz] = 0.5 * after_y(m, i, j, N - 2);
 }
 
 // ---------------------------------------------------------------------------

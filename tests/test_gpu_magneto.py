@@ -123,11 +123,12 @@ def test_magnetosphere_mode_refusals():
         MHDStep(bad)
 
 
-@pytest.mark.parametrize("name", ["defaults", "partial_e"])
+@pytest.mark.parametrize("name", sorted(CASES))
 def test_fill_forms_agree(name, monkeypatch):
-    """the fills with the mask applied by the producers and the y and z rules
-    in one launch give the same state, bit for bit, as a mask launch per fill
-    and separate y and z launches (the reference's routine order)"""
+    """the fills with the mask applied by the producers and the x, y and z
+    rules in one launch (two for bndrytt) give the same state, bit for bit,
+    as a mask launch per fill and separate x, y and z launches (the
+    reference's routine order)"""
     from dasha_amd.mhd import MHDStep
     states = []
     for mask, split in (("1", "0"), ("0", "1")):
